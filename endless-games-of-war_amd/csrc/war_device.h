@@ -1,0 +1,203 @@
+// Device-side War arithmetic for gfx950: packed piles, one turn, permutations, position builders.
+//
+// Piles are packed W bits per card, top card in the lowest W bits (gpu/war_gpu.py:38-45,
+// cpu/war.cpp:17-23):
+//   W = 4: one u64 per pile, n <= 16 (a full pile of 16 cards is 64 bits);
+//   W = 5: one u128 per pile, 17 <= n <= 20 (gpu/war24.py:31-47 uses 5-bit cards in two words).
+// The state keeps one length (player 1's, as a bit offset); player 2's follows from n. A turn is
+// branch-free: each pile ORs in a two-card value that is zero for the turn's loser.
+//
+// Permutations of m <= 16 cards are nibble-packed in a u64 (card i of the order in nibble i);
+// for m <= 20 they are 5-bit packed in a u128. No per-lane arrays are used anywhere, so nothing
+// spills to scratch.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+typedef unsigned long long u64;
+typedef unsigned int u32;
+typedef unsigned __int128 u128;
+
+namespace war {
+
+// 0! .. 20! (20! < 2^63, so every deal rank of n <= 20 fits a u64).
+__constant__ u64 c_fact[21] = {
+    1ull, 1ull, 2ull, 6ull, 24ull, 120ull, 720ull, 5040ull, 40320ull, 362880ull, 3628800ull,
+    39916800ull, 479001600ull, 6227020800ull, 87178291200ull, 1307674368000ull,
+    20922789888000ull, 355687428096000ull, 6402373705728000ull, 121645100408832000ull,
+    2432902008176640000ull};
+
+template <int W> struct Lay;
+template <> struct Lay<4> { typedef u64 T; };
+template <> struct Lay<5> { typedef u128 T; };
+
+// sa = W * (player 1's cards - 1): the bit index of player 1's bottom card. Player 1 wins a turn:
+// the two cards go to bit sa (after the pop), then sa += W. Player 2 wins: they go to bit
+// C - sa with C = B - 2W, then sa -= W. The game is over iff sa > C as an unsigned number
+// (player 1 empty: sa = -W; player 2 empty: sa = B - W).
+template <int W> struct St {
+    typename Lay<W>::T a, b;   // player 1, player 2
+    u32 sa;
+};
+
+// x >> s and x << s that give 0 for s >= bit width (the hardware masks the shift count).
+template <class T> __device__ __forceinline__ T shr_(T x, u32 s) { return s >= 8 * sizeof(T) ? (T)0 : x >> s; }
+template <class T> __device__ __forceinline__ T shl_(T x, u32 s) { return s >= 8 * sizeof(T) ? (T)0 : x << s; }
+template <class T> __device__ __forceinline__ T lowmask_(u32 s) { return s >= 8 * sizeof(T) ? ~(T)0 : (((T)1 << s) - (T)1); }
+
+template <int W> __device__ __forceinline__ u32 bits1(const St<W> &s) { return s.sa + W; }   // W * cards of player 1
+
+// One turn (gpu/war_gpu.py:38-45). Rule 0: the winner puts the winning card, then the losing
+// card, under its pile; rule 1 reverses the two. C = W * (n - 2). Returns true if player 1 won.
+// Branch-free: each pile ORs in its own two-card value (zero for the loser) at its own offset.
+template <int W, int RULE>
+__device__ __forceinline__ bool turn(St<W> &s, u32 C) {
+    typedef typename Lay<W>::T T;
+    constexpr u32 M = (1u << W) - 1u;
+    const u32 c1 = (u32)s.a & M, c2 = (u32)s.b & M;
+    s.a >>= W;
+    s.b >>= W;
+    const bool p1 = c1 > c2;
+    const u32 v1 = RULE ? (c2 | (c1 << W)) : (c1 | (c2 << W));   // player 1 won: its card pair
+    const u32 v2 = RULE ? (c1 | (c2 << W)) : (c2 | (c1 << W));   // player 2 won
+    const u32 x1 = p1 ? v1 : 0u, x2 = p1 ? 0u : v2;
+    // the masks only matter for a finished lane (whose state is no longer read); for u64 they are
+    // free, the hardware shift already takes the count modulo 64
+    constexpr u32 SM = 8 * sizeof(T) - 1;
+    s.a |= (T)x1 << (s.sa & SM);
+    s.b |= (T)x2 << ((C - s.sa) & SM);
+    s.sa += p1 ? (u32)W : (u32)(-W);
+    return p1;
+}
+
+template <int W> __device__ __forceinline__ bool ended(const St<W> &s, u32 C) { return s.sa > C; }
+
+// The repeat test compares (w1, w2, l1): card 0 and an empty slot look alike (gpu/war_gpu.py:91).
+template <int W> __device__ __forceinline__ bool same(const St<W> &x, const St<W> &y) {
+    return (x.a == y.a) & (x.b == y.b) & (x.sa == y.sa);
+}
+
+template <int W> __device__ __forceinline__ u32 card(typename Lay<W>::T q, int i) {
+    return (u32)(q >> (W * i)) & ((1u << W) - 1u);
+}
+
+// Lexicographic unrank (gpu/war_gpu.py:135-146): digit i = r / (m-1-i)! picks the digit-th lowest
+// unused card. The unused cards are kept as an ascending packed list; picking one removes its
+// field. The first digits of a large rank need a 64-bit division; once r < 2^32 and the factorial
+// fits 32 bits the division is 32-bit.
+template <int W>
+__device__ __forceinline__ typename Lay<W>::T unrank(u64 r, int m) {
+    typedef typename Lay<W>::T T;
+    T avail = 0;
+    for (int i = 0; i < m; i++) avail |= (T)i << (W * i);
+    T q = 0;
+    for (int i = 0; i < m; i++) {
+        const u64 f = c_fact[m - 1 - i];
+        u32 d;
+        if ((r >> 32) == 0 && (f >> 32) == 0) d = (u32)r / (u32)f;
+        else d = (u32)(r / f);
+        r -= (u64)d * f;
+        const u32 sd = W * d;
+        const u32 c = (u32)(avail >> sd) & ((1u << W) - 1u);
+        avail = (avail & lowmask_<T>(sd)) | shl_<T>(shr_<T>(avail, sd + W), sd);
+        q |= (T)c << (W * i);
+    }
+    return q;
+}
+
+// Reverse the order of the W-bit fields of a whole word.
+__device__ __forceinline__ u64 rev_fields4(u64 x) {
+    x = __builtin_bswap64(x);
+    return ((x >> 4) & 0x0F0F0F0F0F0F0F0Full) | ((x & 0x0F0F0F0F0F0F0F0Full) << 4);
+}
+
+// Next lexicographic permutation of the m packed cards (gpu/war_gpu.py:109-119). The caller never
+// asks for the successor of the last permutation.
+template <int W>
+__device__ __forceinline__ typename Lay<W>::T next_perm(typename Lay<W>::T q, int m) {
+    typedef typename Lay<W>::T T;
+    constexpr u32 M = (1u << W) - 1u;
+    int i = m - 2;
+    u32 right = card<W>(q, m - 1);
+    u32 ai = card<W>(q, i);
+    while (i > 0 && ai >= right) { right = ai; i--; ai = card<W>(q, i); }
+    // here a[i] < a[i+1] (a[0] >= a[1] only for the last permutation, never passed in)
+    int j = m - 1;
+    u32 aj = card<W>(q, j);
+    while (aj <= ai) { j--; aj = card<W>(q, j); }
+    // swap a[i], a[j]
+    q &= ~(((T)M << (W * i)) | ((T)M << (W * j)));
+    q |= ((T)aj << (W * i)) | ((T)ai << (W * j));
+    // reverse a[i+1 .. m-1]
+    const int L = m - 1 - i;
+    if (L > 1) {
+        T rs = 0;
+        if constexpr (W == 4) {
+            rs = rev_fields4(q) >> (4 * (16 - m));      // a[m-1] now in field 0
+        } else {
+            for (int t = 0; t < L; t++) rs |= (T)card<W>(q, m - 1 - t) << (W * t);
+        }
+        rs &= lowmask_<T>(W * L);
+        q = (q & lowmask_<T>(W * (i + 1))) | (rs << (W * (i + 1)));
+    }
+    return q;
+}
+
+// Deal (gpu/war_gpu.py:98-107): deck positions 0,2,4,.. to player 1, 1,3,5,.. to player 2,
+// first dealt on top.
+__device__ __forceinline__ u64 even_fields4(u64 x) {
+    x &= 0x0F0F0F0F0F0F0F0Full;
+    x = (x | (x >> 4)) & 0x00FF00FF00FF00FFull;
+    x = (x | (x >> 8)) & 0x0000FFFF0000FFFFull;
+    x = (x | (x >> 16)) & 0x00000000FFFFFFFFull;
+    return x;
+}
+
+template <int W>
+__device__ __forceinline__ St<W> deal(typename Lay<W>::T q, int n) {
+    typedef typename Lay<W>::T T;
+    St<W> s;
+    if constexpr (W == 4) {
+        s.a = even_fields4(q);
+        s.b = even_fields4(q >> 4);
+    } else {
+        s.a = 0; s.b = 0;
+        for (int i = 0; i < n; i++) {
+            const T c = (T)card<W>(q, i);
+            if ((i & 1) == 0) s.a |= c << (W * (i >> 1));
+            else              s.b |= c << (W * (i >> 1));
+        }
+    }
+    s.sa = W * (u32)((n + 1) >> 1) - W;
+    return s;
+}
+
+// Canonical position (gpu/war_gpu.py:210-213): player 1 = [n-1, q0..q(k-1)], player 2 = [qk..q(n-2)].
+template <int W>
+__device__ __forceinline__ St<W> canon(typename Lay<W>::T q, int n, int k) {
+    typedef typename Lay<W>::T T;
+    St<W> s;
+    const u32 sk = W * (u32)k;
+    s.a = (T)(u32)(n - 1) | ((q & lowmask_<T>(sk)) << W);
+    s.b = shr_<T>(q, sk);
+    s.sa = W * (u32)k;
+    return s;
+}
+
+// --half filter (gpu/war_gpu.py:151-154): true if card n-1 sits at an even deck position.
+template <int W>
+__device__ __forceinline__ bool top_card_to_p1(typename Lay<W>::T q, int n) {
+    int pos = 0;
+    for (int i = 0; i < n; i++) if (card<W>(q, i) == (u32)(n - 1)) pos = i;
+    return (pos & 1) == 0;
+}
+
+// Key of a 4-bit state for distinct cycles (cpu/distinct_cycles.c:24-31): player 1's length in
+// bits 60..63, then player 1's cards and player 2's cards, top first, from bit 56 downwards.
+// bl = 4 * (player 1's cards).
+__device__ __forceinline__ u64 key4(u64 a, u64 b, u32 bl) {
+    const u64 c = a | shl_<u64>(b, bl);           // n <= 15 cards, top of player 1 in nibble 0
+    return ((u64)(bl >> 2) << 60) | (rev_fields4(c) >> 4);
+}
+
+}  // namespace war

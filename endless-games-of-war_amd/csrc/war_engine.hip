@@ -1,0 +1,822 @@
+// Host engine and C ABI of libwar_engine.so (declared in include/war_engine.h).
+//
+// One engine per process, bound to one GPU. All GPU work of the engine runs on one HIP stream,
+// guarded by one mutex; the serving loop is a host thread that takes the same mutex per step.
+//
+// A persistent batch is one hipGraph: k_setup, the FRESH stage, the queue stages and (if any
+// request of the batch needs them) the cycle walks. Graphs are captured once per launch shape and
+// replayed; the job list they read lives in device memory and is copied in before each replay.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <tuple>
+#include <vector>
+
+#include "war_engine.h"
+#include "war_kernels.h"
+
+using namespace war;
+
+namespace {
+
+thread_local std::string t_err;
+std::string g_err_shared;
+std::mutex g_err_mu;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    t_err = buf;
+    std::lock_guard<std::mutex> g(g_err_mu);
+    g_err_shared = buf;
+    return code;
+}
+
+#define HIPCHK(call)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess) return fail(100 + (int)e_, "%s: %s (%s:%d)", #call,          \
+                                          hipGetErrorString(e_), __FILE__, __LINE__);        \
+    } while (0)
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+u64 fact(int k) { u64 f = 1; for (int i = 2; i <= k; i++) f *= (u64)i; return f; }
+u64 range_total(int n, int mode) {
+    if (n < 1) return 0;
+    if (mode == WAR_MODE_CANON) return n < 2 ? 0 : fact(n - 1) * (u64)(n - 1);
+    return fact(n);
+}
+
+constexpr int MAXSLOTS = 256;
+constexpr int MAXJOBS = 1024;
+
+// Per request: device counters and histograms (resident until the result is copied out).
+struct ReqHost {
+    war_request rq{};
+    war_result *res = nullptr;
+    int slot = -1;
+    u64 *dmem = nullptr;            // one allocation: counters | hist | lenhist | dhist | htab
+    ReqDev dev{};
+    u64 next = 0;                   // next position to schedule
+    uint64_t id = 0;
+    double t_submit = 0, t_done = 0, dev_ms = 0;
+    bool finished = false;
+    int status = 0;
+    std::string err;
+    size_t bytes = 0;
+};
+
+struct QueueBufs {              // device buffers of one pile layout
+    void *qa[2] = {nullptr, nullptr}, *qb[2] = {nullptr, nullptr};
+    u32 *qreq[2] = {nullptr, nullptr};
+    u64 *qpos[2] = {nullptr, nullptr};
+    void *ca = nullptr, *cb = nullptr;
+    u32 *cmeta = nullptr, *creq = nullptr, *clam = nullptr;
+    u64 *cpos = nullptr;
+    u64 cap = 0, ccap = 0;
+};
+
+struct GraphKey {
+    int W, rule, kernel, wpc, per, cap0, walk, multi, hb, lb;
+    bool operator<(const GraphKey &o) const {
+        return std::tie(W, rule, kernel, wpc, per, cap0, walk, multi, hb, lb) <
+               std::tie(o.W, o.rule, o.kernel, o.wpc, o.per, o.cap0, o.walk, o.multi, o.hb, o.lb);
+    }
+};
+
+struct Engine {
+    int device = 0;
+    bool verbose = false;
+    hipStream_t st = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int cus = 0, clock_mhz = 0;
+    std::string arch;
+    war_tuning tun{16, 1024, 15, 32};
+    u64 batch = 1ull << 28;          // positions per persistent batch (= queue capacity)
+    Pipe *pipe = nullptr;
+    JobHdr *hdr = nullptr;           // device: JobHdr followed by MAXJOBS Jobs
+    Job *jobs = nullptr;
+    void *stage_host = nullptr;      // pinned staging for the job list
+    ReqDev *reqs = nullptr;          // device table of MAXSLOTS
+    QueueBufs qb4, qb5;
+    std::map<GraphKey, hipGraphExec_t> graphs;
+    std::vector<int> free_slots;
+    std::mutex mu;                   // serialises all GPU work of the engine
+
+    // serving
+    std::thread worker;
+    std::mutex smu;
+    std::condition_variable scv;
+    std::deque<std::shared_ptr<ReqHost>> pending;
+    std::map<uint64_t, std::shared_ptr<ReqHost>> all;
+    std::atomic<bool> serving{false};
+    int serve_slots = 64;
+    u64 step_positions = 1ull << 26;
+    uint64_t next_id = 1;
+};
+
+Engine *g_eng = nullptr;
+
+int check_request(const war_request *r) {
+    if (!r) return fail(1, "null request");
+    if (r->n < 1 || r->n > 20) return fail(2, "n must be in [1, 20] (got %d)", r->n);
+    if (r->mode < 0 || r->mode > 2) return fail(3, "mode must be FULL, HALF or CANON");
+    if (r->rule != 0 && r->rule != 1) return fail(4, "rule must be 0 or 1");
+    if (r->kernel != WAR_KERNEL_PLAIN && r->kernel != WAR_KERNEL_PERSISTENT) return fail(5, "unknown kernel");
+    const u64 total = range_total(r->n, r->mode);
+    if (r->lo > r->hi || r->hi > total) return fail(6, "range [%llu, %llu) outside [0, %llu)", r->lo, r->hi, total);
+    if (r->mode == WAR_MODE_HALF && (r->n & 1)) return fail(7, "half mode needs even n");
+    if (r->hbins < 0 || r->hbins > 4096 || r->lbins < 0 || r->lbins > 4096 || r->dbins < 0 || r->dbins > 4096)
+        return fail(8, "bins must be in [0, 4096]");
+    if (r->hbins + r->lbins > 8192) return fail(8, "hbins + lbins must be <= 8192");
+    if ((r->distinct || r->dbins) && (r->mode != WAR_MODE_CANON || r->n > 15))
+        return fail(9, "distinct / dbins need canon mode and n <= 15");
+    if (r->shape_blk && r->mode != WAR_MODE_CANON) return fail(10, "shape needs canon mode");
+    if (r->shape_blk < 0 || r->shape_blk > 512) return fail(10, "shape_blk must be in [0, 512]");
+    return 0;
+}
+
+void resolve_tuning(const Engine &E, const war_tuning &in, war_tuning &t) {
+    t = E.tun;
+    if (in.waves_per_cu > 0) t.waves_per_cu = in.waves_per_cu;
+    if (in.ranks_per_refill > 0) t.ranks_per_refill = in.ranks_per_refill;
+    if (in.brent_cap > 0) t.brent_cap = in.brent_cap;
+    if (in.plain_per > 0) t.plain_per = in.plain_per;
+}
+
+int check_tuning(const war_tuning &t) {
+    if (t.waves_per_cu < 1 || t.waves_per_cu > 32) return fail(11, "waves_per_cu must be in [1, 32]");
+    if (t.ranks_per_refill < 64 || t.ranks_per_refill > 8192 || (t.ranks_per_refill & 63))
+        return fail(11, "ranks_per_refill must be a multiple of 64 in [64, 8192]");
+    const int c = t.brent_cap;
+    if (c < 1 || c > 65535 || ((c + 1) & c)) return fail(11, "brent_cap must be 2^k - 1 in [1, 65535]");
+    if (t.plain_per < 1 || t.plain_per > 4096) return fail(11, "plain_per must be in [1, 4096]");
+    return 0;
+}
+
+// ---- per-request device memory
+int req_alloc(Engine &E, ReqHost &h) {
+    const war_request &r = h.rq;
+    const size_t nc = C_N, nh = (size_t)r.hbins, nl = (size_t)r.lbins, nd = (size_t)r.dbins;
+    u32 hlog2 = 0;
+    size_t nt = 0;
+    if (r.distinct) {
+        // table sized for the cycles a range can touch: at most the on-cycle positions of n
+        hlog2 = r.n <= 13 ? 21 : (r.n == 14 ? 17 : 27);
+        nt = (size_t)1 << hlog2;
+    }
+    h.bytes = (nc + nh + nl + nd + nt) * sizeof(u64);
+    void *p = nullptr;
+    HIPCHK(hipMalloc(&p, h.bytes));
+    HIPCHK(hipMemsetAsync(p, 0, h.bytes, E.st));
+    h.dmem = (u64 *)p;
+    ReqDev &d = h.dev;
+    d = ReqDev{};
+    d.n = (u32)r.n; d.mode = (u32)r.mode; d.rule = (u32)r.rule;
+    d.hbins = (u32)r.hbins; d.lbins = (u32)r.lbins; d.dbins = (u32)r.dbins;
+    d.distinct = (u32)r.distinct; d.shape_blk = (u32)r.shape_blk;
+    d.walk = (r.mode == WAR_MODE_CANON && (r.distinct || r.dbins || r.shape_blk)) ? 1u : 0u;
+    d.hlog2 = hlog2;
+    d.cnt = h.dmem;
+    d.hist = d.cnt + nc;
+    d.lenhist = d.hist + nh;
+    d.dhist = d.lenhist + nl;
+    d.htab = nt ? d.dhist + nd : nullptr;
+    HIPCHK(hipMemcpyAsync(&E.reqs[h.slot], &d, sizeof(ReqDev), hipMemcpyHostToDevice, E.st));
+    return 0;
+}
+
+int req_collect(Engine &E, ReqHost &h) {
+    const war_request &r = h.rq;
+    std::vector<u64> buf(C_N + (size_t)r.hbins + (size_t)r.lbins + (size_t)r.dbins);
+    HIPCHK(hipMemcpyAsync(buf.data(), h.dmem, buf.size() * sizeof(u64), hipMemcpyDeviceToHost, E.st));
+    HIPCHK(hipStreamSynchronize(E.st));
+    war_result *o = h.res;
+    if (o) {
+        o->cyc = buf[C_CYC]; o->done = buf[C_DONE]; o->maxfin = buf[C_MAXFIN]; o->turns = buf[C_TURNS];
+        o->oncycle = buf[C_ONCYCLE]; o->distinct_minkey = buf[C_MINKEY]; o->distinct_hashset = buf[C_HASH];
+        o->shape_nonstd = buf[C_NONSTD]; o->shape_unchk = buf[C_UNCHK]; o->undetermined = buf[C_UNDET];
+        const u64 *hp = buf.data() + C_N, *lp = hp + r.hbins, *dp = lp + r.lbins;
+        if (o->hist) for (int i = 0; i < std::min(o->hbins, r.hbins); i++) o->hist[i] = hp[i];
+        if (o->lenhist) for (int i = 0; i < std::min(o->lbins, r.lbins); i++) o->lenhist[i] = lp[i];
+        if (o->dhist) for (int i = 0; i < std::min(o->dbins, r.dbins); i++) o->dhist[i] = dp[i];
+        o->device_ms = h.dev_ms;
+    }
+    if (buf[C_HOVF]) return fail(20, "distinct hash set overflowed (%llu keys dropped)", buf[C_HOVF]);
+    if (buf[C_UNDET]) return fail(21, "%llu games still running after 2^24 turns", buf[C_UNDET]);
+    const u64 want = r.hi - r.lo;
+    if (r.mode != WAR_MODE_HALF && buf[C_DONE] != want)
+        return fail(22, "played %llu positions, expected %llu", buf[C_DONE], want);
+    return 0;
+}
+
+void req_free(Engine &E, ReqHost &h) {
+    if (h.dmem) { (void)hipFree(h.dmem); h.dmem = nullptr; }
+    if (h.slot >= 0) { E.free_slots.push_back(h.slot); h.slot = -1; }
+}
+
+// ---- queue buffers of one layout, allocated on first use
+template <int W> QueueBufs &qbufs(Engine &E) { return W == 4 ? E.qb4 : E.qb5; }
+
+template <int W> int ensure_queues(Engine &E, bool walk) {
+    typedef typename Lay<W>::T T;
+    QueueBufs &Q = qbufs<W>(E);
+    if (!Q.cap) {
+        for (int i = 0; i < 2; i++) {
+            HIPCHK(hipMalloc(&Q.qa[i], E.batch * sizeof(T)));
+            HIPCHK(hipMalloc(&Q.qb[i], E.batch * sizeof(T)));
+            HIPCHK(hipMalloc((void **)&Q.qreq[i], E.batch * sizeof(u32)));
+            HIPCHK(hipMalloc((void **)&Q.qpos[i], E.batch * sizeof(u64)));
+        }
+        Q.cap = E.batch;
+    }
+    if (walk && !Q.ccap) {
+        HIPCHK(hipMalloc(&Q.ca, E.batch * sizeof(T)));
+        HIPCHK(hipMalloc(&Q.cb, E.batch * sizeof(T)));
+        HIPCHK(hipMalloc((void **)&Q.cmeta, E.batch * sizeof(u32)));
+        HIPCHK(hipMalloc((void **)&Q.creq, E.batch * sizeof(u32)));
+        HIPCHK(hipMalloc((void **)&Q.clam, E.batch * sizeof(u32)));
+        HIPCHK(hipMalloc((void **)&Q.cpos, E.batch * sizeof(u64)));
+        Q.ccap = E.batch;
+    }
+    return 0;
+}
+
+template <int W> CycQueue<W> cyc_queue(Engine &E) {
+    typedef typename Lay<W>::T T;
+    QueueBufs &Q = qbufs<W>(E);
+    CycQueue<W> c;
+    c.a = (T *)Q.ca; c.b = (T *)Q.cb; c.meta = Q.cmeta; c.req = Q.creq; c.lam = Q.clam; c.pos = Q.cpos;
+    return c;
+}
+
+int round_pow2(int x) { int p = 0; if (x <= 0) return 0; p = 1; while (p < x) p <<= 1; return p; }
+
+// Stage boundaries: cap0, 2 cap0 + 1, ... while below 1024, then the unbounded last stage.
+std::vector<u32> stage_caps(int cap0) {
+    std::vector<u32> c;
+    u32 a = (u32)cap0;
+    while (a < 1024u && (int)c.size() < MAXSTAGE - 1) { c.push_back(a); a = 2 * a + 1; }
+    c.push_back(UNBOUNDED);
+    return c;
+}
+
+template <int W, int RULE>
+int build_graph(Engine &E, const GraphKey &key, hipGraphExec_t *out) {
+    typedef typename Lay<W>::T T;
+    QueueBufs &Q = qbufs<W>(E);
+    const int blocks = std::max(1, E.cus * key.wpc / 4);
+    const size_t lds = (size_t)4 * (key.hb + key.lb) * sizeof(u32);
+    const std::vector<u32> caps = stage_caps(key.cap0);
+    HIPCHK(hipStreamBeginCapture(E.st, hipStreamCaptureModeThreadLocal));
+    hipLaunchKernelGGL(k_setup, dim3(1), dim3(64), 0, E.st, E.pipe);
+    u32 a0 = 0;
+    for (size_t s = 0; s < caps.size(); s++) {
+        StageArgs<W> A{};
+        A.pipe = E.pipe; A.hdr = E.hdr; A.jobs = E.jobs; A.reqs = E.reqs;
+        const int in = (int)((s + 1) & 1), out = (int)(s & 1);
+        A.qin.a = (T *)Q.qa[in]; A.qin.b = (T *)Q.qb[in]; A.qin.req = Q.qreq[in]; A.qin.pos = Q.qpos[in];
+        A.qout.a = (T *)Q.qa[out]; A.qout.b = (T *)Q.qb[out]; A.qout.req = Q.qreq[out]; A.qout.pos = Q.qpos[out];
+        A.cq = cyc_queue<W>(E);
+        A.qcap = Q.cap; A.cqcap = key.walk ? Q.ccap : 0;
+        A.per = (u32)key.per; A.tile_pos = 64u * (u32)key.per;
+        A.a0 = a0; A.a1 = caps[s]; A.stage = (int)s;
+        A.hb = (u32)key.hb; A.lb = (u32)key.lb;
+        A.need_req = (u32)key.multi; A.need_pos = (u32)key.walk;
+        if (s == 0) hipLaunchKernelGGL((k_stage<W, RULE, true>), dim3(blocks), dim3(256), lds, E.st, A);
+        else        hipLaunchKernelGGL((k_stage<W, RULE, false>), dim3(blocks), dim3(256), lds, E.st, A);
+        a0 = caps[s];
+    }
+    if (key.walk) {
+        WalkArgs<W> WA{};
+        WA.pipe = E.pipe; WA.reqs = E.reqs; WA.cq = cyc_queue<W>(E); WA.cqcap = Q.ccap;
+        hipLaunchKernelGGL((k_walk<W, RULE>), dim3(blocks), dim3(256), 0, E.st, WA);
+    }
+    hipGraph_t g = nullptr;
+    hipError_t ce = hipStreamEndCapture(E.st, &g);
+    if (ce != hipSuccess) return fail(30, "graph capture failed: %s", hipGetErrorString(ce));
+    HIPCHK(hipGraphInstantiate(out, g, nullptr, nullptr, 0));
+    HIPCHK(hipGraphDestroy(g));
+    return 0;
+}
+
+// One batch of jobs (all of layout W and rule RULE) through the persistent pipeline.
+template <int W, int RULE>
+int run_persistent_batch(Engine &E, std::vector<Job> &jobs, const war_tuning &t, int hb, int lb, bool walk,
+                         bool multi, float *ms) {
+    if (jobs.empty()) return 0;
+    if ((int)jobs.size() > MAXJOBS) return fail(31, "too many jobs in one batch");
+    int rc = ensure_queues<W>(E, walk);
+    if (rc) return rc;
+    const u64 tile_pos = (u64)t.ranks_per_refill;
+    u64 tiles = 0;
+    for (Job &j : jobs) { j.tile_base = tiles; tiles += (j.hi - j.lo + tile_pos - 1) / tile_pos; }
+    JobHdr h{};
+    h.njobs = (u32)jobs.size(); h.req0 = jobs[0].req; h.ntiles = tiles;
+    // copy the job list in (the staging buffer is free: the previous batch was synchronised)
+    char *sp = (char *)E.stage_host;
+    memcpy(sp, &h, sizeof h);
+    memcpy(sp + 64, jobs.data(), jobs.size() * sizeof(Job));
+    HIPCHK(hipMemcpyAsync(E.hdr, sp, 64 + jobs.size() * sizeof(Job), hipMemcpyHostToDevice, E.st));
+    GraphKey key{W, RULE, 1, t.waves_per_cu, t.ranks_per_refill / 64, t.brent_cap, walk ? 1 : 0, multi ? 1 : 0,
+                 round_pow2(hb), round_pow2(lb)};
+    auto it = E.graphs.find(key);
+    hipGraphExec_t ge = nullptr;
+    if (it == E.graphs.end()) {
+        rc = build_graph<W, RULE>(E, key, &ge);
+        if (rc) return rc;
+        E.graphs[key] = ge;
+    } else {
+        ge = it->second;
+    }
+    HIPCHK(hipEventRecord(E.ev0, E.st));
+    HIPCHK(hipGraphLaunch(ge, E.st));
+    HIPCHK(hipEventRecord(E.ev1, E.st));
+    HIPCHK(hipStreamSynchronize(E.st));
+    float m = 0;
+    HIPCHK(hipEventElapsedTime(&m, E.ev0, E.ev1));
+    if (ms) *ms = m;
+    return 0;
+}
+
+template <int W, int RULE>
+int run_plain_batch(Engine &E, ReqHost &h, u64 lo, u64 hi, const war_tuning &t, float *ms) {
+    typedef typename Lay<W>::T T;
+    const bool walk = h.dev.walk != 0;
+    int rc = ensure_queues<W>(E, walk);
+    if (rc) return rc;
+    PlainArgs<W> P{};
+    P.lo = lo; P.hi = hi; P.per = (u32)t.plain_per;
+    P.hb = (u32)h.rq.hbins; P.lb = (u32)h.rq.lbins;
+    P.R = h.dev; P.req = (u32)h.slot; P.pipe = E.pipe;
+    P.cq = cyc_queue<W>(E); P.cqcap = walk ? qbufs<W>(E).ccap : 0;
+    const u64 threads = (hi - lo + P.per - 1) / P.per;
+    const u64 blocks = (threads + 255) / 256;
+    const size_t lds = (size_t)(P.hb + P.lb) * sizeof(u32);
+    HIPCHK(hipEventRecord(E.ev0, E.st));
+    hipLaunchKernelGGL(k_setup, dim3(1), dim3(64), 0, E.st, E.pipe);
+    hipLaunchKernelGGL((k_plain<W, RULE>), dim3((unsigned)blocks), dim3(256), lds, E.st, P);
+    HIPCHK(hipGetLastError());
+    if (walk) {
+        WalkArgs<W> WA{};
+        WA.pipe = E.pipe; WA.reqs = E.reqs; WA.cq = cyc_queue<W>(E); WA.cqcap = qbufs<W>(E).ccap;
+        hipLaunchKernelGGL((k_walk<W, RULE>), dim3(std::max(1, E.cus * 4)), dim3(256), 0, E.st, WA);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipEventRecord(E.ev1, E.st));
+    HIPCHK(hipStreamSynchronize(E.st));
+    float m = 0;
+    HIPCHK(hipEventElapsedTime(&m, E.ev0, E.ev1));
+    if (ms) *ms = m;
+    return 0;
+}
+
+template <int W>
+int run_batch_dispatch(Engine &E, int rule, std::vector<Job> &jobs, const war_tuning &t, int hb, int lb, bool walk,
+                       bool multi, float *ms) {
+    return rule ? run_persistent_batch<W, 1>(E, jobs, t, hb, lb, walk, multi, ms)
+                : run_persistent_batch<W, 0>(E, jobs, t, hb, lb, walk, multi, ms);
+}
+
+int alloc_slot(Engine &E) {
+    if (E.free_slots.empty()) return -1;
+    int s = E.free_slots.back();
+    E.free_slots.pop_back();
+    return s;
+}
+
+// Whole range of one request, synchronously (engine mutex held by the caller).
+int run_request_locked(Engine &E, ReqHost &h) {
+    const war_request &r = h.rq;
+    war_tuning t;
+    resolve_tuning(E, r.tuning, t);
+    int rc = check_tuning(t);
+    if (rc) return rc;
+    h.slot = alloc_slot(E);
+    if (h.slot < 0) return fail(40, "no free request slot");
+    rc = req_alloc(E, h);
+    if (rc) { req_free(E, h); return rc; }
+    const int W = r.n <= 16 ? 4 : 5;
+    double dev = 0;
+    if (r.kernel == WAR_KERNEL_PLAIN) {
+        const u64 chunk = 1ull << 26;   // contiguous chunks of 2^26 positions per launch
+        for (u64 lo = r.lo; lo < r.hi && !rc; lo += chunk) {
+            const u64 hi = std::min<u64>(r.hi, lo + chunk);
+            float ms = 0;
+            if (W == 4) rc = r.rule ? run_plain_batch<4, 1>(E, h, lo, hi, t, &ms) : run_plain_batch<4, 0>(E, h, lo, hi, t, &ms);
+            else        rc = r.rule ? run_plain_batch<5, 1>(E, h, lo, hi, t, &ms) : run_plain_batch<5, 0>(E, h, lo, hi, t, &ms);
+            dev += ms;
+        }
+    } else {
+        for (u64 lo = r.lo; lo < r.hi && !rc; lo += E.batch) {
+            const u64 hi = std::min<u64>(r.hi, lo + E.batch);
+            std::vector<Job> jobs(1);
+            jobs[0].lo = lo; jobs[0].hi = hi; jobs[0].req = (u32)h.slot;
+            float ms = 0;
+            if (W == 4) rc = run_batch_dispatch<4>(E, r.rule, jobs, t, r.hbins, r.lbins, h.dev.walk != 0, false, &ms);
+            else        rc = run_batch_dispatch<5>(E, r.rule, jobs, t, r.hbins, r.lbins, h.dev.walk != 0, false, &ms);
+            dev += ms;
+        }
+    }
+    h.dev_ms = dev;
+    if (!rc) rc = req_collect(E, h);
+    req_free(E, h);
+    return rc;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Serving loop: continuous batching. Each step takes, from every admitted request, its next
+// share of positions (step_positions split evenly, whole tiles), runs the groups of one layout and
+// rule as one persistent batch, and retires finished requests. Requests that arrive during a step
+// are admitted at the next one, while slots and device memory allow.
+void serve_loop(Engine *Ep) {
+    Engine &E = *Ep;
+    std::vector<std::shared_ptr<ReqHost>> active;
+    for (;;) {
+        {
+            std::unique_lock<std::mutex> lk(E.smu);
+            E.scv.wait(lk, [&] { return !E.serving || !E.pending.empty() || !active.empty(); });
+            if (!E.serving && E.pending.empty() && active.empty()) break;
+        }
+        std::lock_guard<std::mutex> g(E.mu);
+        // admit
+        for (;;) {
+            std::shared_ptr<ReqHost> h;
+            {
+                std::lock_guard<std::mutex> lk(E.smu);
+                if (E.pending.empty() || (int)active.size() >= E.serve_slots) break;
+                h = E.pending.front();
+            }
+            size_t fr = 0, tot = 0;
+            (void)hipMemGetInfo(&fr, &tot);
+            const size_t need = (size_t)(C_N + h->rq.hbins + h->rq.lbins + h->rq.dbins) * 8 +
+                                (h->rq.distinct ? ((size_t)8 << 27) : 0) + ((size_t)1 << 30);
+            if (fr < need && !active.empty()) break;   // wait for memory to free up
+            {
+                std::lock_guard<std::mutex> lk(E.smu);
+                E.pending.pop_front();
+            }
+            int rc = 0;
+            h->slot = alloc_slot(E);
+            if (h->slot < 0) rc = fail(40, "no free request slot");
+            if (!rc) rc = req_alloc(E, *h);
+            if (rc) {
+                h->status = rc; h->err = t_err; req_free(E, *h);
+                std::lock_guard<std::mutex> lk(E.smu);
+                h->finished = true; h->t_done = now_ms();
+                E.scv.notify_all();
+                continue;
+            }
+            h->next = h->rq.lo;
+            active.push_back(h);
+        }
+        if (active.empty()) continue;
+        // one step per (layout, rule) group
+        int rc = 0;
+        for (int W : {4, 5}) {
+            for (int rule : {0, 1}) {
+                std::vector<std::shared_ptr<ReqHost>> grp;
+                for (auto &h : active) if ((h->rq.n <= 16 ? 4 : 5) == W && h->rq.rule == rule && h->next < h->rq.hi) grp.push_back(h);
+                if (grp.empty()) continue;
+                const u64 tile = (u64)E.tun.ranks_per_refill;
+                u64 share = std::max<u64>(tile, E.step_positions / grp.size());
+                share = (share + tile - 1) / tile * tile;
+                std::vector<Job> jobs;
+                int hb = 0, lb = 0;
+                bool walk = false;
+                for (auto &h : grp) {
+                    Job j{};
+                    j.lo = h->next; j.hi = std::min<u64>(h->rq.hi, h->next + share); j.req = (u32)h->slot;
+                    h->next = j.hi;
+                    jobs.push_back(j);
+                    hb = std::max(hb, h->rq.hbins); lb = std::max(lb, h->rq.lbins);
+                    walk = walk || h->dev.walk;
+                }
+                float ms = 0;
+                rc = (W == 4) ? run_batch_dispatch<4>(E, rule, jobs, E.tun, hb, lb, walk, true, &ms)
+                              : run_batch_dispatch<5>(E, rule, jobs, E.tun, hb, lb, walk, true, &ms);
+                for (auto &h : grp) h->dev_ms += ms;   // device time of the steps the request took part in
+                if (rc) {
+                    for (auto &h : grp) { h->status = rc; h->err = t_err; h->next = h->rq.hi; }
+                    rc = 0;
+                }
+            }
+        }
+        // retire
+        for (size_t i = 0; i < active.size();) {
+            auto h = active[i];
+            if (h->next >= h->rq.hi) {
+                if (!h->status) {
+                    int c = req_collect(E, *h);
+                    if (c) { h->status = c; h->err = t_err; }
+                }
+                req_free(E, *h);
+                {
+                    std::lock_guard<std::mutex> lk(E.smu);
+                    h->finished = true;
+                    h->t_done = now_ms();
+                    if (h->res) h->res->latency_ms = h->t_done - h->t_submit;
+                    E.scv.notify_all();
+                }
+                active.erase(active.begin() + (long)i);
+            } else {
+                i++;
+            }
+        }
+    }
+}
+
+}  // namespace
+
+// =============================================================================================
+extern "C" {
+
+int war_abi_version(void) { return WAR_ABI_VERSION; }
+
+const char *war_last_error(void) {
+    if (!t_err.empty()) return t_err.c_str();
+    std::lock_guard<std::mutex> g(g_err_mu);
+    static thread_local std::string copy;
+    copy = g_err_shared;
+    return copy.c_str();
+}
+
+int war_init(int32_t device, uint32_t flags) {
+    t_err.clear();
+    if (g_eng) return 0;
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev <= 0) return fail(50, "no HIP device available (%s)", hipGetErrorString(e));
+    if (device < 0 || device >= ndev) return fail(51, "device %d out of range (%d devices)", device, ndev);
+    std::unique_ptr<Engine> E(new Engine());
+    E->device = device;
+    E->verbose = flags & 1u;
+    HIPCHK(hipSetDevice(device));
+    hipDeviceProp_t p;
+    HIPCHK(hipGetDeviceProperties(&p, device));
+    E->cus = p.multiProcessorCount;
+    E->clock_mhz = p.clockRate / 1000;
+    E->arch = p.gcnArchName;
+    if (E->arch.rfind("gfx950", 0) != 0)
+        return fail(52, "libwar_engine is built for gfx950; device %d is %s", device, E->arch.c_str());
+    HIPCHK(hipStreamCreateWithFlags(&E->st, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&E->ev0));
+    HIPCHK(hipEventCreate(&E->ev1));
+    HIPCHK(hipMalloc((void **)&E->pipe, sizeof(Pipe)));
+    void *jb = nullptr;
+    HIPCHK(hipMalloc(&jb, 64 + MAXJOBS * sizeof(Job)));
+    E->hdr = (JobHdr *)jb;
+    E->jobs = (Job *)((char *)jb + 64);
+    HIPCHK(hipHostMalloc(&E->stage_host, 64 + MAXJOBS * sizeof(Job), hipHostMallocDefault));
+    HIPCHK(hipMalloc((void **)&E->reqs, MAXSLOTS * sizeof(ReqDev)));
+    HIPCHK(hipMemsetAsync(E->reqs, 0, MAXSLOTS * sizeof(ReqDev), E->st));
+    HIPCHK(hipMemsetAsync(E->pipe, 0, sizeof(Pipe), E->st));
+    HIPCHK(hipStreamSynchronize(E->st));
+    for (int i = MAXSLOTS - 1; i >= 0; i--) E->free_slots.push_back(i);
+    if (E->verbose) fprintf(stderr, "war_engine: device %d %s, %d CUs, %d MHz\n", device, E->arch.c_str(), E->cus, E->clock_mhz);
+    g_eng = E.release();
+    return 0;
+}
+
+int war_shutdown(void) {
+    t_err.clear();
+    if (!g_eng) return 0;
+    war_serve_stop();
+    Engine *E = g_eng;
+    {
+        std::lock_guard<std::mutex> g(E->mu);
+        (void)hipStreamSynchronize(E->st);
+        for (auto &kv : E->graphs) (void)hipGraphExecDestroy(kv.second);
+        for (QueueBufs *Q : {&E->qb4, &E->qb5}) {
+            for (int i = 0; i < 2; i++) { (void)hipFree(Q->qa[i]); (void)hipFree(Q->qb[i]); (void)hipFree(Q->qreq[i]); (void)hipFree(Q->qpos[i]); }
+            (void)hipFree(Q->ca); (void)hipFree(Q->cb); (void)hipFree(Q->cmeta); (void)hipFree(Q->creq); (void)hipFree(Q->clam); (void)hipFree(Q->cpos);
+        }
+        (void)hipFree(E->pipe);
+        (void)hipFree(E->hdr);
+        (void)hipFree(E->reqs);
+        (void)hipHostFree(E->stage_host);
+        (void)hipEventDestroy(E->ev0);
+        (void)hipEventDestroy(E->ev1);
+        (void)hipStreamDestroy(E->st);
+    }
+    delete E;
+    g_eng = nullptr;
+    return 0;
+}
+
+int war_device_info(war_devinfo *out) {
+    t_err.clear();
+    if (!g_eng) return fail(60, "war_init has not been called");
+    if (!out) return fail(1, "null output");
+    memset(out, 0, sizeof *out);
+    out->cus = g_eng->cus; out->max_clock_mhz = g_eng->clock_mhz; out->wave_size = 64;
+    size_t fr = 0, tot = 0;
+    HIPCHK(hipMemGetInfo(&fr, &tot));
+    out->free_bytes = fr; out->total_bytes = tot;
+    snprintf(out->arch, sizeof out->arch, "%s", g_eng->arch.c_str());
+    return 0;
+}
+
+int war_set_tuning(const war_tuning *t) {
+    t_err.clear();
+    if (!g_eng) return fail(60, "war_init has not been called");
+    if (!t) return fail(1, "null tuning");
+    war_tuning r;
+    resolve_tuning(*g_eng, *t, r);
+    int rc = check_tuning(r);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> g(g_eng->mu);
+    g_eng->tun = r;
+    return 0;
+}
+
+int war_get_tuning(war_tuning *t) {
+    t_err.clear();
+    if (!g_eng) return fail(60, "war_init has not been called");
+    if (!t) return fail(1, "null tuning");
+    *t = g_eng->tun;
+    return 0;
+}
+
+int war_run_range(const war_request *req, war_result *res) {
+    t_err.clear();
+    if (!g_eng) return fail(60, "war_init has not been called");
+    int rc = check_request(req);
+    if (rc) return rc;
+    if (!res) return fail(1, "null result");
+    ReqHost h;
+    h.rq = *req;
+    h.res = res;
+    const double t0 = now_ms();
+    {
+        std::lock_guard<std::mutex> g(g_eng->mu);
+        rc = run_request_locked(*g_eng, h);
+    }
+    res->latency_ms = now_ms() - t0;
+    return rc;
+}
+
+int war_autotune(int32_t n, int32_t mode, int32_t rule, double budget_ms, war_tuning *best) {
+    t_err.clear();
+    if (!g_eng) return fail(60, "war_init has not been called");
+    Engine &E = *g_eng;
+    war_request r{};
+    r.n = n; r.mode = mode; r.rule = rule; r.kernel = WAR_KERNEL_PERSISTENT;
+    const u64 total = range_total(n, mode);
+    // sample: a slice from the middle of the range, big enough to fill the GPU several times
+    const u64 want = std::min<u64>(total, 1ull << 25);
+    r.lo = total / 2 - std::min(total / 2, want / 2);
+    r.hi = r.lo + want;
+    int rc = check_request(&r);
+    if (rc) return rc;
+    war_tuning cur = E.tun;
+    war_result ref_res{};
+    double best_ms = 1e30;
+    bool have_ref = false;
+    const double t_start = now_ms();
+    auto trial = [&](const war_tuning &t, double *ms) -> int {
+        war_request q = r;
+        q.tuning = t;
+        war_result o{};
+        ReqHost h;
+        h.rq = q; h.res = &o;
+        int c;
+        {
+            std::lock_guard<std::mutex> g(E.mu);
+            c = run_request_locked(E, h);
+            if (!c) { h.rq.tuning = t; c = run_request_locked(E, h); }   // second run: graph already built
+        }
+        if (c) return c;
+        if (!have_ref) { ref_res = o; have_ref = true; }
+        else if (o.cyc != ref_res.cyc || o.done != ref_res.done || o.maxfin != ref_res.maxfin || o.turns != ref_res.turns)
+            return fail(70, "autotune: schedule changed the counts (exactness violated)");
+        *ms = o.device_ms;
+        return 0;
+    };
+    // coordinate descent over (brent_cap, ranks_per_refill, waves_per_cu)
+    const int caps[] = {7, 15, 31, 63, 127};
+    const int rprs[] = {64, 256, 1024, 4096};
+    const int wpcs[] = {4, 8, 16, 24, 32};
+    for (int pass = 0; pass < 2; pass++) {
+        for (int dim = 0; dim < 3; dim++) {
+            const int nc = dim == 0 ? 5 : dim == 1 ? 4 : 5;
+            war_tuning bestt = cur;
+            for (int i = 0; i < nc; i++) {
+                if (now_ms() - t_start > budget_ms && have_ref) break;
+                war_tuning t = cur;
+                if (dim == 0) t.brent_cap = caps[i];
+                else if (dim == 1) t.ranks_per_refill = rprs[i];
+                else t.waves_per_cu = wpcs[i];
+                double ms = 0;
+                rc = trial(t, &ms);
+                if (rc) return rc;
+                if (E.verbose) fprintf(stderr, "autotune n=%d mode=%d cap=%d rpr=%d wpc=%d: %.3f ms\n", n, mode, t.brent_cap, t.ranks_per_refill, t.waves_per_cu, ms);
+                if (ms < best_ms) { best_ms = ms; bestt = t; }
+            }
+            cur = bestt;
+        }
+    }
+    {
+        std::lock_guard<std::mutex> g(E.mu);
+        E.tun = cur;
+    }
+    if (best) *best = cur;
+    return 0;
+}
+
+int war_serve_start(int32_t max_slots, uint64_t step_positions) {
+    t_err.clear();
+    if (!g_eng) return fail(60, "war_init has not been called");
+    Engine &E = *g_eng;
+    if (E.serving) return 0;
+    if (max_slots < 1 || max_slots > MAXSLOTS - 8) return fail(80, "max_slots must be in [1, %d]", MAXSLOTS - 8);
+    E.serve_slots = max_slots;
+    if (step_positions) E.step_positions = std::min<u64>(step_positions, E.batch);
+    E.serving = true;
+    E.worker = std::thread(serve_loop, &E);
+    return 0;
+}
+
+int war_serve_stop(void) {
+    t_err.clear();
+    if (!g_eng) return 0;
+    Engine &E = *g_eng;
+    if (!E.serving) return 0;
+    {
+        std::lock_guard<std::mutex> lk(E.smu);
+        E.serving = false;
+        E.scv.notify_all();
+    }
+    if (E.worker.joinable()) E.worker.join();
+    return 0;
+}
+
+int war_submit(const war_request *req, war_result *res, uint64_t *id) {
+    t_err.clear();
+    if (!g_eng) return fail(60, "war_init has not been called");
+    Engine &E = *g_eng;
+    if (!E.serving) return fail(81, "serving loop not started (war_serve_start)");
+    int rc = check_request(req);
+    if (rc) return rc;
+    if (req->kernel != WAR_KERNEL_PERSISTENT) return fail(82, "the serving loop runs the persistent kernel only");
+    if (!res || !id) return fail(1, "null result or id");
+    auto h = std::make_shared<ReqHost>();
+    h->rq = *req;
+    h->res = res;
+    h->t_submit = now_ms();
+    std::lock_guard<std::mutex> lk(E.smu);
+    h->id = E.next_id++;
+    *id = h->id;
+    E.all[h->id] = h;
+    E.pending.push_back(h);
+    E.scv.notify_all();
+    return 0;
+}
+
+int war_poll(uint64_t id, int32_t *done) {
+    t_err.clear();
+    if (!g_eng) return fail(60, "war_init has not been called");
+    Engine &E = *g_eng;
+    std::lock_guard<std::mutex> lk(E.smu);
+    auto it = E.all.find(id);
+    if (it == E.all.end()) return fail(83, "unknown request id %llu", (unsigned long long)id);
+    if (done) *done = it->second->finished ? 1 : 0;
+    return 0;
+}
+
+int war_wait(uint64_t id) {
+    t_err.clear();
+    if (!g_eng) return fail(60, "war_init has not been called");
+    Engine &E = *g_eng;
+    std::shared_ptr<ReqHost> h;
+    {
+        std::unique_lock<std::mutex> lk(E.smu);
+        auto it = E.all.find(id);
+        if (it == E.all.end()) return fail(83, "unknown request id %llu", (unsigned long long)id);
+        h = it->second;
+        E.scv.wait(lk, [&] { return h->finished; });
+        E.all.erase(id);
+    }
+    if (h->status) return fail(h->status, "%s", h->err.c_str());
+    return 0;
+}
+
+}  // extern "C"

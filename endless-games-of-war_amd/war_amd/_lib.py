@@ -1,0 +1,113 @@
+"""ctypes binding of libwar_engine.so (the C ABI in include/war_engine.h).
+
+The library is the only compute path: if it is missing or cannot be loaded, load() raises. There
+is no CPU fallback anywhere in this package.
+"""
+import ctypes
+import os
+import re
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.environ.get("WAR_ENGINE_LIB", os.path.join(PKG_DIR, "libwar_engine.so"))
+HEADER = os.path.join(os.path.dirname(PKG_DIR), "include", "war_engine.h")
+
+MODE_FULL, MODE_HALF, MODE_CANON = 0, 1, 2
+KERNEL_PLAIN, KERNEL_PERSISTENT = 0, 1
+KERNELS = {"plain": KERNEL_PLAIN, "persistent": KERNEL_PERSISTENT}
+
+
+class WarTuning(ctypes.Structure):
+    _fields_ = [("waves_per_cu", ctypes.c_int32), ("ranks_per_refill", ctypes.c_int32),
+                ("brent_cap", ctypes.c_int32), ("plain_per", ctypes.c_int32)]
+
+
+class WarRequest(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int32), ("mode", ctypes.c_int32), ("rule", ctypes.c_int32),
+                ("kernel", ctypes.c_int32), ("lo", ctypes.c_uint64), ("hi", ctypes.c_uint64),
+                ("hbins", ctypes.c_int32), ("lbins", ctypes.c_int32), ("dbins", ctypes.c_int32),
+                ("distinct", ctypes.c_int32), ("shape_blk", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("tuning", WarTuning)]
+
+
+_U64P = ctypes.POINTER(ctypes.c_uint64)
+
+
+class WarResult(ctypes.Structure):
+    _fields_ = [("cyc", ctypes.c_uint64), ("done", ctypes.c_uint64), ("maxfin", ctypes.c_uint64),
+                ("turns", ctypes.c_uint64), ("oncycle", ctypes.c_uint64), ("distinct_minkey", ctypes.c_uint64),
+                ("distinct_hashset", ctypes.c_uint64), ("shape_nonstd", ctypes.c_uint64),
+                ("shape_unchk", ctypes.c_uint64), ("undetermined", ctypes.c_uint64),
+                ("hist", _U64P), ("hbins", ctypes.c_int32),
+                ("lenhist", _U64P), ("lbins", ctypes.c_int32),
+                ("dhist", _U64P), ("dbins", ctypes.c_int32),
+                ("device_ms", ctypes.c_double), ("latency_ms", ctypes.c_double)]
+
+
+class WarDevinfo(ctypes.Structure):
+    _fields_ = [("cus", ctypes.c_int32), ("max_clock_mhz", ctypes.c_int32), ("wave_size", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("free_bytes", ctypes.c_uint64), ("total_bytes", ctypes.c_uint64),
+                ("arch", ctypes.c_char * 64)]
+
+
+class WarError(RuntimeError):
+    """A non-zero status from the engine (the message is war_last_error())."""
+
+    def __init__(self, status, msg):
+        super().__init__("war_engine status %d: %s" % (status, msg))
+        self.status = status
+
+
+def header_functions(path=HEADER):
+    """Names of the functions include/war_engine.h declares."""
+    with open(path) as f:
+        text = f.read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(war_\w+)\s*\(", text, flags=re.M)
+
+
+_lib = None
+
+_SIGS = {
+    "war_abi_version": ([], ctypes.c_int),
+    "war_init": ([ctypes.c_int32, ctypes.c_uint32], ctypes.c_int),
+    "war_shutdown": ([], ctypes.c_int),
+    "war_last_error": ([], ctypes.c_char_p),
+    "war_device_info": ([ctypes.POINTER(WarDevinfo)], ctypes.c_int),
+    "war_run_range": ([ctypes.POINTER(WarRequest), ctypes.POINTER(WarResult)], ctypes.c_int),
+    "war_set_tuning": ([ctypes.POINTER(WarTuning)], ctypes.c_int),
+    "war_get_tuning": ([ctypes.POINTER(WarTuning)], ctypes.c_int),
+    "war_autotune": ([ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.POINTER(WarTuning)],
+                     ctypes.c_int),
+    "war_serve_start": ([ctypes.c_int32, ctypes.c_uint64], ctypes.c_int),
+    "war_serve_stop": ([], ctypes.c_int),
+    "war_submit": ([ctypes.POINTER(WarRequest), ctypes.POINTER(WarResult), ctypes.POINTER(ctypes.c_uint64)],
+                   ctypes.c_int),
+    "war_poll": ([ctypes.c_uint64, ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
+    "war_wait": ([ctypes.c_uint64], ctypes.c_int),
+}
+
+
+def load(path=None):
+    """Load libwar_engine.so (once). Raises OSError if it is missing: build it first."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise OSError("libwar_engine.so not found at %s; build it with __graft_entry__.build() "
+                      "or make -C endless-games-of-war_amd/csrc" % p)
+    lib = ctypes.CDLL(p)
+    for name, (args, res) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def check(status):
+    if status != 0:
+        msg = _lib.war_last_error() if _lib is not None else b""
+        raise WarError(status, (msg or b"").decode(errors="replace"))
+    return status

@@ -1,0 +1,143 @@
+"""Python face of the MI355X War engine: range queries, autotuning and the serving loop.
+
+All counting happens in libwar_engine.so on the GPU; this module only marshals requests and
+results. Range units: deal ranks in [0, n!) for full/half mode, canonical positions
+p = r*(n-1) + k in [0, (n-1)!(n-1)) for canon mode.
+"""
+import ctypes
+import math
+import threading
+
+from . import _lib
+from ._lib import (KERNEL_PERSISTENT, KERNEL_PLAIN, KERNELS, MODE_CANON, MODE_FULL, MODE_HALF, WarDevinfo,
+                   WarRequest, WarResult, WarTuning, check)
+
+MODES = {"full": MODE_FULL, "half": MODE_HALF, "canon": MODE_CANON}
+
+
+def total_positions(n, mode):
+    """Size of the range of (n, mode): n! deals, or (n-1)!(n-1) canonical positions."""
+    mode = MODES.get(mode, mode)
+    if n < 1:
+        return 0
+    if mode == MODE_CANON:
+        return 0 if n < 2 else math.factorial(n - 1) * (n - 1)
+    return math.factorial(n)
+
+
+def _tuning(t):
+    if t is None:
+        return WarTuning(0, 0, 0, 0)
+    if isinstance(t, WarTuning):
+        return t
+    return WarTuning(int(t.get("waves_per_cu", 0)), int(t.get("ranks_per_refill", 0)),
+                     int(t.get("brent_cap", 0)), int(t.get("plain_per", 0)))
+
+
+def make_request(n, mode="full", lo=0, hi=None, rule=0, kernel="persistent", hbins=0, lbins=0, dbins=0,
+                 distinct=False, shape_blk=0, tuning=None):
+    mode = MODES.get(mode, mode)
+    if hi is None:
+        hi = total_positions(n, mode)
+    return WarRequest(int(n), int(mode), int(rule), int(KERNELS.get(kernel, kernel)), int(lo), int(hi),
+                      int(hbins), int(lbins), int(dbins), int(bool(distinct)), int(shape_blk), 0, _tuning(tuning))
+
+
+class _ResultBuf:
+    """A WarResult with its caller-owned histogram arrays."""
+
+    def __init__(self, req):
+        self.req = req
+        self.hist = (ctypes.c_uint64 * max(1, req.hbins))()
+        self.lenhist = (ctypes.c_uint64 * max(1, req.lbins))()
+        self.dhist = (ctypes.c_uint64 * max(1, req.dbins))()
+        self.res = WarResult()
+        self.res.hist = ctypes.cast(self.hist, ctypes.POINTER(ctypes.c_uint64))
+        self.res.hbins = req.hbins
+        self.res.lenhist = ctypes.cast(self.lenhist, ctypes.POINTER(ctypes.c_uint64))
+        self.res.lbins = req.lbins
+        self.res.dhist = ctypes.cast(self.dhist, ctypes.POINTER(ctypes.c_uint64))
+        self.res.dbins = req.dbins
+
+    def as_dict(self):
+        r, q = self.res, self.req
+        return {
+            "n": q.n, "mode": q.mode, "rule": q.rule, "lo": q.lo, "hi": q.hi,
+            "cyc": r.cyc, "done": r.done, "maxfin": r.maxfin, "turns": r.turns,
+            "oncycle": r.oncycle, "distinct_minkey": r.distinct_minkey, "distinct_hashset": r.distinct_hashset,
+            "shape_nonstd": r.shape_nonstd, "shape_unchk": r.shape_unchk, "undetermined": r.undetermined,
+            "hist": [int(self.hist[i]) for i in range(q.hbins)],
+            "lenhist": [int(self.lenhist[i]) for i in range(q.lbins)],
+            "dhist": [int(self.dhist[i]) for i in range(q.dbins)],
+            "device_ms": r.device_ms, "latency_ms": r.latency_ms,
+        }
+
+
+class Engine:
+    """One engine per process (the library holds a single device context)."""
+
+    _lock = threading.Lock()
+
+    def __init__(self, device=0, verbose=False):
+        self.lib = _lib.load()
+        with Engine._lock:
+            check(self.lib.war_init(int(device), 1 if verbose else 0))
+        self._pending = {}
+
+    # ---- info / tuning
+    def device_info(self):
+        d = WarDevinfo()
+        check(self.lib.war_device_info(ctypes.byref(d)))
+        return {"cus": d.cus, "max_clock_mhz": d.max_clock_mhz, "wave_size": d.wave_size,
+                "free_bytes": d.free_bytes, "total_bytes": d.total_bytes, "arch": d.arch.decode()}
+
+    def get_tuning(self):
+        t = WarTuning()
+        check(self.lib.war_get_tuning(ctypes.byref(t)))
+        return {"waves_per_cu": t.waves_per_cu, "ranks_per_refill": t.ranks_per_refill,
+                "brent_cap": t.brent_cap, "plain_per": t.plain_per}
+
+    def set_tuning(self, **kw):
+        t = _tuning(kw)
+        check(self.lib.war_set_tuning(ctypes.byref(t)))
+
+    def autotune(self, n, mode="canon", rule=0, budget_ms=20000.0):
+        t = WarTuning()
+        check(self.lib.war_autotune(int(n), int(MODES.get(mode, mode)), int(rule), float(budget_ms), ctypes.byref(t)))
+        return {"waves_per_cu": t.waves_per_cu, "ranks_per_refill": t.ranks_per_refill,
+                "brent_cap": t.brent_cap, "plain_per": t.plain_per}
+
+    # ---- synchronous range query
+    def run_range(self, n, mode="full", lo=0, hi=None, **kw):
+        req = make_request(n, mode, lo, hi, **kw)
+        buf = _ResultBuf(req)
+        check(self.lib.war_run_range(ctypes.byref(req), ctypes.byref(buf.res)))
+        return buf.as_dict()
+
+    # ---- serving loop
+    def serve_start(self, max_slots=64, step_positions=0):
+        check(self.lib.war_serve_start(int(max_slots), int(step_positions)))
+
+    def serve_stop(self):
+        check(self.lib.war_serve_stop())
+
+    def submit(self, n, mode="full", lo=0, hi=None, **kw):
+        req = make_request(n, mode, lo, hi, **kw)
+        buf = _ResultBuf(req)
+        rid = ctypes.c_uint64(0)
+        check(self.lib.war_submit(ctypes.byref(req), ctypes.byref(buf.res), ctypes.byref(rid)))
+        self._pending[rid.value] = buf
+        return rid.value
+
+    def poll(self, rid):
+        d = ctypes.c_int32(0)
+        check(self.lib.war_poll(ctypes.c_uint64(rid), ctypes.byref(d)))
+        return bool(d.value)
+
+    def wait(self, rid):
+        buf = self._pending.pop(rid)
+        check(self.lib.war_wait(ctypes.c_uint64(rid)))
+        return buf.as_dict()
+
+    def close(self):
+        check(self.lib.war_shutdown())

@@ -1,0 +1,125 @@
+/* war_engine.h -- C ABI of the MI355X (gfx950) War enumeration engine, libwar_engine.so.
+ *
+ * The engine counts, over a range of deals or canonical positions of single-suit War, the games
+ * that never end, exactly as the reference's GPU counter does (gpu/war_gpu.py), and adds a
+ * persistent compacting kernel, an autotuner and a continuous-batching request loop.
+ *
+ * Plain pointers and fixed-width integers only; no framework types cross this boundary. Every
+ * entry point returns 0 on success and a non-zero status otherwise (war_last_error() then gives
+ * the reason). The library never aborts the process.
+ *
+ * The reference has no FFI of its own; the calls below replace the host driver of
+ * gpu/war_gpu.py and its kernel launches:
+ *   war_run_range   <- run() (gpu/war_gpu.py:272-352): counters, batching, launches, sync, and the
+ *                      war_count / war_canon kernels (gpu/war_gpu.py:121-176, :183-259)
+ *   war_result      <- the 7-tuple run() returns (gpu/war_gpu.py:346-352), plus lenhist/turns
+ *   war_request.mode   FULL / HALF / CANON <- run(half=, canon=) (gpu/war_gpu.py:272-290)
+ *   war_request.hbins  <- --hist (gpu/war_gpu.py:364), dbins <- --distinct (:369-370,379)
+ *   war_request.shape_blk <- --shape / WAR_BLK_OVERRIDE (gpu/war_gpu.py:290-292)
+ *   war_request.rule   <- --rule (gpu/war_gpu.py:367-368)
+ *   5-bit piles for n >= 17 <- gpu/war24.py:31-47 (layout only)
+ */
+#ifndef WAR_ENGINE_H
+#define WAR_ENGINE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WAR_ABI_VERSION 1
+
+/* mode */
+#define WAR_MODE_FULL  0   /* deal ranks r in [0, n!), deck positions 0,2,4.. to player 1     */
+#define WAR_MODE_HALF  1   /* as FULL, but only deals giving card n-1 to player 1 are played
+                              (even n); counts are NOT doubled here (run() doubles them)       */
+#define WAR_MODE_CANON 2   /* positions p = r*(n-1)+k: player 1 = [n-1, q0..q(k-1)],
+                              player 2 = [qk..q(n-2)], r = lexicographic rank of q            */
+
+/* kernel */
+#define WAR_KERNEL_PLAIN      0  /* one position per lane, per-lane Brent (reference structure) */
+#define WAR_KERNEL_PERSISTENT 1  /* persistent, age-synchronous stages with survivor compaction */
+
+typedef struct war_tuning {
+    int32_t waves_per_cu;      /* persistent grid = CUs x waves_per_cu wave64s (1..16)          */
+    int32_t ranks_per_refill;  /* positions a wave takes per atomic grab (64..8192, mult of 64)  */
+    int32_t brent_cap;         /* first stage boundary 2^k-1 (7..1023); later stages double     */
+    int32_t plain_per;         /* plain kernel: consecutive positions per lane (1..4096)        */
+} war_tuning;
+
+typedef struct war_request {
+    int32_t  n;            /* cards: 1..16 (4-bit piles), 17..20 (5-bit piles)                 */
+    int32_t  mode;         /* WAR_MODE_*                                                         */
+    int32_t  rule;         /* 0: winner's card first (Spivey); 1: loser's card first            */
+    int32_t  kernel;       /* WAR_KERNEL_*                                                       */
+    uint64_t lo, hi;       /* half-open range: deal ranks (FULL/HALF) or positions (CANON)      */
+    int32_t  hbins;        /* cycle-length histogram bins, last = overflow (0 = off, <= 4096)   */
+    int32_t  lbins;        /* ending-game-length histogram bins, last = overflow (0 = off)      */
+    int32_t  dbins;        /* distinct weight bins (CANON, n <= 15; 0 = off, <= 4096)           */
+    int32_t  distinct;     /* CANON, n <= 15: min-key count + device hash set of cycle keys    */
+    int32_t  shape_blk;    /* CANON: Theorem-3 run length to test (0 = off)                      */
+    int32_t  reserved;
+    war_tuning tuning;     /* zeros = engine defaults                                            */
+} war_request;
+
+typedef struct war_result {
+    uint64_t cyc;              /* endless deals / cycling positions                              */
+    uint64_t done;             /* positions played (must equal the range length; HALF: taken)   */
+    uint64_t maxfin;           /* longest ending game, in turns                                  */
+    uint64_t turns;            /* sum of Brent rounds (game length, or turns to the repeat)      */
+    uint64_t oncycle;          /* distinct/dbins: positions lying on their own cycle             */
+    uint64_t distinct_minkey;  /* distinct: positions whose key is their cycle's minimum key    */
+    uint64_t distinct_hashset; /* distinct: different cycle-minimum keys met (device hash set)  */
+    uint64_t shape_nonstd;     /* shape: cycles not made of winner runs of exactly shape_blk    */
+    uint64_t shape_unchk;      /* shape: cycles longer than 512 turns (not checked)             */
+    uint64_t undetermined;     /* games still running at the safety cap (2^24 turns); must be 0 */
+    uint64_t *hist;    int32_t hbins;   /* caller-owned, hbins entries (or NULL)               */
+    uint64_t *lenhist; int32_t lbins;   /* caller-owned, lbins entries (or NULL)               */
+    uint64_t *dhist;   int32_t dbins;   /* caller-owned, dbins entries (or NULL)               */
+    double   device_ms;        /* device time of the request's kernels (HIP events)             */
+    double   latency_ms;       /* serving: submit -> completion, host clock                     */
+} war_result;
+
+typedef struct war_devinfo {
+    int32_t  cus;              /* compute units                                                  */
+    int32_t  max_clock_mhz;
+    int32_t  wave_size;
+    int32_t  reserved;
+    uint64_t free_bytes, total_bytes;
+    char     arch[64];
+} war_devinfo;
+
+/* Lifecycle. war_init selects the device, creates the engine's stream and device buffers.
+ * flags: bit 0 = verbose to stderr. */
+int war_init(int32_t device, uint32_t flags);
+int war_shutdown(void);
+const char *war_last_error(void);
+int war_abi_version(void);
+int war_device_info(war_devinfo *out);
+
+/* Synchronous range query: plays every position of [lo, hi) on the GPU and fills *res. */
+int war_run_range(const war_request *req, war_result *res);
+
+/* Engine-wide default tuning (used when a request's tuning is zero). */
+int war_set_tuning(const war_tuning *t);
+int war_get_tuning(war_tuning *t);
+
+/* Autotuner: times candidate schedules of the persistent kernel on a sample of (n, mode) and
+ * installs the fastest as the default. budget_ms bounds the search; *best receives the choice. */
+int war_autotune(int32_t n, int32_t mode, int32_t rule, double budget_ms, war_tuning *best);
+
+/* Continuous-batching request loop. Requests submitted while others run are admitted into free
+ * slots (bounded by slots and by free device memory) and share the persistent launches.
+ * war_submit copies *req; results are retrieved once with war_wait (which blocks) after war_poll
+ * reports done. The result's hist/lenhist/dhist pointers must stay valid until war_wait returns. */
+int war_serve_start(int32_t max_slots, uint64_t step_positions);
+int war_serve_stop(void);
+int war_submit(const war_request *req, war_result *res, uint64_t *id);
+int war_poll(uint64_t id, int32_t *done);
+int war_wait(uint64_t id);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WAR_ENGINE_H */

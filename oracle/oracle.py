@@ -1,0 +1,100 @@
+"""ctypes binding of the CPU oracle (oracle/war_oracle.cpp -> oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check and by the benchmark's CPU-baseline
+leg, always as the checker or the CPU baseline, never as part of the product path.
+"""
+import ctypes
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF_DIR = os.path.join(HERE, "_ref")
+
+FULL, HALF, CANON = 0, 1, 2
+_MODES = {"full": FULL, "half": HALF, "canon": CANON}
+
+_U64P = ctypes.POINTER(ctypes.c_uint64)
+
+
+class OrcRes(ctypes.Structure):
+    _fields_ = [("cyc", ctypes.c_uint64), ("done", ctypes.c_uint64), ("maxfin", ctypes.c_uint64),
+                ("turns", ctypes.c_uint64), ("oncycle", ctypes.c_uint64), ("distinct_minkey", ctypes.c_uint64),
+                ("distinct_keys", ctypes.c_uint64), ("shape_nonstd", ctypes.c_uint64),
+                ("shape_unchk", ctypes.c_uint64),
+                ("hist", _U64P), ("hbins", ctypes.c_int),
+                ("lenhist", _U64P), ("lbins", ctypes.c_int),
+                ("dhist", _U64P), ("dbins", ctypes.c_int)]
+
+
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE, "all"])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        L.orc_run.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64,
+                              ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(OrcRes)]
+        L.orc_run.restype = ctypes.c_int
+        L.orc_last_error.restype = ctypes.c_char_p
+        L.orc_total.argtypes = [ctypes.c_int, ctypes.c_int]
+        L.orc_total.restype = ctypes.c_uint64
+        L.orc_play.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                               ctypes.c_int, ctypes.c_int, _U64P, _U64P]
+        L.orc_play.restype = ctypes.c_int
+        L.orc_unrank.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.orc_unrank.restype = None
+        _lib = L
+    return _lib
+
+
+def total(n, mode="full"):
+    return int(lib().orc_total(int(n), _MODES.get(mode, mode)))
+
+
+def run(n, mode="full", lo=0, hi=None, rule=0, hbins=0, lbins=0, dbins=0, distinct=False, shape_blk=0, threads=0):
+    """Play every position of [lo, hi) on the CPU; returns a dict shaped like the engine's result."""
+    L = lib()
+    m = _MODES.get(mode, mode)
+    if hi is None:
+        hi = total(n, m)
+    H = (ctypes.c_uint64 * max(hbins, 1))()
+    Ln = (ctypes.c_uint64 * max(lbins, 1))()
+    D = (ctypes.c_uint64 * max(dbins, 1))()
+    r = OrcRes()
+    r.hist = ctypes.cast(H, _U64P); r.hbins = hbins
+    r.lenhist = ctypes.cast(Ln, _U64P); r.lbins = lbins
+    r.dhist = ctypes.cast(D, _U64P); r.dbins = dbins
+    rc = L.orc_run(int(n), m, int(rule), int(lo), int(hi), int(threads), int(bool(distinct)), int(shape_blk),
+                   ctypes.byref(r))
+    if rc:
+        raise ValueError("oracle: %s" % L.orc_last_error().decode())
+    return {"cyc": r.cyc, "done": r.done, "maxfin": r.maxfin, "turns": r.turns, "oncycle": r.oncycle,
+            "distinct_minkey": r.distinct_minkey, "distinct_keys": r.distinct_keys,
+            "shape_nonstd": r.shape_nonstd, "shape_unchk": r.shape_unchk,
+            "hist": [int(H[i]) for i in range(hbins)], "lenhist": [int(Ln[i]) for i in range(lbins)],
+            "dhist": [int(D[i]) for i in range(dbins)]}
+
+
+def play(n, p1, p2, rule=0):
+    """One position (piles top card first). Returns (endless, rounds, lam)."""
+    L = lib()
+    a = (ctypes.c_int * max(1, len(p1)))(*p1)
+    b = (ctypes.c_int * max(1, len(p2)))(*p2)
+    rounds = ctypes.c_uint64(0)
+    lam = ctypes.c_uint64(0)
+    c = L.orc_play(int(n), a, len(p1), b, len(p2), int(rule), ctypes.byref(rounds), ctypes.byref(lam))
+    return bool(c), rounds.value, lam.value
+
+
+def unrank(r, m):
+    p = (ctypes.c_int * max(1, m))()
+    lib().orc_unrank(int(r), int(m), p)
+    return [p[i] for i in range(m)]

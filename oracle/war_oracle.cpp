@@ -1,0 +1,386 @@
+// War enumeration oracle: a CPU restatement (C++17 + OpenMP) of the reference's hot path.
+//
+// TEST INFRASTRUCTURE ONLY. Nothing in the product (the HIP engine, its C ABI or the Python
+// drop-in) links, loads or calls this file. It is used by the test suite as the checker, by
+// the smoke check, and by the benchmark's CPU-baseline leg.
+//
+// What it restates (reference file:line):
+//   turn ........................ gpu/war_gpu.py:38-45 (WAR_STEP), cpu/war.cpp:17-23 (step)
+//   Brent cycle test ............ gpu/war_gpu.py:77-96 (war_sim), cpu/war.cpp:26-37 (run)
+//   deal -> piles ............... gpu/war_gpu.py:98-107, cpu/war.cpp:48-49
+//   lexicographic unrank ........ gpu/war_gpu.py:135-146 and :196-206
+//   next permutation ............ gpu/war_gpu.py:109-119 (std::next_permutation in war.cpp:53)
+//   deal counter (+ --half) ..... gpu/war_gpu.py:121-176, cpu/war.cpp:38-60
+//   canonical positions ......... gpu/war_gpu.py:183-259, cpu/war_all.cpp:47-51
+//   distinct weights (dhist) .... gpu/war_gpu.py:217-235, :387-392
+//   distinct by minimum key ..... cpu/distinct_cycles.c:24-31 (key), :57-80
+//   loop shape (Theorem-3 test) . gpu/war_gpu.py:51-73, :238-242
+//   5-bit piles (n >= 17) ....... gpu/war24.py:31-47 (only the layout; the game is the same)
+//
+// Extensions the reference does not have (pinned by invariants in the tests, not by goldens):
+// arbitrary rank / position ranges, the histogram of ending-game lengths (lenhist), the total
+// number of Brent turns (turns), and the count of distinct cycle keys touched by a range.
+//
+// Range units: full / half mode use deal ranks r in [0, n!); canon mode uses positions
+// p = r*(n-1) + k with r in [0, (n-1)!) and split k in [0, n-2] (k = cards below P1's top).
+//
+// Build: see oracle/Makefile (g++ -O3 -march=native -fopenmp -shared -fPIC).
+#include <algorithm>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <set>
+#include <string>
+#include <vector>
+#include <omp.h>
+
+typedef unsigned long long u64;
+typedef unsigned __int128 u128;
+
+enum { ORC_FULL = 0, ORC_HALF = 1, ORC_CANON = 2 };
+
+extern "C" {
+struct orc_res {
+    u64 cyc;              // endless deals / cycling positions
+    u64 done;             // positions played (half mode: the deals taken)
+    u64 maxfin;           // longest ending game, in turns
+    u64 turns;            // sum over positions of Brent's rounds (game length, or turns to detection)
+    u64 oncycle;          // canon + distinct: positions that lie on their own cycle
+    u64 distinct_minkey;  // canon + distinct: positions whose key is their cycle's minimum key
+    u64 distinct_keys;    // canon + distinct: number of different cycle-minimum keys met in the range
+    u64 shape_nonstd;     // canon + shape: cycles whose winner runs are not all of length blk
+    u64 shape_unchk;      // canon + shape: cycles longer than 512 turns (not checked)
+    u64* hist;  int hbins;    // lambda histogram of endless positions (last bin = overflow)
+    u64* lenhist; int lbins;  // histogram of ending-game lengths (last bin = overflow)
+    u64* dhist; int dbins;    // distinct weights (canon + distinct)
+};
+}
+
+// ---------------------------------------------------------------------------------------------
+// Pile layouts. Card i of a pile (0 = top) sits at bit W*i. W = 4 for n <= 16 (u64), 5 for
+// n <= 25 (u128). The game rules do not depend on the layout.
+template <int W> struct Lay;
+template <> struct Lay<4> { typedef u64 T; };
+template <> struct Lay<5> { typedef u128 T; };
+
+template <int W> struct St {
+    typedef typename Lay<W>::T T;
+    T a, b;   // P1, P2
+    int la, lb;
+};
+
+template <int W> static inline bool same(const St<W>& x, const St<W>& y) {
+    // (w1, w2, l1) as in war_gpu.py:91 / war.cpp:24; l1 is needed because card 0 and an empty
+    // slot look alike.
+    return x.a == y.a && x.b == y.b && x.la == y.la;
+}
+
+// One turn (war_gpu.py:38-45). Returns 1 if player 1 won the turn.
+template <int W> static inline int step(St<W>& s, int rule) {
+    typedef typename Lay<W>::T T;
+    const T M = (T)((1u << W) - 1u);
+    T c1 = s.a & M, c2 = s.b & M;
+    s.a >>= W; s.b >>= W; s.la--; s.lb--;
+    if (c1 > c2) {
+        T x = rule ? c2 : c1, y = rule ? c1 : c2;
+        s.a |= x << (W * s.la); s.la++;
+        s.a |= y << (W * s.la); s.la++;
+        return 1;
+    }
+    T x = rule ? c1 : c2, y = rule ? c2 : c1;
+    s.b |= x << (W * s.lb); s.lb++;
+    s.b |= y << (W * s.lb); s.lb++;
+    return 0;
+}
+
+// Brent (war_gpu.py:77-96). Returns true if the game never ends; rounds = turns played until
+// the end or until the repeat is seen; lam = cycle length; meet = state at detection.
+template <int W> static inline bool brent(St<W> s, int rule, u64& rounds, u64& lam_out, St<W>* meet) {
+    if (s.la == 0 || s.lb == 0) { rounds = 0; return false; }
+    St<W> t = s;
+    u64 power = 1, lam = 0, r = 0;
+    for (;;) {
+        step<W>(s, rule);
+        r++;
+        if (s.la == 0 || s.lb == 0) { rounds = r; return false; }
+        lam++;
+        if (same<W>(s, t)) { rounds = r; lam_out = lam; if (meet) *meet = s; return true; }
+        if (lam == power) { t = s; power <<= 1; lam = 0; }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Permutations (lexicographic, p[0] most significant), war_gpu.py:135-146 and :109-119.
+static u64 fact_u64(int k) { u64 f = 1; for (int i = 2; i <= k; i++) f *= (u64)i; return f; }
+
+static void unrank(u64 r, int m, int* p) {
+    int avail[32]; int na = m;
+    for (int i = 0; i < m; i++) avail[i] = i;
+    for (int i = 0; i < m; i++) {
+        u64 f = fact_u64(m - 1 - i);
+        int d = (int)(r / f); r -= (u64)d * f;
+        p[i] = avail[d];
+        for (int t = d; t < na - 1; t++) avail[t] = avail[t + 1];
+        na--;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Position builders.
+template <int W> static inline St<W> deal(const int* p, int n) {   // war_gpu.py:98-107
+    typedef typename Lay<W>::T T;
+    St<W> s; s.a = 0; s.b = 0; s.la = 0; s.lb = 0;
+    for (int i = 0; i < n; i++) {
+        if ((i & 1) == 0) { s.a |= ((T)p[i]) << (W * s.la); s.la++; }
+        else              { s.b |= ((T)p[i]) << (W * s.lb); s.lb++; }
+    }
+    return s;
+}
+
+template <int W> static inline St<W> canon_pos(const int* q, int n, int k) {   // war_gpu.py:210-213
+    typedef typename Lay<W>::T T;
+    int m = n - 1;
+    St<W> s; s.a = (T)(n - 1); s.la = 1; s.b = 0; s.lb = 0;
+    for (int i = 0; i < k; i++) { s.a |= ((T)q[i]) << (W * s.la); s.la++; }
+    for (int i = k; i < m; i++) { s.b |= ((T)q[i]) << (W * s.lb); s.lb++; }
+    return s;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Distinct cycles. key() follows cpu/distinct_cycles.c:24-31 (4-bit cards, n <= 15).
+static inline u64 key4(const St<4>& s) {
+    u64 k = (u64)s.la << 60; int pos = 56;
+    u64 v = s.a;
+    for (int i = 0; i < s.la; i++) { k |= (v & 15ULL) << pos; v >>= 4; pos -= 4; }
+    v = s.b;
+    for (int i = 0; i < s.lb; i++) { k |= (v & 15ULL) << pos; v >>= 4; pos -= 4; }
+    return k;
+}
+static inline St<4> swapst(const St<4>& s) { St<4> t; t.a = s.b; t.b = s.a; t.la = s.lb; t.lb = s.la; return t; }
+
+// loop_shape (war_gpu.py:51-73): 0 = runs of exactly blk, 1 = other shape, 2 = lam > 512.
+template <int W> static int loop_shape(St<W> s, u64 lam, int blk, int rule) {
+    if (lam > 512) return 2;
+    std::vector<int> wb((size_t)lam);
+    for (u64 i = 0; i < lam; i++) wb[i] = step<W>(s, rule);
+    int L = (int)lam;
+    auto WB = [&](int i) { return wb[(size_t)(i % L)]; };
+    int st = -1;
+    for (int i = 0; i < L; i++) if (WB(i) != WB(i + L - 1)) { st = i; break; }
+    if (st < 0) return 1;
+    int run = 1;
+    for (int j = 1; j <= L; j++) {
+        if (j < L && WB(st + j) == WB(st + j - 1)) { run++; continue; }
+        if (run != blk) return 1;
+        run = 1;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------------------------
+struct Acc {
+    u64 cyc = 0, done = 0, maxfin = 0, turns = 0, oncycle = 0, minkey = 0, nonstd = 0, unchk = 0;
+    std::vector<u64> hist, lenhist, dhist;
+    std::vector<u64> keys;   // cycle-minimum keys of on-cycle positions (for the distinct-key count)
+    Acc(int hb, int lb, int db) : hist(hb > 0 ? hb : 0, 0), lenhist(lb > 0 ? lb : 0, 0), dhist(db > 0 ? db : 0, 0) {}
+};
+
+struct Cfg { int n, mode, rule, hbins, lbins, dbins, distinct, blk; };
+
+template <int W> static void play(const St<W>& s, const Cfg& c, Acc& A) {
+    u64 rounds = 0, lam = 0; St<W> meet;
+    bool cyc = brent<W>(s, c.rule, rounds, lam, &meet);
+    A.done++;
+    A.turns += rounds;
+    if (!cyc) {
+        if (rounds > A.maxfin) A.maxfin = rounds;
+        if (c.lbins) A.lenhist[rounds < (u64)(c.lbins - 1) ? rounds : (u64)(c.lbins - 1)]++;
+        return;
+    }
+    A.cyc++;
+    if (c.hbins) A.hist[lam < (u64)(c.hbins - 1) ? lam : (u64)(c.hbins - 1)]++;
+    if (c.mode == ORC_CANON && (c.dbins || c.distinct)) {
+        if constexpr (W == 4) {
+            // war_gpu.py:224-234 (weights) and distinct_cycles.c:184-197 (minimum key)
+            St<4> v = s; u64 c1n = 0, c2n = 0; int sym = 0;
+            u64 K = ~0ULL;
+            for (u64 t = 0; t < lam; t++) {
+                if (v.la > 0 && (int)(v.a & 15ULL) == c.n - 1) { c1n++; u64 k = key4(v); if (k < K) K = k; }
+                if (v.lb > 0 && (int)(v.b & 15ULL) == c.n - 1) { c2n++; u64 k = key4(swapst(v)); if (k < K) K = k; }
+                if (v.a == s.b && v.b == s.a && v.la == s.lb) sym = 1;
+                step<4>(v, c.rule);
+            }
+            if (same<4>(v, s)) {
+                A.oncycle++;
+                if (c.dbins) {
+                    u64 d = sym ? c1n : c1n + c2n;
+                    A.dhist[d < (u64)(c.dbins - 1) ? d : (u64)(c.dbins - 1)]++;
+                }
+                if (c.distinct) {
+                    if (key4(s) == K) A.minkey++;
+                    A.keys.push_back(K);
+                }
+            }
+        }
+    }
+    if (c.blk) {
+        int shp = loop_shape<W>(meet, lam, c.blk, c.rule);
+        if (shp == 1) A.nonstd++; else if (shp == 2) A.unchk++;
+    }
+}
+
+template <int W> static void run_chunk(const Cfg& c, u64 lo, u64 hi, Acc& A) {
+    int n = c.n, p[32];
+    if (c.mode == ORC_CANON) {
+        int m = n - 1;
+        u64 r = lo / (u64)m; int k = (int)(lo % (u64)m);
+        unrank(r, m, p);
+        for (u64 pos = lo; pos < hi; pos++) {
+            play<W>(canon_pos<W>(p, n, k), c, A);
+            if (++k == m) { k = 0; std::next_permutation(p, p + m); }
+        }
+        return;
+    }
+    unrank(lo, n, p);
+    for (u64 r = lo; r < hi; r++) {
+        bool take = true;
+        if (c.mode == ORC_HALF) {   // war_gpu.py:151-154: keep deals where card n-1 goes to player 1
+            int pos = 0; while (p[pos] != n - 1) pos++;
+            take = (pos & 1) == 0;
+        }
+        if (take) play<W>(deal<W>(p, n), c, A);
+        std::next_permutation(p, p + n);
+    }
+}
+
+static u64 range_total(int n, int mode) {
+    if (n < 1) return 0;
+    if (mode == ORC_CANON) return n < 2 ? 0 : fact_u64(n - 1) * (u64)(n - 1);
+    return fact_u64(n);
+}
+
+static std::string g_err;
+
+extern "C" const char* orc_last_error(void) { return g_err.c_str(); }
+
+extern "C" u64 orc_total(int n, int mode) { return range_total(n, mode); }
+
+// Play every position in [lo, hi) (deal ranks, or canon positions). 0 on success.
+extern "C" int orc_run(int n, int mode, int rule, u64 lo, u64 hi, int threads,
+                       int distinct, int shape_blk, orc_res* res) {
+    if (!res) { g_err = "null result"; return 1; }
+    if (n < 1 || n > 20) { g_err = "n must be in [1, 20]"; return 2; }
+    if (mode < 0 || mode > 2) { g_err = "bad mode"; return 3; }
+    if (rule != 0 && rule != 1) { g_err = "rule must be 0 or 1"; return 4; }
+    u64 total = range_total(n, mode);
+    if (lo > hi || hi > total) { g_err = "range outside [0, total]"; return 5; }
+    if (mode == ORC_HALF && (n & 1)) { g_err = "half mode needs even n"; return 6; }
+    if ((distinct || res->dbins) && (mode != ORC_CANON || n > 15)) { g_err = "distinct needs canon mode and n <= 15"; return 7; }
+    if (shape_blk && mode != ORC_CANON) { g_err = "shape needs canon mode"; return 8; }
+    Cfg c{n, mode, rule, res->hist ? res->hbins : 0, res->lenhist ? res->lbins : 0,
+          res->dhist ? res->dbins : 0, distinct, shape_blk};
+    if (threads > 0) omp_set_num_threads(threads);
+    const u64 CH = (mode == ORC_CANON) ? (u64)std::max(1, n - 1) * 512 : 4096;
+    u64 nchunks = (hi - lo + CH - 1) / CH;   // 0 for an empty range
+    Acc G(c.hbins, c.lbins, c.dbins);
+    #pragma omp parallel
+    {
+        Acc A(c.hbins, c.lbins, c.dbins);
+        #pragma omp for schedule(dynamic, 1)
+        for (long long ci = 0; ci < (long long)nchunks; ci++) {
+            u64 a = lo + (u64)ci * CH, b = std::min(hi, a + CH);
+            if (n <= 16) run_chunk<4>(c, a, b, A); else run_chunk<5>(c, a, b, A);
+        }
+        #pragma omp critical
+        {
+            G.cyc += A.cyc; G.done += A.done; G.turns += A.turns; G.oncycle += A.oncycle;
+            G.minkey += A.minkey; G.nonstd += A.nonstd; G.unchk += A.unchk;
+            G.maxfin = std::max(G.maxfin, A.maxfin);
+            for (size_t i = 0; i < G.hist.size(); i++) G.hist[i] += A.hist[i];
+            for (size_t i = 0; i < G.lenhist.size(); i++) G.lenhist[i] += A.lenhist[i];
+            for (size_t i = 0; i < G.dhist.size(); i++) G.dhist[i] += A.dhist[i];
+            G.keys.insert(G.keys.end(), A.keys.begin(), A.keys.end());
+        }
+    }
+    std::sort(G.keys.begin(), G.keys.end());
+    u64 nk = (u64)(std::unique(G.keys.begin(), G.keys.end()) - G.keys.begin());
+    res->cyc = G.cyc; res->done = G.done; res->maxfin = G.maxfin; res->turns = G.turns;
+    res->oncycle = G.oncycle; res->distinct_minkey = G.minkey; res->distinct_keys = nk;
+    res->shape_nonstd = G.nonstd; res->shape_unchk = G.unchk;
+    for (int i = 0; i < c.hbins; i++) res->hist[i] = G.hist[(size_t)i];
+    for (int i = 0; i < c.lbins; i++) res->lenhist[i] = G.lenhist[(size_t)i];
+    for (int i = 0; i < c.dbins; i++) res->dhist[i] = G.dhist[(size_t)i];
+    return 0;
+}
+
+// One position given as two piles (top card first). Returns 1 if endless, 0 if it ends.
+// rounds / lam as in orc_run. Used for known-answer tests on single published positions.
+extern "C" int orc_play(int n, const int* p1, int l1, const int* p2, int l2, int rule,
+                        u64* rounds, u64* lam) {
+    u64 r = 0, L = 0;
+    bool cyc;
+    if (n <= 16) {
+        St<4> s; s.a = 0; s.b = 0; s.la = l1; s.lb = l2;
+        for (int i = 0; i < l1; i++) s.a |= (u64)p1[i] << (4 * i);
+        for (int i = 0; i < l2; i++) s.b |= (u64)p2[i] << (4 * i);
+        cyc = brent<4>(s, rule, r, L, nullptr);
+    } else {
+        St<5> s; s.a = 0; s.b = 0; s.la = l1; s.lb = l2;
+        for (int i = 0; i < l1; i++) s.a |= (u128)p1[i] << (5 * i);
+        for (int i = 0; i < l2; i++) s.b |= (u128)p2[i] << (5 * i);
+        cyc = brent<5>(s, rule, r, L, nullptr);
+    }
+    if (rounds) *rounds = r;
+    if (lam) *lam = cyc ? L : 0;
+    return cyc ? 1 : 0;
+}
+
+// Lexicographic unrank, exposed for the permutation tests.
+extern "C" void orc_unrank(u64 r, int m, int* p) { unrank(r, m, p); }
+
+#ifdef ORC_MAIN
+static void usage() {
+    fprintf(stderr, "usage: war_oracle n [--mode full|half|canon] [--rule 0|1] [--lo L] [--hi H]\n"
+                    "                  [--hist B] [--lbins B] [--dbins B] [--distinct] [--shape] [--threads T]\n");
+}
+int main(int argc, char** argv) {
+    if (argc < 2) { usage(); return 2; }
+    int n = atoi(argv[1]), mode = ORC_FULL, rule = 0, threads = 0, hb = 0, lb = 0, db = 0, distinct = 0, shape = 0;
+    long long lo = 0, hi = -1;
+    for (int i = 2; i < argc; i++) {
+        std::string a = argv[i];
+        auto nxt = [&]() { if (i + 1 >= argc) { usage(); exit(2); } return std::string(argv[++i]); };
+        if (a == "--mode") { std::string m = nxt(); mode = m == "canon" ? ORC_CANON : m == "half" ? ORC_HALF : ORC_FULL; }
+        else if (a == "--rule") rule = atoi(nxt().c_str());
+        else if (a == "--lo") lo = atoll(nxt().c_str());
+        else if (a == "--hi") hi = atoll(nxt().c_str());
+        else if (a == "--hist") hb = atoi(nxt().c_str());
+        else if (a == "--lbins") lb = atoi(nxt().c_str());
+        else if (a == "--dbins") db = atoi(nxt().c_str());
+        else if (a == "--distinct") distinct = 1;
+        else if (a == "--shape") shape = 1;
+        else if (a == "--threads") threads = atoi(nxt().c_str());
+        else { usage(); return 2; }
+    }
+    u64 total = range_total(n, mode);
+    if (hi < 0) hi = (long long)total;
+    std::vector<u64> H(hb), L(lb), D(db);
+    orc_res r{}; r.hist = H.data(); r.hbins = hb; r.lenhist = L.data(); r.lbins = lb; r.dhist = D.data(); r.dbins = db;
+    int blk = shape ? (n & -n) : 0;
+    double t0 = omp_get_wtime();
+    int rc = orc_run(n, mode, rule, (u64)lo, (u64)hi, threads, distinct, blk, &r);
+    double el = omp_get_wtime() - t0;
+    if (rc) { fprintf(stderr, "error: %s\n", orc_last_error()); return 1; }
+    printf("n=%d mode=%d rule=%d range=[%lld,%lld) done=%llu cycling=%llu longest_finite=%llu turns=%llu "
+           "threads=%d time=%.3fs\n", n, mode, rule, lo, hi, r.done, r.cyc, r.maxfin, r.turns,
+           threads > 0 ? threads : omp_get_max_threads(), el);
+    if (distinct || db)
+        printf("   oncycle=%llu distinct_minkey=%llu distinct_keys=%llu\n", r.oncycle, r.distinct_minkey, r.distinct_keys);
+    if (shape) printf("   shape blk=%d nonstd=%llu unchecked=%llu\n", blk, r.shape_nonstd, r.shape_unchk);
+    if (hb) { printf("   hist:"); for (int i = 0; i < hb; i++) if (H[i]) printf(" %d:%llu", i, H[i]); printf("\n"); }
+    if (lb) { printf("   lenhist:"); for (int i = 0; i < lb; i++) if (L[i]) printf(" %d:%llu", i, L[i]); printf("\n"); }
+    if (db) { printf("   dhist:"); for (int i = 0; i < db; i++) if (D[i]) printf(" %d:%llu", i, D[i]); printf("\n"); }
+    return 0;
+}
+#endif

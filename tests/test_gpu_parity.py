@@ -1,0 +1,126 @@
+"""Parity of the HIP engine (through the C ABI) with the CPU oracle and the published goldens.
+Integer counts: bit-exact, tolerance 0."""
+import random
+
+import pytest
+
+from war_amd import WarError
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("cyc", "done", "maxfin", "turns", "hist", "lenhist")
+
+
+def check(eng, orc, n, mode, lo, hi, rule=0, kernel="persistent", fields=FIELDS, tuning=None, **kw):
+    want = orc.run(n, mode, lo, hi, rule=rule, hbins=256, lbins=512, **kw)
+    got = eng.run_range(n, mode, lo, hi, rule=rule, kernel=kernel, hbins=256, lbins=512, tuning=tuning, **kw)
+    for k in fields:
+        assert got[k] == want[k], (n, mode, lo, hi, rule, kernel, k, got[k], want[k])
+    return got, want
+
+
+@pytest.mark.parametrize("kernel", ["plain", "persistent"])
+@pytest.mark.parametrize("rule", [0, 1])
+@pytest.mark.parametrize("mode", ["full", "canon"])
+@pytest.mark.parametrize("n", list(range(1, 11)))
+def test_whole_range_small_n(eng, orc, n, mode, rule, kernel):
+    print("case", n, mode, rule, kernel, flush=True)
+    check(eng, orc, n, mode, 0, None, rule=rule, kernel=kernel)
+
+
+@pytest.mark.parametrize("kernel", ["plain", "persistent"])
+@pytest.mark.parametrize("n,mode", [(11, "full"), (12, "full"), (13, "full"), (14, "full"), (15, "full"),
+                                    (16, "full"), (11, "canon"), (12, "canon"), (13, "canon"), (14, "canon"),
+                                    (15, "canon"), (16, "canon")])
+def test_sampled_ranges(eng, orc, n, mode, kernel):
+    rng = random.Random(1000 * n + len(mode))
+    total = orc.total(n, mode)
+    for _ in range(3):
+        ln = rng.choice([1, 63, 64, 65, 1000, 1 << 16, 300_001])
+        lo = rng.randrange(0, total - ln)
+        print("case", n, mode, kernel, lo, ln, flush=True)
+        check(eng, orc, n, mode, lo, lo + ln, kernel=kernel)
+    # the last positions of the range
+    check(eng, orc, n, mode, total - 5000, total, kernel=kernel)
+
+
+@pytest.mark.parametrize("n", [4, 8, 10, 12])
+def test_half_mode(eng, orc, n):
+    total = orc.total(n, "full")
+    lo, hi = (0, total) if n <= 10 else (total // 3, total // 3 + 2_000_000)
+    for kernel in ("plain", "persistent"):
+        check(eng, orc, n, "half", lo, hi, kernel=kernel)
+
+
+@pytest.mark.parametrize("n", [7, 9, 11])
+def test_distinct_shape_whole_range(eng, orc, goldens, n):
+    got, want = check(eng, orc, n, "canon", 0, None, dbins=4096, distinct=True, shape_blk=n & -n,
+                      fields=FIELDS + ("oncycle", "distinct_minkey", "dhist", "shape_nonstd", "shape_unchk"))
+    assert got["distinct_hashset"] == want["distinct_keys"] == goldens["canon"][str(n)]["cycles"]
+
+
+@pytest.mark.parametrize("n", [11, 13])
+def test_distinct_partial_ranges(eng, orc, n):
+    rng = random.Random(n)
+    total = orc.total(n, "canon")
+    for _ in range(3):
+        lo = rng.randrange(0, total - 3_000_000)
+        got, want = check(eng, orc, n, "canon", lo, lo + 3_000_000, dbins=4096, distinct=True,
+                          fields=FIELDS + ("oncycle", "distinct_minkey", "dhist"))
+        assert got["distinct_hashset"] == want["distinct_keys"]
+
+
+@pytest.mark.parametrize("tuning", [
+    {"brent_cap": 7, "ranks_per_refill": 64, "waves_per_cu": 4},
+    {"brent_cap": 31, "ranks_per_refill": 4096, "waves_per_cu": 32},
+    {"brent_cap": 127, "ranks_per_refill": 256, "waves_per_cu": 8},
+    {"brent_cap": 1023, "ranks_per_refill": 1024, "waves_per_cu": 16},
+])
+def test_schedules_are_exact(eng, orc, tuning):
+    check(eng, orc, 13, "canon", 123_456_789, 123_456_789 + 2_000_000, tuning=tuning)
+    check(eng, orc, 13, "full", 3_000_000_000, 3_000_000_000 + 1_000_000, tuning=tuning)
+
+
+def test_5bit_ranges(eng, orc):
+    for n, lo, ln in [(17, 10**14, 200_000), (17, 0, 100_000), (18, 5 * 10**15, 50_000), (20, 10**18, 20_000)]:
+        for kernel in ("plain", "persistent"):
+            print("case 5bit", n, lo, ln, kernel, flush=True)
+            check(eng, orc, n, "full", lo, lo + ln, kernel=kernel)
+    check(eng, orc, 17, "canon", 10**13, 10**13 + 100_000)
+
+
+def test_edges_and_errors(eng):
+    r = eng.run_range(9, "full", 100, 100)
+    assert r["done"] == 0 and r["cyc"] == 0
+    r = eng.run_range(1, "full")
+    assert (r["done"], r["cyc"], r["maxfin"]) == (1, 0, 0)
+    r = eng.run_range(2, "canon")
+    assert (r["done"], r["cyc"], r["maxfin"]) == (1, 0, 1)
+    for bad in [dict(n=0), dict(n=21), dict(n=9, hi=10**9), dict(n=9, mode="half"), dict(n=9, distinct=True),
+                dict(n=16, mode="canon", distinct=True), dict(n=9, hbins=5000),
+                dict(n=9, tuning={"brent_cap": 10})]:
+        kw = dict(bad)
+        n = kw.pop("n")
+        with pytest.raises(WarError):
+            eng.run_range(n, **kw)
+
+
+def test_n13_full_goldens(eng, goldens):
+    g = goldens["deals"]["13"]
+    r = eng.run_range(13, "full", hbins=256, lbins=512)
+    assert r["done"] == g["deals"] and r["cyc"] == g["never_end"] and r["maxfin"] == g["longest"]
+    assert {str(i): v for i, v in enumerate(r["hist"]) if v} == g["hist"]
+    assert sum(r["lenhist"]) == g["deals"] - g["never_end"]
+    print("n13 full persistent device_ms", r["device_ms"], flush=True)
+
+
+def test_n14_canon_goldens(eng, goldens):
+    g = goldens["long"]["canon14"]
+    r = eng.run_range(14, "canon", hbins=256, lbins=512, dbins=4096, distinct=True, shape_blk=2)
+    assert r["done"] == g["positions"] and r["cyc"] == g["cycling"] and r["maxfin"] == g["longest"]
+    assert {str(i): v for i, v in enumerate(r["hist"]) if v} == g["hist"]
+    assert {str(i): v for i, v in enumerate(r["dhist"]) if v} == g["weights"]
+    assert (r["shape_nonstd"], r["shape_unchk"]) == (g["shape"]["nonstd"], g["shape"]["unchk"])
+    assert r["distinct_minkey"] == r["distinct_hashset"] == g["distinct"]
+    assert r["oncycle"] == goldens["canon"]["14"]["on_cycle"]
+    print("n14 canon persistent device_ms", r["device_ms"], flush=True)
