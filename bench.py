@@ -1,0 +1,233 @@
+"""Benchmark of the MI355X War engine; prints ONE JSON line.
+
+Headline workload (value): the n = 13 full enumeration, every one of the 13! = 6,227,020,800
+deals, played to its end or to its cycle by the persistent batched kernel, counts checked
+bit-exactly against the published a(13) = 4,880,546,113, longest 57 and cycle-length histogram.
+One step = one whole enumeration. Beside it, on the same GPU in the same run:
+  plain_kernel   the one-deal-per-lane kernel on the same n = 13 enumeration;
+  canon14        the n = 14 canonical scan (13! * 13 positions), persistent kernel, against the
+                 published counts, and both kernels on the same sampled chunks (the batched/plain
+                 ratio);
+  serving        a continuous-batching run of mixed queries with p50/p99 latency, beside the same
+                 queries one by one through the plain kernel;
+  roofline       integer VALU: useful lane-ops/s (Brent turns x VALU ops per turn of the
+                 age-synchronous loop) against CUs x 64 lanes x clock;
+  cpu_baseline   the CPU oracle (OpenMP) on a bounded sample of the same n = 13 workload.
+
+  python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--only main|canon14] [--no-extras]
+"""
+import argparse
+import json
+import math
+import os
+import random
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "endless-games-of-war_amd"))
+
+METRIC = "deals/sec (n=13 full, n=14 canon), bit-exact; batched vs plain kernel; p99 latency"
+PUBLISHED_N13_DEALS_PER_S = 6227020800 / 15.3      # RTX 5080, war_gpu.py 13 (reference data/checks.txt:23)
+A13 = {"cyc": 4880546113, "maxfin": 57, "hist": {42: 1475465743, 56: 2394985450, 70: 1010094920}}
+C14 = {"cyc": 2872129173, "maxfin": 465, "hist": {96: 43669233, 112: 2828459940}}
+# VALU instructions per War turn in the age-synchronous loop (turn + end test + three repeat
+# compares), counted in the gfx950 assembly of k_stage; the algorithmic unit is one Brent turn.
+VALU_OPS_PER_TURN = 24
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def hist_of(r):
+    return {i: v for i, v in enumerate(r["hist"]) if v}
+
+
+def check_n13(r):
+    assert r["done"] == math.factorial(13), r["done"]
+    assert r["cyc"] == A13["cyc"] and r["maxfin"] == A13["maxfin"] and hist_of(r) == A13["hist"], r
+    assert r["undetermined"] == 0
+
+
+def chunks_spread(total, count, size, seed):
+    rng = random.Random(seed)
+    step = total // count
+    out = []
+    for i in range(count):
+        lo = i * step + rng.randrange(0, step - size)
+        out.append((lo, lo + size))
+    return out
+
+
+def cpu_baseline(n, mode, chunks, threads):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    oracle.lib()
+    t0 = time.time()
+    res = []
+    for lo, hi in chunks:
+        res.append(oracle.run(n, mode, lo, hi, hbins=256, threads=threads))
+    el = time.time() - t0
+    return res, el
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--only", choices=["main", "canon14", "plain13"], default=None,
+                    help="run one workload only (for profiling)")
+    ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--cpu-chunks", type=int, default=64)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py: this engine runs on one GPU; --gpus %d is not supported (the rank space would be "
+              "split across GPUs with one all-reduce of the counters, not built here)" % a.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    from war_amd import Engine
+    eng = Engine(0)
+    dev = eng.device_info()
+    log("device", dev)
+    n13 = math.factorial(13)
+    tun = eng.get_tuning()
+
+    if a.only == "canon14":
+        r = eng.run_range(14, "canon", hbins=256)
+        assert r["cyc"] == C14["cyc"] and r["maxfin"] == C14["maxfin"] and hist_of(r) == C14["hist"]
+        log("canon14 persistent device_ms %.1f" % r["device_ms"])
+        return
+    if a.only == "plain13":
+        r = eng.run_range(13, "full", kernel="plain", hbins=256)
+        check_n13(r)
+        log("n13 plain device_ms %.1f" % r["device_ms"])
+        return
+
+    # ---- headline: n = 13 full enumeration, persistent kernel
+    for _ in range(a.warmup):
+        check_n13(eng.run_range(13, "full", hbins=256))
+    t0 = time.perf_counter()
+    dev_ms = 0.0
+    turns = 0
+    for _ in range(a.steps):
+        r = eng.run_range(13, "full", hbins=256, lbins=512)
+        check_n13(r)
+        dev_ms += r["device_ms"]
+        turns += r["turns"]
+    wall = time.perf_counter() - t0
+    value = n13 * a.steps / wall
+    peak_lane_ops = dev["cus"] * 64 * dev["max_clock_mhz"] * 1e6
+    achieved = turns * VALU_OPS_PER_TURN / (dev_ms / 1e3)
+    out = {
+        "metric": METRIC, "value": value, "unit": "deals/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": 1e3 * wall / a.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": value / PUBLISHED_N13_DEALS_PER_S, "dtype": "u64",
+        "data": "synthetic: exhaustive enumeration of all 13! deals (no input data)",
+        "config": {"workload": "n13_full: all 13! = 6227020800 deals, persistent batched kernel",
+                   "n": 13, "mode": "full", "rule": 0, "positions": n13, "kernel": "persistent", "tuning": tun,
+                   "parity": "a(13), longest 57, cycle-length histogram equal the published values every step"},
+        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": peak_lane_ops / 1e12,
+                     "unit": "Tlane-op/s", "frac": achieved / peak_lane_ops, "traffic": None,
+                     "turns_per_step": turns // a.steps, "valu_ops_per_turn": VALU_OPS_PER_TURN,
+                     "device_ms_per_step": dev_ms / a.steps,
+                     "note": "achieved = Brent turns x VALU ops per turn / device time (HIP events)"},
+    }
+    log("n13 persistent: %.3e deals/s, %.1f ms/step" % (value, 1e3 * wall / a.steps))
+
+    if not a.no_extras:
+        # ---- plain kernel on the same workload
+        t0 = time.perf_counter()
+        r = eng.run_range(13, "full", kernel="plain", hbins=256)
+        pw = time.perf_counter() - t0
+        check_n13(r)
+        out["plain_kernel"] = {"workload": "n13_full", "value": n13 / pw, "unit": "deals/s",
+                               "device_ms": r["device_ms"], "persistent_over_plain": value / (n13 / pw)}
+        log("n13 plain: %.3e deals/s" % (n13 / pw))
+
+        # ---- n = 14 canonical scan, persistent, whole range
+        t0 = time.perf_counter()
+        r = eng.run_range(14, "canon", hbins=256)
+        cw = time.perf_counter() - t0
+        assert r["cyc"] == C14["cyc"] and r["maxfin"] == C14["maxfin"] and hist_of(r) == C14["hist"]
+        tot14 = math.factorial(13) * 13
+        canon = {"workload": "n14_canon: all 13!*13 = 80951270400 positions", "persistent_value": tot14 / cw,
+                 "unit": "positions/s", "persistent_device_ms": r["device_ms"],
+                 "parity": "cycling 2872129173, longest 465, hist {96,112} equal the published values"}
+        # both kernels on the same sampled chunks (seed 14): batched / plain ratio
+        ch = chunks_spread(tot14, 16, 1 << 24, 14)
+        tp = tq = 0.0
+        for lo, hi in ch:
+            rp = eng.run_range(14, "canon", lo, hi, kernel="plain")
+            rq = eng.run_range(14, "canon", lo, hi, kernel="persistent")
+            assert (rp["cyc"], rp["maxfin"], rp["turns"]) == (rq["cyc"], rq["maxfin"], rq["turns"])
+            tp += rp["device_ms"]; tq += rq["device_ms"]
+        npos = sum(hi - lo for lo, hi in ch)
+        canon.update({"sample": "16 chunks of 2^24 positions, seed 14", "plain_sample_value": npos / (tp / 1e3),
+                      "persistent_sample_value": npos / (tq / 1e3), "persistent_over_plain": tp / tq})
+        out["canon14"] = canon
+        log("canon14: %.3e pos/s persistent (whole), ratio %.2f on samples" % (tot14 / cw, tp / tq))
+
+        # ---- serving: continuous batching of mixed queries
+        out["serving"] = serving(eng)
+
+        # ---- CPU baseline: the oracle on sampled n = 13 chunks, compared chunk by chunk
+        threads = min(16, os.cpu_count() or 1)
+        ch = chunks_spread(n13, a.cpu_chunks, 1 << 22, 13)
+        cres, cel = cpu_baseline(13, "full", ch, threads)
+        for (lo, hi), c in zip(ch, cres):
+            g = eng.run_range(13, "full", lo, hi, hbins=256)
+            assert (g["cyc"], g["maxfin"], g["turns"], g["hist"]) == (c["cyc"], c["maxfin"], c["turns"], c["hist"])
+        out["cpu_baseline"] = {"value": sum(hi - lo for lo, hi in ch) / cel, "unit": "deals/s", "cores": threads,
+                               "kind": "port",
+                               "sample": "%d chunks of 2^22 n=13 deal ranks spread over 13! (seed 13), "
+                                         "each equal to the GPU's counts" % len(ch)}
+        log("cpu baseline: %.3e deals/s on %d threads" % (out["cpu_baseline"]["value"], threads))
+    print(json.dumps(out), flush=True)
+
+
+def serving(eng, nreq=64, seed=20261203):
+    """Poisson arrivals of (n, range, mode) queries; p50/p99 latency and aggregate positions/s,
+    beside the same queries one by one through the plain kernel."""
+    rng = random.Random(seed)
+    qs = []
+    for _ in range(nreq):
+        n = rng.randint(7, 17)
+        mode = rng.choice(["full", "canon"])
+        total = math.factorial(n - 1) * (n - 1) if mode == "canon" else math.factorial(n)
+        ln = min(total, int(10 ** rng.uniform(3, 8.5)))
+        lo = rng.randrange(0, total - ln + 1)
+        distinct = mode == "canon" and n <= 13 and rng.random() < 0.3
+        qs.append((n, mode, lo, lo + ln, distinct))
+    eng.serve_start(max_slots=64, step_positions=1 << 26)
+    ids = []
+    t0 = time.perf_counter()
+    rate = 200.0     # arrivals per second
+    for q in qs:
+        time.sleep(rng.expovariate(rate))
+        n, mode, lo, hi, d = q
+        ids.append(eng.submit(n, mode, lo, hi, hbins=128, distinct=d))
+    res = [eng.wait(i) for i in ids]
+    wall = time.perf_counter() - t0
+    eng.serve_stop()
+    lat = sorted(r["latency_ms"] for r in res)
+    # the same queries alone through the plain kernel: equal counts, summed latency
+    tplain = 0.0
+    for q, r in zip(qs, res):
+        n, mode, lo, hi, d = q
+        t1 = time.perf_counter()
+        p = eng.run_range(n, mode, lo, hi, kernel="plain", hbins=128, distinct=d)
+        tplain += time.perf_counter() - t1
+        assert (p["cyc"], p["done"], p["maxfin"], p["turns"], p["hist"]) == \
+            (r["cyc"], r["done"], r["maxfin"], r["turns"], r["hist"]), q
+    npos = sum(hi - lo for _, _, lo, hi, _ in qs)
+    return {"requests": nreq, "arrival_rate_per_s": rate, "p50_ms": lat[len(lat) // 2],
+            "p99_ms": lat[min(len(lat) - 1, int(0.99 * len(lat)))], "positions_per_s": npos / wall,
+            "plain_one_by_one_positions_per_s": npos / tplain,
+            "queries": "n uniform 7..17, full or canon (distinct for 30% of canon n<=13), length log-uniform "
+                       "[1e3, 10^8.5], Poisson arrivals; every count equal to the plain kernel's"}
+
+
+if __name__ == "__main__":
+    main()
