@@ -94,6 +94,11 @@ def main():
     n13 = math.factorial(13)
     tun = eng.get_tuning()
 
+    if a.only == "main":
+        r = eng.run_range(13, "full", hbins=256)
+        check_n13(r)
+        log("n13 persistent device_ms %.1f" % r["device_ms"])
+        return
     if a.only == "canon14":
         r = eng.run_range(14, "canon", hbins=256)
         assert r["cyc"] == C14["cyc"] and r["maxfin"] == C14["maxfin"] and hist_of(r) == C14["hist"]

@@ -67,6 +67,7 @@ u64 range_total(int n, int mode) {
 
 constexpr int MAXSLOTS = 256;
 constexpr int MAXJOBS = 1024;
+constexpr size_t JOBS_OFF = 256;   // byte offset of the Job array after the JobHdr
 
 // Per request: device counters and histograms (resident until the result is copied out).
 struct ReqHost {
@@ -84,15 +85,23 @@ struct ReqHost {
     size_t bytes = 0;
 };
 
-struct QueueBufs {              // device buffers of one pile layout
+// Device buffers of one pile layout: two region buffers (ping-pong between stages) of `cap` slots
+// with their per-tile counts, and the cycle-record regions with their per-tile counts.
+struct QueueBufs {
     void *qa[2] = {nullptr, nullptr}, *qb[2] = {nullptr, nullptr};
     u32 *qreq[2] = {nullptr, nullptr};
     u64 *qpos[2] = {nullptr, nullptr};
+    u32 *qhome[2] = {nullptr, nullptr};
+    u32 *qcnt[2] = {nullptr, nullptr};
     void *ca = nullptr, *cb = nullptr;
-    u32 *cmeta = nullptr, *creq = nullptr, *clam = nullptr;
+    u32 *cmeta = nullptr, *creq = nullptr, *clam = nullptr, *ccnt = nullptr;
     u64 *cpos = nullptr;
-    u64 cap = 0, ccap = 0;
+    u64 cap = 0, tmax = 0;
+    bool have_c = false;
 };
+
+constexpr u64 MIN_TILE = 64, MAX_TILE = 8192;
+constexpr u32 PLAIN_CYC_TILE = 1024;    // home-tile size of the plain kernel's cycle records
 
 struct GraphKey {
     int W, rule, kernel, wpc, per, cap0, walk, multi, hb, lb;
@@ -109,7 +118,7 @@ struct Engine {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int cus = 0, clock_mhz = 0;
     std::string arch;
-    war_tuning tun{16, 1024, 15, 32};
+    war_tuning tun{32, 1024, 15, 32};
     u64 batch = 1ull << 28;          // positions per persistent batch (= queue capacity)
     Pipe *pipe = nullptr;
     JobHdr *hdr = nullptr;           // device: JobHdr followed by MAXJOBS Jobs
@@ -240,22 +249,28 @@ template <int W> int ensure_queues(Engine &E, bool walk) {
     typedef typename Lay<W>::T T;
     QueueBufs &Q = qbufs<W>(E);
     if (!Q.cap) {
+        const u64 cap = E.batch + (u64)MAXJOBS * MAX_TILE;   // T * tile_pos <= batch + jobs * tile_pos
+        const u64 tmax = cap / MIN_TILE + MAXJOBS;
         for (int i = 0; i < 2; i++) {
-            HIPCHK(hipMalloc(&Q.qa[i], E.batch * sizeof(T)));
-            HIPCHK(hipMalloc(&Q.qb[i], E.batch * sizeof(T)));
-            HIPCHK(hipMalloc((void **)&Q.qreq[i], E.batch * sizeof(u32)));
-            HIPCHK(hipMalloc((void **)&Q.qpos[i], E.batch * sizeof(u64)));
+            HIPCHK(hipMalloc(&Q.qa[i], cap * sizeof(T)));
+            HIPCHK(hipMalloc(&Q.qb[i], cap * sizeof(T)));
+            HIPCHK(hipMalloc((void **)&Q.qreq[i], cap * sizeof(u32)));
+            HIPCHK(hipMalloc((void **)&Q.qpos[i], cap * sizeof(u64)));
+            HIPCHK(hipMalloc((void **)&Q.qhome[i], cap * sizeof(u32)));
+            HIPCHK(hipMalloc((void **)&Q.qcnt[i], tmax * sizeof(u32)));
         }
-        Q.cap = E.batch;
+        Q.cap = cap;
+        Q.tmax = tmax;
     }
-    if (walk && !Q.ccap) {
-        HIPCHK(hipMalloc(&Q.ca, E.batch * sizeof(T)));
-        HIPCHK(hipMalloc(&Q.cb, E.batch * sizeof(T)));
-        HIPCHK(hipMalloc((void **)&Q.cmeta, E.batch * sizeof(u32)));
-        HIPCHK(hipMalloc((void **)&Q.creq, E.batch * sizeof(u32)));
-        HIPCHK(hipMalloc((void **)&Q.clam, E.batch * sizeof(u32)));
-        HIPCHK(hipMalloc((void **)&Q.cpos, E.batch * sizeof(u64)));
-        Q.ccap = E.batch;
+    if (walk && !Q.have_c) {
+        HIPCHK(hipMalloc(&Q.ca, Q.cap * sizeof(T)));
+        HIPCHK(hipMalloc(&Q.cb, Q.cap * sizeof(T)));
+        HIPCHK(hipMalloc((void **)&Q.cmeta, Q.cap * sizeof(u32)));
+        HIPCHK(hipMalloc((void **)&Q.creq, Q.cap * sizeof(u32)));
+        HIPCHK(hipMalloc((void **)&Q.clam, Q.cap * sizeof(u32)));
+        HIPCHK(hipMalloc((void **)&Q.cpos, Q.cap * sizeof(u64)));
+        HIPCHK(hipMalloc((void **)&Q.ccnt, Q.tmax * sizeof(u32)));
+        Q.have_c = true;
     }
     return 0;
 }
@@ -265,6 +280,7 @@ template <int W> CycQueue<W> cyc_queue(Engine &E) {
     QueueBufs &Q = qbufs<W>(E);
     CycQueue<W> c;
     c.a = (T *)Q.ca; c.b = (T *)Q.cb; c.meta = Q.cmeta; c.req = Q.creq; c.lam = Q.clam; c.pos = Q.cpos;
+    c.cnt = Q.ccnt;
     return c;
 }
 
@@ -279,6 +295,22 @@ std::vector<u32> stage_caps(int cap0) {
     return c;
 }
 
+template <int W, int RULE, bool FRESH>
+void launch_stage_f(int flags, int blocks, size_t lds, hipStream_t st, const StageArgs<W> &A) {
+    switch (flags) {
+        case 0: hipLaunchKernelGGL((k_stage<W, RULE, FRESH, 0>), dim3(blocks), dim3(256), lds, st, A); break;
+        case F_MULTI: hipLaunchKernelGGL((k_stage<W, RULE, FRESH, F_MULTI>), dim3(blocks), dim3(256), lds, st, A); break;
+        case F_WALK: hipLaunchKernelGGL((k_stage<W, RULE, FRESH, F_WALK>), dim3(blocks), dim3(256), lds, st, A); break;
+        default: hipLaunchKernelGGL((k_stage<W, RULE, FRESH, F_MULTI | F_WALK>), dim3(blocks), dim3(256), lds, st, A); break;
+    }
+}
+
+template <int W, int RULE>
+void launch_stage(bool fresh, int flags, int blocks, size_t lds, hipStream_t st, const StageArgs<W> &A) {
+    if (fresh) launch_stage_f<W, RULE, true>(flags, blocks, lds, st, A);
+    else       launch_stage_f<W, RULE, false>(flags, blocks, lds, st, A);
+}
+
 template <int W, int RULE>
 int build_graph(Engine &E, const GraphKey &key, hipGraphExec_t *out) {
     typedef typename Lay<W>::T T;
@@ -287,27 +319,27 @@ int build_graph(Engine &E, const GraphKey &key, hipGraphExec_t *out) {
     const size_t lds = (size_t)4 * (key.hb + key.lb) * sizeof(u32);
     const std::vector<u32> caps = stage_caps(key.cap0);
     HIPCHK(hipStreamBeginCapture(E.st, hipStreamCaptureModeThreadLocal));
-    hipLaunchKernelGGL(k_setup, dim3(1), dim3(64), 0, E.st, E.pipe);
+    hipLaunchKernelGGL(k_setup, dim3(256), dim3(256), 0, E.st, E.pipe, (const JobHdr *)E.hdr,
+                       key.walk ? Q.ccnt : (u32 *)nullptr);
     u32 a0 = 0;
     for (size_t s = 0; s < caps.size(); s++) {
         StageArgs<W> A{};
         A.pipe = E.pipe; A.hdr = E.hdr; A.jobs = E.jobs; A.reqs = E.reqs;
         const int in = (int)((s + 1) & 1), out = (int)(s & 1);
         A.qin.a = (T *)Q.qa[in]; A.qin.b = (T *)Q.qb[in]; A.qin.req = Q.qreq[in]; A.qin.pos = Q.qpos[in];
+        A.qin.home = Q.qhome[in]; A.qin.cnt = Q.qcnt[in];
         A.qout.a = (T *)Q.qa[out]; A.qout.b = (T *)Q.qb[out]; A.qout.req = Q.qreq[out]; A.qout.pos = Q.qpos[out];
+        A.qout.home = Q.qhome[out]; A.qout.cnt = Q.qcnt[out];
         A.cq = cyc_queue<W>(E);
-        A.qcap = Q.cap; A.cqcap = key.walk ? Q.ccap : 0;
         A.per = (u32)key.per; A.tile_pos = 64u * (u32)key.per;
         A.a0 = a0; A.a1 = caps[s]; A.stage = (int)s;
         A.hb = (u32)key.hb; A.lb = (u32)key.lb;
-        A.need_req = (u32)key.multi; A.need_pos = (u32)key.walk;
-        if (s == 0) hipLaunchKernelGGL((k_stage<W, RULE, true>), dim3(blocks), dim3(256), lds, E.st, A);
-        else        hipLaunchKernelGGL((k_stage<W, RULE, false>), dim3(blocks), dim3(256), lds, E.st, A);
+        launch_stage<W, RULE>(s == 0, (key.multi ? F_MULTI : 0) | (key.walk ? F_WALK : 0), blocks, lds, E.st, A);
         a0 = caps[s];
     }
     if (key.walk) {
         WalkArgs<W> WA{};
-        WA.pipe = E.pipe; WA.reqs = E.reqs; WA.cq = cyc_queue<W>(E); WA.cqcap = Q.ccap;
+        WA.pipe = E.pipe; WA.hdr = E.hdr; WA.reqs = E.reqs; WA.cq = cyc_queue<W>(E); WA.tile_pos = 64u * (u32)key.per;
         hipLaunchKernelGGL((k_walk<W, RULE>), dim3(blocks), dim3(256), 0, E.st, WA);
     }
     hipGraph_t g = nullptr;
@@ -327,17 +359,18 @@ int run_persistent_batch(Engine &E, std::vector<Job> &jobs, const war_tuning &t,
     int rc = ensure_queues<W>(E, walk);
     if (rc) return rc;
     const u64 tile_pos = (u64)t.ranks_per_refill;
-    u64 tiles = 0;
-    for (Job &j : jobs) { j.tile_base = tiles; tiles += (j.hi - j.lo + tile_pos - 1) / tile_pos; }
+    u64 tiles = 0, npos = 0;
+    for (Job &j : jobs) { j.tile_base = tiles; tiles += (j.hi - j.lo + tile_pos - 1) / tile_pos; npos += j.hi - j.lo; }
+    GraphKey key{W, RULE, 1, t.waves_per_cu, t.ranks_per_refill / 64, t.brent_cap, walk ? 1 : 0, multi ? 1 : 0,
+                 round_pow2(hb), round_pow2(lb)};
+    if (tiles > qbufs<W>(E).tmax || tiles * tile_pos > qbufs<W>(E).cap) return fail(32, "batch exceeds the queue capacity");
     JobHdr h{};
     h.njobs = (u32)jobs.size(); h.req0 = jobs[0].req; h.ntiles = tiles;
     // copy the job list in (the staging buffer is free: the previous batch was synchronised)
     char *sp = (char *)E.stage_host;
     memcpy(sp, &h, sizeof h);
-    memcpy(sp + 64, jobs.data(), jobs.size() * sizeof(Job));
-    HIPCHK(hipMemcpyAsync(E.hdr, sp, 64 + jobs.size() * sizeof(Job), hipMemcpyHostToDevice, E.st));
-    GraphKey key{W, RULE, 1, t.waves_per_cu, t.ranks_per_refill / 64, t.brent_cap, walk ? 1 : 0, multi ? 1 : 0,
-                 round_pow2(hb), round_pow2(lb)};
+    memcpy(sp + JOBS_OFF, jobs.data(), jobs.size() * sizeof(Job));
+    HIPCHK(hipMemcpyAsync(E.hdr, sp, JOBS_OFF + jobs.size() * sizeof(Job), hipMemcpyHostToDevice, E.st));
     auto it = E.graphs.find(key);
     hipGraphExec_t ge = nullptr;
     if (it == E.graphs.end()) {
@@ -354,31 +387,38 @@ int run_persistent_batch(Engine &E, std::vector<Job> &jobs, const war_tuning &t,
     float m = 0;
     HIPCHK(hipEventElapsedTime(&m, E.ev0, E.ev1));
     if (ms) *ms = m;
+    (void)npos;
     return 0;
 }
 
 template <int W, int RULE>
 int run_plain_batch(Engine &E, ReqHost &h, u64 lo, u64 hi, const war_tuning &t, float *ms) {
-    typedef typename Lay<W>::T T;
     const bool walk = h.dev.walk != 0;
     int rc = ensure_queues<W>(E, walk);
     if (rc) return rc;
     PlainArgs<W> P{};
     P.lo = lo; P.hi = hi; P.per = (u32)t.plain_per;
-    P.hb = (u32)h.rq.hbins; P.lb = (u32)h.rq.lbins;
+    P.hb = (u32)h.rq.hbins; P.lb = (u32)h.rq.lbins; P.tile_pos = PLAIN_CYC_TILE;
     P.R = h.dev; P.req = (u32)h.slot; P.pipe = E.pipe;
-    P.cq = cyc_queue<W>(E); P.cqcap = walk ? qbufs<W>(E).ccap : 0;
+    P.cq = cyc_queue<W>(E);
+    // the walks read the cycle records by home tile: (pos - lo) / PLAIN_CYC_TILE
+    const u64 tiles = (hi - lo + PLAIN_CYC_TILE - 1) / PLAIN_CYC_TILE;
+    JobHdr jh{};
+    jh.njobs = 0; jh.req0 = (u32)h.slot; jh.ntiles = tiles;
+    memcpy(E.stage_host, &jh, sizeof jh);
+    HIPCHK(hipMemcpyAsync(E.hdr, E.stage_host, sizeof jh, hipMemcpyHostToDevice, E.st));
     const u64 threads = (hi - lo + P.per - 1) / P.per;
     const u64 blocks = (threads + 255) / 256;
     const size_t lds = (size_t)(P.hb + P.lb) * sizeof(u32);
     HIPCHK(hipEventRecord(E.ev0, E.st));
-    hipLaunchKernelGGL(k_setup, dim3(1), dim3(64), 0, E.st, E.pipe);
+    hipLaunchKernelGGL(k_setup, dim3(256), dim3(256), 0, E.st, E.pipe, (const JobHdr *)E.hdr,
+                       walk ? qbufs<W>(E).ccnt : (u32 *)nullptr);
     hipLaunchKernelGGL((k_plain<W, RULE>), dim3((unsigned)blocks), dim3(256), lds, E.st, P);
     HIPCHK(hipGetLastError());
     if (walk) {
         WalkArgs<W> WA{};
-        WA.pipe = E.pipe; WA.reqs = E.reqs; WA.cq = cyc_queue<W>(E); WA.cqcap = qbufs<W>(E).ccap;
-        hipLaunchKernelGGL((k_walk<W, RULE>), dim3(std::max(1, E.cus * 4)), dim3(256), 0, E.st, WA);
+        WA.pipe = E.pipe; WA.hdr = E.hdr; WA.reqs = E.reqs; WA.cq = cyc_queue<W>(E); WA.tile_pos = PLAIN_CYC_TILE;
+        hipLaunchKernelGGL((k_walk<W, RULE>), dim3(std::max(1, E.cus * 8)), dim3(256), 0, E.st, WA);
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipEventRecord(E.ev1, E.st));
@@ -582,10 +622,11 @@ int war_init(int32_t device, uint32_t flags) {
     HIPCHK(hipEventCreate(&E->ev1));
     HIPCHK(hipMalloc((void **)&E->pipe, sizeof(Pipe)));
     void *jb = nullptr;
-    HIPCHK(hipMalloc(&jb, 64 + MAXJOBS * sizeof(Job)));
+    static_assert(sizeof(JobHdr) <= JOBS_OFF, "job header too large");
+    HIPCHK(hipMalloc(&jb, JOBS_OFF + MAXJOBS * sizeof(Job)));
     E->hdr = (JobHdr *)jb;
-    E->jobs = (Job *)((char *)jb + 64);
-    HIPCHK(hipHostMalloc(&E->stage_host, 64 + MAXJOBS * sizeof(Job), hipHostMallocDefault));
+    E->jobs = (Job *)((char *)jb + JOBS_OFF);
+    HIPCHK(hipHostMalloc(&E->stage_host, JOBS_OFF + MAXJOBS * sizeof(Job), hipHostMallocDefault));
     HIPCHK(hipMalloc((void **)&E->reqs, MAXSLOTS * sizeof(ReqDev)));
     HIPCHK(hipMemsetAsync(E->reqs, 0, MAXSLOTS * sizeof(ReqDev), E->st));
     HIPCHK(hipMemsetAsync(E->pipe, 0, sizeof(Pipe), E->st));
@@ -606,8 +647,12 @@ int war_shutdown(void) {
         (void)hipStreamSynchronize(E->st);
         for (auto &kv : E->graphs) (void)hipGraphExecDestroy(kv.second);
         for (QueueBufs *Q : {&E->qb4, &E->qb5}) {
-            for (int i = 0; i < 2; i++) { (void)hipFree(Q->qa[i]); (void)hipFree(Q->qb[i]); (void)hipFree(Q->qreq[i]); (void)hipFree(Q->qpos[i]); }
-            (void)hipFree(Q->ca); (void)hipFree(Q->cb); (void)hipFree(Q->cmeta); (void)hipFree(Q->creq); (void)hipFree(Q->clam); (void)hipFree(Q->cpos);
+            for (int i = 0; i < 2; i++) {
+                (void)hipFree(Q->qa[i]); (void)hipFree(Q->qb[i]); (void)hipFree(Q->qreq[i]); (void)hipFree(Q->qpos[i]);
+                (void)hipFree(Q->qhome[i]); (void)hipFree(Q->qcnt[i]);
+            }
+            (void)hipFree(Q->ca); (void)hipFree(Q->cb); (void)hipFree(Q->cmeta); (void)hipFree(Q->creq); (void)hipFree(Q->clam);
+            (void)hipFree(Q->cpos); (void)hipFree(Q->ccnt);
         }
         (void)hipFree(E->pipe);
         (void)hipFree(E->hdr);

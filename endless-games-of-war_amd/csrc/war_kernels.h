@@ -41,24 +41,32 @@ struct ReqDev {
 };
 
 struct Job { u64 lo, hi, tile_base; u32 req, pad; };
-struct JobHdr { u32 njobs, req0; u64 ntiles; };   // req0: the request of job 0
+// Per batch: the job list and the number of tiles T.
+struct JobHdr {
+    u32 njobs, req0;        // req0: the request of job 0
+    u64 ntiles;
+};
 
 struct Pipe {
-    u64 tile_ctr;
-    u64 qcount[MAXSTAGE];   // records written by stage s
-    u64 qread[MAXSTAGE];    // read cursor of stage s over the records of stage s-1
-    u64 ccount, cread;      // cycle-record queue (walk requests)
+    u64 grab[MAXSTAGE];     // grab counter of stage s
+    u64 stotal[MAXSTAGE];   // records written by stage s (for the host's group sizing)
+    u64 wgrab, ctotal;      // walk grab counter; cycle records written
     u64 pad[6];
 };
 
-// Survivor records. The pile lengths and n ride in the unused top bits of the packed piles
-// (4-bit: an alive pile holds <= 15 cards = 60 bits; 5-bit: <= 19 cards = 95 bits), so a record is
-// 16 B (4-bit) or 32 B (5-bit). req / pos are written only when the batch needs them.
+// Survivor records live in per-tile regions of tile_pos slots: tile t's records are
+// a[t*tile_pos + 0 .. cnt[t]). A wave that takes a group of tiles writes its survivors densely into
+// the same group's span of the other buffer, so no append needs an atomic. The pile lengths and n
+// ride in the unused top bits of the packed piles (4-bit: an alive pile holds <= 15 cards = 60 bits;
+// 5-bit: <= 19 cards = 95 bits): a record is 16 B (4-bit) or 32 B (5-bit). req / pos / home are
+// written only when the batch needs them.
 template <int W> struct Queue {
     typedef typename Lay<W>::T T;
     T *a, *b;
     u32 *req;
     u64 *pos;
+    u32 *home;   // the tile the position was enumerated in (cycle records go to its region)
+    u32 *cnt;    // records per tile
 };
 
 template <int W> __device__ __forceinline__ void pack(const St<W> &s, u32 n, typename Lay<W>::T &qa, typename Lay<W>::T &qb) {
@@ -83,6 +91,7 @@ template <int W> __device__ __forceinline__ void unpack(typename Lay<W>::T qa, t
     }
 }
 
+// Cycle records (requests that need walks), in the region of their home tile.
 template <int W> struct CycQueue {
     typedef typename Lay<W>::T T;
     T *a, *b;    // meeting state
@@ -90,6 +99,7 @@ template <int W> struct CycQueue {
     u32 *req;
     u32 *lam;
     u64 *pos;    // position index (the start is rebuilt from it)
+    u32 *cnt;    // records per home tile (zeroed per batch)
 };
 
 template <int W> struct StageArgs {
@@ -99,12 +109,10 @@ template <int W> struct StageArgs {
     const ReqDev *reqs;
     Queue<W> qin, qout;
     CycQueue<W> cq;
-    u64 qcap, cqcap;
-    u32 tile_pos, per;     // FRESH: positions per tile, positions per lane (tile_pos = 64 * per)
+    u32 tile_pos, per;     // positions per tile (= slots per region), FRESH positions per lane
     u32 a0, a1;            // ages played: (a0, a1]
-    int stage;             // index of the counters this stage writes (qcount[stage])
+    int stage;
     u32 hb, lb;            // LDS histogram bins per wave
-    u32 need_req, need_pos;  // batch holds several requests / a request that needs cycle walks
 };
 
 // ------------------------------------------------------------------------------------------
@@ -128,9 +136,49 @@ __device__ __forceinline__ u64 wave_fetch_add(u64 *p, u64 v) {
     return uni64(old);
 }
 
-__global__ void k_setup(Pipe *pipe) {
+// Zero the pipeline counters and the per-tile cycle counts of one batch (grid-stride).
+__global__ void k_setup(Pipe *pipe, const JobHdr *hdr, u32 *ccnt) {
+    const u32 tid = blockIdx.x * blockDim.x + threadIdx.x, nth = gridDim.x * blockDim.x;
     u64 *p = reinterpret_cast<u64 *>(pipe);
-    for (u32 i = threadIdx.x; i < sizeof(Pipe) / 8; i += blockDim.x) p[i] = 0;
+    for (u32 i = tid; i < sizeof(Pipe) / 8; i += nth) p[i] = 0;
+    if (ccnt) {
+        const u64 T = hdr->ntiles;
+        for (u64 i = tid; i < T; i += nth) ccnt[i] = 0;
+    }
+}
+
+// Inclusive prefix sum over the 64 lanes.
+__device__ __forceinline__ u32 wave_scan(u32 v) {
+    const u32 lane = lane_id();
+    for (int off = 1; off < 64; off <<= 1) {
+        const u32 t = __shfl_up(v, off);
+        if (lane >= (u32)off) v += t;
+    }
+    return v;
+}
+
+// Tiles per grab so that a grab holds about 256 records (a power of two, at most 64).
+__device__ __forceinline__ u32 group_size(u64 records, u64 tiles, u32 tile_pos) {
+    u32 g = 1;
+    while (g < 64 && (u64)g * records < 256ull * tiles) g <<= 1;
+    (void)tile_pos;
+    return g;
+}
+
+// Position k of a group's record list -> (tile i within the group, offset in its region).
+// excl holds the exclusive prefix of the counts, lane j for tile j (j < ng). Must be called with all
+// lanes of the wave active (cross-lane reads).
+__device__ __forceinline__ void group_locate(u32 excl, u32 ng, u32 k, u32 &tile, u32 &off) {
+    u32 lo = 0, elo = __shfl(excl, 0);
+    if (ng > 1) {
+        for (u32 step = 32; step; step >>= 1) {
+            const u32 m = lo + step;
+            const u32 e = __shfl(excl, (int)(m < 63u ? m : 63u));
+            if (m < ng && e <= k) { lo = m; elo = e; }
+        }
+    }
+    tile = lo;
+    off = k - elo;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -139,11 +187,12 @@ struct WaveAcc {
     u32 req;                   // request slot these totals belong to (UNBOUNDED = none)
     u32 hb, lb;                // that request's bins
     u64 cyc, done, turns, maxfin, undet;
+    u64 pushed;                // cycle records this wave wrote (all requests)
     u32 *lh, *ll;              // LDS: hist[hbmax], lenhist[lbmax] of this wave
     bool dirty;
 
     __device__ void init(u32 *lds, u32 hbmax, u32 lbmax) {
-        req = UNBOUNDED; hb = lb = 0; cyc = done = turns = maxfin = undet = 0; dirty = false;
+        req = UNBOUNDED; hb = lb = 0; cyc = done = turns = maxfin = undet = 0; dirty = false; pushed = 0;
         lh = lds; ll = lds + hbmax;
         for (u32 i = lane_id(); i < hbmax + lbmax; i += 64) lds[i] = 0;
     }
@@ -185,25 +234,125 @@ struct WaveAcc {
     }
 };
 
-// Append the lanes of `mask` to the cycle-record queue.
+// Append the lanes of `mask` to the cycle records of their home tiles (one atomic per home tile
+// present in the mask; the counters are spread over the tiles, so they do not contend).
 template <int W>
-__device__ __forceinline__ void push_cycles(const StageArgs<W> &A, u64 mask, const St<W> &s, u32 n, u32 req,
-                                            u32 lam, u64 pos) {
-    const u32 c = __popcll(mask);
-    const u64 base = wave_fetch_add(&A.pipe->ccount, c);
-    if ((mask >> lane_id()) & 1ull) {
-        const u64 i = base + rank_in(mask);
-        if (i < A.cqcap) {
-            A.cq.a[i] = s.a; A.cq.b[i] = s.b; A.cq.meta[i] = s.sa | (n << 8);
-            A.cq.req[i] = req; A.cq.lam[i] = lam; A.cq.pos[i] = pos;
+__device__ __forceinline__ u32 push_cycles(const CycQueue<W> &cq, u32 tile_pos, u64 mask, const St<W> &s, u32 n,
+                                           u32 req, u32 lam, u64 pos, u32 home) {
+    const u32 lane = lane_id();
+    const u32 total = (u32)__popcll(mask);
+    while (mask) {
+        const u32 leader = (u32)(__ffsll((long long)mask) - 1);
+        const u32 h = lanev(home, leader);
+        const u64 m = ballot(home == h) & mask;
+        mask &= ~m;
+        u32 base = 0;
+        if (lane == leader) base = atomicAdd(&cq.cnt[h], (u32)__popcll(m));
+        base = lanev(base, leader);
+        if ((m >> lane) & 1ull) {
+            const u32 slot = base + rank_in(m);
+            if (slot < tile_pos) {
+                const u64 i = (u64)h * tile_pos + slot;
+                cq.a[i] = s.a; cq.b[i] = s.b; cq.meta[i] = s.sa | (n << 8);
+                cq.req[i] = req; cq.lam[i] = lam; cq.pos[i] = pos;
+            }
         }
+    }
+    return total;
+}
+
+// ------------------------------------------------------------------------------------------
+// The lockstep game of one wave for ages (a0, a1]: every alive lane starts at age a0 with its
+// tortoise equal to its state (a0 = 0 or 2^k - 1, where Brent resets). Ended / repeated lanes are
+// accounted (wave-uniform values: the age is the game length, age - R the cycle length); returns
+// the lanes still alive at a1 (or, for the last stage, at the safety stop).
+struct LaneCtx {
+    u32 req, home;
+    u64 pos;
+    bool walk;
+};
+
+// FLAGS: bit 0 = the batch holds several requests (records carry req), bit 1 = some request needs
+// cycle walks (records carry pos and home). Compile-time, so the record loads are unconditional.
+enum { F_MULTI = 1, F_WALK = 2 };
+
+template <int W, int RULE, bool FRESH, int FLAGS>
+__device__ __forceinline__ u64 play(const StageArgs<W> &A, WaveAcc &acc, St<W> &s, u32 C, u32 n, u64 alive,
+                                    bool uniform_req, const LaneCtx &L) {
+    const ReqDev *reqs = A.reqs;
+    const bool last = (A.a1 == UNBOUNDED);
+    St<W> t = s;
+    u32 R = A.a0;
+    u32 age = A.a0;
+    const u32 a1 = last ? AGE_CAP : A.a1;
+    for (;;) {
+        ++age;
+        turn<W, RULE>(s, C);
+        const u64 em = ballot(ended<W>(s, C)) & alive;
+        // three ballots of plain compares: each is one v_cmp into an SGPR pair
+        const u64 dm = ballot(s.a == t.a) & ballot(s.b == t.b) & ballot(s.sa == t.sa) & alive & ~em;
+        if (em | dm) {
+            const u32 lam = age - R;
+            if (uniform_req) {
+                if (em) acc.ended(__popcll(em), age);
+                if (dm) acc.cycled(__popcll(dm), age, lam);
+            } else {
+                u64 rem = em | dm;
+                while (rem) {
+                    const u32 r = lanev(L.req, (u32)(__ffsll((long long)rem) - 1));
+                    const u64 mr = ballot(L.req == r) & rem;
+                    rem &= ~mr;
+                    acc.select(r, reqs);
+                    if (mr & em) acc.ended(__popcll(mr & em), age);
+                    if (mr & dm) acc.cycled(__popcll(mr & dm), age, lam);
+                }
+            }
+            if ((FLAGS & F_WALK) && dm) {
+                const u64 wm = ballot(L.walk) & dm;
+                if (wm) acc.pushed += push_cycles<W>(A.cq, A.tile_pos, wm, s, n, L.req, lam, L.pos, L.home);
+            }
+            alive &= ~(em | dm);
+            if (!alive) break;
+        }
+        if (age == a1) break;
+        if ((age & (age + 1u)) == 0u) { t = s; R = age; }   // Brent: tortoise := hare
+    }
+    if (alive && last) {
+        // safety stop reached: counted as undetermined (an error for the caller)
+        u64 rem = alive;
+        while (rem) {
+            const u32 r = lanev(L.req, (u32)(__ffsll((long long)rem) - 1));
+            const u64 mr = ballot(L.req == r) & rem;
+            rem &= ~mr;
+            acc.select(r, reqs);
+            acc.undet += __popcll(mr); acc.done += __popcll(mr); acc.dirty = true;
+        }
+        alive = 0;
+    }
+    return alive;
+}
+
+// Write the survivors (lanes of `alive`) densely at span + out_n.
+template <int W, int FLAGS>
+__device__ __forceinline__ void put_survivors(const StageArgs<W> &A, u64 span, u32 out_n, u64 alive, const St<W> &s,
+                                              u32 n, const LaneCtx &L) {
+    typedef typename Lay<W>::T T;
+    if ((alive >> lane_id()) & 1ull) {
+        const u64 i = span + out_n + rank_in(alive);
+        T qa, qb;
+        pack<W>(s, n, qa, qb);
+        A.qout.a[i] = qa; A.qout.b[i] = qb;
+        if (FLAGS & F_MULTI) A.qout.req[i] = L.req;
+        if (FLAGS & F_WALK) { A.qout.pos[i] = L.pos; A.qout.home[i] = L.home; }
     }
 }
 
 // ------------------------------------------------------------------------------------------
-// The age-synchronous stage. FRESH: positions come from tiles of the job list. Otherwise: records
-// written by the previous stage.
-template <int W, int RULE, bool FRESH>
+// The age-synchronous stage. FRESH: a wave takes one tile of the job list at a time (lane l plays
+// positions l*per .. l*per+per-1 of the tile, one per round). Otherwise: a wave takes a group of
+// tiles of the previous stage's regions and plays their records 64 at a time. Survivors of a
+// tile / group go densely into the same tiles' regions of the other buffer.
+template <int W, int RULE, bool FRESH, int FLAGS>
 __global__ __launch_bounds__(256) void k_stage(StageArgs<W> A) {
     typedef typename Lay<W>::T T;
     extern __shared__ u32 lds[];
@@ -213,159 +362,139 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs<W> A) {
     acc.init(lds + wid * (A.hb + A.lb), A.hb, A.lb);
     const ReqDev *reqs = A.reqs;
     const bool last = (A.a1 == UNBOUNDED);
-    const u64 *qcount_in = FRESH ? nullptr : &A.pipe->qcount[A.stage - 1];
+    const u64 T_ = A.hdr->ntiles;
+    const u32 req0 = A.hdr->req0;
+    const u32 TP = A.tile_pos;
+    // tiles per grab: enough for ~256 records, from what the previous stage wrote in this batch
+    u32 G = 1;
+    if constexpr (!FRESH) {
+        const u64 recs = A.pipe->stotal[A.stage - 1];
+        if (recs == 0) { acc.flush(reqs); return; }
+        G = group_size(recs, T_, TP);
+    }
+    u64 *grab = &A.pipe->grab[A.stage];
+    // the next grab is requested one round ahead, so its latency hides behind a round of play
+    u64 nxt = 0;
+    if (lane == 0) nxt = atomicAdd((unsigned long long *)grab, 1ull);
 
     for (;;) {
-        // ---- obtain work: per lane a start state, its request, its position index
-        St<W> s;
-        u32 C = 0, n = 0, req = 0;
-        u64 pos = 0;
-        u64 alive = 0;
-        // FRESH bookkeeping
-        u32 per = 1, cnt = 0, mode = 0;
-        T q = 0;
-        int k = 0;
-        bool uniform_req = true, walk = false;
+        const u64 g0 = uni64(nxt) * G;
+        if (g0 >= T_) break;
+        if (lane == 0) nxt = atomicAdd((unsigned long long *)grab, 1ull);
+        const u64 span = g0 * TP;           // this tile / group's slots in the output buffer
+        u32 out_n = 0;                      // survivors written so far (wave-uniform)
+
         if constexpr (FRESH) {
-            const u64 t = wave_fetch_add(&A.pipe->tile_ctr, 1);
-            if (t >= A.hdr->ntiles) break;
-            // the job holding tile t (jobs are few; binary search on tile_base)
+            // the job holding tile g0 (jobs are few; binary search on tile_base)
             u32 lo = 0, hi = A.hdr->njobs;
-            while (hi - lo > 1) { u32 mid = (lo + hi) >> 1; if (A.jobs[mid].tile_base <= t) lo = mid; else hi = mid; }
+            while (hi - lo > 1) { u32 mid = (lo + hi) >> 1; if (A.jobs[mid].tile_base <= g0) lo = mid; else hi = mid; }
             const Job J = A.jobs[lo];
-            req = J.req;
-            const ReqDev &R = reqs[req];
-            n = R.n; mode = R.mode; walk = R.walk != 0;
-            C = W * n - 2u * W;
-            per = A.per;
-            const u64 first = J.lo + (t - J.tile_base) * (u64)A.tile_pos + (u64)lane * per;
-            cnt = first < J.hi ? (u32)((J.hi - first) < per ? (J.hi - first) : per) : 0u;
-            pos = first;
+            const ReqDev &Rq = reqs[J.req];
+            const u32 n = Rq.n, mode = Rq.mode;
+            const u32 C = W * n - 2u * W;
+            const u32 per = A.per;
+            acc.select(J.req, reqs);
+            LaneCtx L;
+            L.req = J.req; L.home = (u32)g0; L.walk = Rq.walk != 0;
+            const u64 first = J.lo + (g0 - J.tile_base) * (u64)TP + (u64)lane * per;
+            const u32 cnt = first < J.hi ? (u32)((J.hi - first) < per ? (J.hi - first) : per) : 0u;
+            L.pos = first;
+            T q = 0;
+            int k = 0;
             if (cnt) {
                 if (mode == 2) { const u32 m = n - 1; const u64 r = first / m; k = (int)(first - r * m); q = unrank<W>(r, (int)m); }
                 else q = unrank<W>(first, (int)n);
             }
-            acc.select(req, reqs);
-        } else {
-            const u64 base = wave_fetch_add(&A.pipe->qread[A.stage], 64);
-            const u64 total = *qcount_in < A.qcap ? *qcount_in : A.qcap;
-            if (base >= total) break;
-            const u64 i = base + lane;
-            const bool act = i < total;
-            if (act) {
-                unpack<W>(A.qin.a[i], A.qin.b[i], s, n);
-                req = A.need_req ? A.qin.req[i] : A.hdr->req0;
-                pos = A.need_pos ? A.qin.pos[i] : 0ull;
-            } else {
-                s.a = 0; s.b = 0; s.sa = 0; n = 2; req = 0;
-            }
-            C = W * n - 2u * W;
-            alive = ballot(act);
-            const u32 r0 = lanev(req, (u32)(__ffsll((long long)alive) - 1));
-            uniform_req = (ballot(req == r0) & alive) == alive;
-            if (uniform_req) acc.select(r0, reqs);
-        }
-
-        // ---- FRESH: loop over each lane's `per` positions; otherwise a single round
-        for (u32 j = 0;; j++) {
-            if constexpr (FRESH) {
-                if (j >= per || ballot(j < cnt) == 0) break;
+            for (u32 j = 0; j < per; j++) {
+                if (ballot(j < cnt) == 0) break;
                 bool act = j < cnt;
+                St<W> s;
+                s.a = 0; s.b = 0; s.sa = 0;
                 if (act) {
                     s = (mode == 2) ? canon<W>(q, (int)n, k) : deal<W>(q, (int)n);
                     if (mode == 1) act = top_card_to_p1<W>(q, (int)n);
                 }
-                u64 act_m = ballot(act);
+                const u64 act_m = ballot(act);
                 // a position that is over before it starts: rounds 0 (gpu/war_gpu.py:80)
                 const u64 e0 = ballot(act && (n < 2 || ended<W>(s, C))) & act_m;
                 if (e0) acc.ended(__popcll(e0), 0);
-                alive = act_m & ~e0;
-            }
-
-            // ---- the lockstep game
-            if (alive) {
-                St<W> t = s;
-                u32 R = A.a0;
-                u32 age = A.a0;
-                const u32 a1 = last ? AGE_CAP : A.a1;
-                for (;;) {
-                    ++age;
-                    turn<W, RULE>(s, C);
-                    const u64 em = ballot(ended<W>(s, C)) & alive;
-                    // three ballots of plain compares: each is one v_cmp into an SGPR pair
-                    const u64 dm = ballot(s.a == t.a) & ballot(s.b == t.b) & ballot(s.sa == t.sa) & alive & ~em;
-                    if (em | dm) {
-                        const u32 lam = age - R;
-                        u64 rem = em | dm;
-                        if (uniform_req) {
-                            if (em) acc.ended(__popcll(em), age);
-                            if (dm) acc.cycled(__popcll(dm), age, lam);
-                        } else {
-                            while (rem) {
-                                const u32 r = lanev(req, (u32)(__ffsll((long long)rem) - 1));
-                                const u64 mr = ballot(req == r) & rem;
-                                rem &= ~mr;
-                                acc.select(r, reqs);
-                                if (mr & em) acc.ended(__popcll(mr & em), age);
-                                if (mr & dm) acc.cycled(__popcll(mr & dm), age, lam);
-                            }
-                        }
-                        if (dm) {
-                            bool need = false;
-                            if constexpr (FRESH) need = walk;
-                            else need = reqs[req].walk != 0;
-                            const u64 wm = ballot(need) & dm;
-                            if (wm) push_cycles<W>(A, wm, s, n, req, lam, pos);
-                        }
-                        alive &= ~(em | dm);
-                        if (!alive) break;
-                    }
-                    if (age == a1) break;
-                    if ((age & (age + 1u)) == 0u) { t = s; R = age; }   // Brent: tortoise := hare
-                }
+                u64 alive = act_m & ~e0;
+                if (alive) alive = play<W, RULE, FRESH, FLAGS>(A, acc, s, C, n, alive, true, L);
                 if (alive) {
-                    if (last) {
-                        // safety stop reached: count as undetermined (an error for the caller)
-                        if (uniform_req) { acc.undet += __popcll(alive); acc.done += __popcll(alive); acc.dirty = true; }
-                        else {
-                            u64 rem = alive;
-                            while (rem) {
-                                const u32 r = lanev(req, (u32)(__ffsll((long long)rem) - 1));
-                                const u64 mr = ballot(req == r) & rem;
-                                rem &= ~mr;
-                                acc.select(r, reqs);
-                                acc.undet += __popcll(mr); acc.done += __popcll(mr); acc.dirty = true;
-                            }
-                        }
-                    } else {
-                        // compact the survivors into the next stage's queue
-                        const u64 base = wave_fetch_add(&A.pipe->qcount[A.stage], __popcll(alive));
-                        if ((alive >> lane) & 1ull) {
-                            const u64 i = base + rank_in(alive);
-                            if (i < A.qcap) {
-                                T qa, qb;
-                                pack<W>(s, n, qa, qb);
-                                A.qout.a[i] = qa; A.qout.b[i] = qb;
-                                if (A.need_req) A.qout.req[i] = req;
-                                if (A.need_pos) A.qout.pos[i] = pos;
-                            }
-                        }
-                    }
+                    put_survivors<W, FLAGS>(A, span, out_n, alive, s, n, L);
+                    out_n += __popcll(alive);
                 }
-            }
-
-            if constexpr (FRESH) {
                 if (j + 1 < cnt) {
                     if (mode == 2) { if (++k == (int)n - 1) { k = 0; q = next_perm<W>(q, (int)n - 1); } }
                     else q = next_perm<W>(q, (int)n);
                 }
-                pos++;
-            } else {
-                break;
+                L.pos++;
+            }
+            if (!last && lane == 0) A.qout.cnt[g0] = out_n;
+        } else {
+            // the records of tiles g0 .. g0+ng-1
+            const u32 ng = (u32)((T_ - g0) < G ? (T_ - g0) : G);
+            const u32 c = lane < ng ? A.qin.cnt[g0 + lane] : 0u;
+            const u32 incl = wave_scan(c);
+            const u32 excl = incl - c;
+            const u32 total = lanev(incl, 63);
+            // records of the next batch are loaded one batch ahead (their latency hides behind play)
+            T na = 0, nb = 0;
+            u32 nreq = req0, nhome = 0;
+            u64 npos = 0;
+            // called with every lane active: group_locate reads other lanes (ds_bpermute), and a
+            // read from an inactive lane does not return its value
+            auto fetch = [&](u32 kk) {
+                u32 ti, off;
+                group_locate(excl, ng, kk < total ? kk : total - 1u, ti, off);
+                if (kk < total && off < TP) {
+                    const u64 i = (g0 + ti) * (u64)TP + off;
+                    na = A.qin.a[i]; nb = A.qin.b[i];
+                    if constexpr ((FLAGS & F_MULTI) != 0) nreq = A.qin.req[i];
+                    if constexpr ((FLAGS & F_WALK) != 0) { npos = A.qin.pos[i]; nhome = A.qin.home[i]; }
+                }
+            };
+            fetch(lane);
+            for (u32 b = 0; b < total; b += 64) {
+                const u32 kk = b + lane;
+                const bool act = kk < total;
+                St<W> s;
+                u32 n = 2;
+                LaneCtx L;
+                L.req = req0; L.pos = 0; L.home = 0; L.walk = false;
+                if (act) {
+                    unpack<W>(na, nb, s, n);
+                    L.req = nreq; L.pos = npos; L.home = nhome;
+                } else {
+                    s.a = 0; s.b = 0; s.sa = 0;
+                }
+                if (b + 64 < total) fetch(b + 64 + lane);
+                const u32 C = W * n - 2u * W;
+                u64 alive = ballot(act);
+                bool uniform_req = true;
+                if constexpr ((FLAGS & F_MULTI) != 0) {
+                    const u32 r0 = lanev(L.req, 0);
+                    uniform_req = (ballot(L.req == r0) & alive) == alive;
+                    if (uniform_req) acc.select(r0, reqs);
+                } else {
+                    acc.select(req0, reqs);
+                }
+                if constexpr ((FLAGS & F_WALK) != 0) L.walk = act && reqs[L.req].walk != 0;
+                alive = play<W, RULE, FRESH, FLAGS>(A, acc, s, C, n, alive, uniform_req, L);
+                if (alive) {
+                    put_survivors<W, FLAGS>(A, span, out_n, alive, s, n, L);
+                    out_n += __popcll(alive);
+                }
+            }
+            if (!last && lane < ng) {
+                const u32 before = lane * TP;
+                A.qout.cnt[g0 + lane] = out_n > before ? (out_n - before < TP ? out_n - before : TP) : 0u;
             }
         }
+        if (!last && out_n && lane == 0) atomicAdd((unsigned long long *)&A.pipe->stotal[A.stage], (unsigned long long)out_n);
     }
     acc.flush(reqs);
+    if (acc.pushed && lane == 0) atomicAdd((unsigned long long *)&A.pipe->ctotal, (unsigned long long)acc.pushed);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -373,12 +502,11 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs<W> A) {
 // one at a time, each with its own Brent loop (per-lane tortoise, power and lam).
 template <int W> struct PlainArgs {
     u64 lo, hi;
-    u32 per, hb, lb, pad;
+    u32 per, hb, lb, tile_pos;   // tile_pos: slots per home tile of the cycle records
     ReqDev R;
     u32 req;
-    Pipe *pipe;
     CycQueue<W> cq;
-    u64 cqcap;
+    Pipe *pipe;
 };
 
 template <int W, int RULE>
@@ -397,7 +525,7 @@ __global__ __launch_bounds__(256) void k_plain(PlainArgs<W> P) {
         if (mode == 2) { const u32 m = n - 1; const u64 r = first / m; k = (int)(first - r * m); q = unrank<W>(r, (int)m); }
         else q = unrank<W>(first, (int)n);
     }
-    u64 cyc = 0, done = 0, maxfin = 0, turns = 0, undet = 0;
+    u64 cyc = 0, done = 0, maxfin = 0, turns = 0, undet = 0, pushed = 0;
     for (u32 j = 0; j < cnt; j++) {
         St<W> s = (mode == 2) ? canon<W>(q, (int)n, k) : deal<W>(q, (int)n);
         const bool take = (mode == 1) ? top_card_to_p1<W>(q, (int)n) : true;
@@ -428,18 +556,12 @@ __global__ __launch_bounds__(256) void k_plain(PlainArgs<W> P) {
                 if (P.lb) atomicAdd(&ll[rounds < P.lb - 1 ? (u32)rounds : P.lb - 1], 1u);
             }
         }
-        // cycle records for the walks (reconverged here: wave-aggregated append)
+        // cycle records for the walks (reconverged here: aggregated per home tile)
         if (P.R.walk) {
             const u64 wm = ballot(cy);
             if (wm) {
-                const u64 base = wave_fetch_add(&P.pipe->ccount, __popcll(wm));
-                if (cy) {
-                    const u64 i = base + rank_in(wm);
-                    if (i < P.cqcap) {
-                        P.cq.a[i] = s.a; P.cq.b[i] = s.b; P.cq.meta[i] = s.sa | (n << 8);
-                        P.cq.req[i] = P.req; P.cq.lam[i] = lam; P.cq.pos[i] = first + j;
-                    }
-                }
+                const u64 pos = first + j;
+                pushed += push_cycles<W>(P.cq, P.tile_pos, wm, s, n, P.req, lam, pos, (u32)((pos - P.lo) / P.tile_pos));
             }
         }
         if (j + 1 < cnt) {
@@ -455,6 +577,8 @@ __global__ __launch_bounds__(256) void k_plain(PlainArgs<W> P) {
         undet += __shfl_xor(undet, off);
         const u64 o = __shfl_xor(maxfin, off);
         maxfin = o > maxfin ? o : maxfin;
+        const u64 pu = __shfl_xor(pushed, off);      // every lane active in a push saw its whole total
+        pushed = pu > pushed ? pu : pushed;
     }
     if (lane_id() == 0 && done) {
         atomicAdd((unsigned long long *)&P.R.cnt[C_CYC], (unsigned long long)cyc);
@@ -463,6 +587,7 @@ __global__ __launch_bounds__(256) void k_plain(PlainArgs<W> P) {
         if (undet) atomicAdd((unsigned long long *)&P.R.cnt[C_UNDET], (unsigned long long)undet);
         atomicMax((unsigned long long *)&P.R.cnt[C_MAXFIN], (unsigned long long)maxfin);
     }
+    if (lane_id() == 0 && pushed) atomicAdd((unsigned long long *)&P.pipe->ctotal, (unsigned long long)pushed);
     __syncthreads();
     for (u32 i = threadIdx.x; i < P.hb; i += blockDim.x)
         if (lh[i]) atomicAdd((unsigned long long *)&P.R.hist[i], (unsigned long long)lh[i]);
@@ -474,9 +599,10 @@ __global__ __launch_bounds__(256) void k_plain(PlainArgs<W> P) {
 // Cycle walks. One record per lane; lockstep over the longest lam of the wave.
 template <int W> struct WalkArgs {
     Pipe *pipe;
+    const JobHdr *hdr;
     const ReqDev *reqs;
     CycQueue<W> cq;
-    u64 cqcap;
+    u32 tile_pos;
 };
 
 // Insert key K into the open-addressing set; returns 1 if it was new. The probe length is bounded;
@@ -498,16 +624,30 @@ template <int W, int RULE>
 __global__ __launch_bounds__(256) void k_walk(WalkArgs<W> A) {
     typedef typename Lay<W>::T T;
     const u32 lane = lane_id();
+    const u64 T_ = A.hdr->ntiles;
+    const u64 recs = A.pipe->ctotal;
+    if (recs == 0) return;
+    const u32 G = group_size(recs, T_, A.tile_pos);
+    const u32 TP = A.tile_pos;
     for (;;) {
-        const u64 base = wave_fetch_add(&A.pipe->cread, 64);
-        const u64 total = A.pipe->ccount < A.cqcap ? A.pipe->ccount : A.cqcap;
-        if (base >= total) break;
-        const u64 i = base + lane;
-        const bool act = i < total;
+      const u64 g0 = wave_fetch_add(&A.pipe->wgrab, 1) * G;
+      if (g0 >= T_) break;
+      const u32 ng = (u32)((T_ - g0) < G ? (T_ - g0) : G);
+      const u32 cc = lane < ng ? A.cq.cnt[g0 + lane] : 0u;
+      const u32 c = cc < TP ? cc : TP;
+      const u32 incl = wave_scan(c);
+      const u32 excl = incl - c;
+      const u32 total = lanev(incl, 63);
+      for (u32 bb = 0; bb < total; bb += 64) {
+        const u32 kk = bb + lane;
+        const bool act = kk < total;
+        u32 ti, off;
+        group_locate(excl, ng, act ? kk : 0u, ti, off);
+        const u64 i = (g0 + ti) * (u64)TP + off;
         St<W> m;
         u32 n = 0, req = 0, lam = 0;
         u64 pos = 0;
-        if (act) {
+        if (act && off < TP) {
             m.a = A.cq.a[i]; m.b = A.cq.b[i];
             const u32 meta = A.cq.meta[i];
             m.sa = meta & 0xFFu; n = meta >> 8;
@@ -603,6 +743,7 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs<W> A) {
                 }
             }
         }
+    }
     }
 }
 

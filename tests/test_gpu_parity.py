@@ -75,10 +75,15 @@ def test_distinct_partial_ranges(eng, orc, n):
     {"brent_cap": 31, "ranks_per_refill": 4096, "waves_per_cu": 32},
     {"brent_cap": 127, "ranks_per_refill": 256, "waves_per_cu": 8},
     {"brent_cap": 1023, "ranks_per_refill": 1024, "waves_per_cu": 16},
+    {"brent_cap": 1, "ranks_per_refill": 64, "waves_per_cu": 2},      # sparse tiles: grabs span many tiles
+    {"brent_cap": 3, "ranks_per_refill": 8192, "waves_per_cu": 1},
 ])
 def test_schedules_are_exact(eng, orc, tuning):
     check(eng, orc, 13, "canon", 123_456_789, 123_456_789 + 2_000_000, tuning=tuning)
     check(eng, orc, 13, "full", 3_000_000_000, 3_000_000_000 + 1_000_000, tuning=tuning)
+    check(eng, orc, 8, "full", 0, None, rule=1, tuning=tuning)          # short games: sparse later stages
+    check(eng, orc, 12, "canon", 7_000_000, 7_000_000 + 500_000, tuning=tuning, dbins=64, distinct=True,
+          shape_blk=4, fields=FIELDS + ("oncycle", "distinct_minkey", "dhist", "shape_nonstd"))
 
 
 def test_5bit_ranges(eng, orc):
