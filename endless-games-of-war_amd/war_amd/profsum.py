@@ -1,0 +1,70 @@
+"""Summarise a rocprofv3 results database (kernel trace or PMC counters) as Markdown.
+
+    python -m war_amd.profsum trace  <results.db>      per-kernel totals and per-batch stage times
+    python -m war_amd.profsum pmc    <results.db>      counters per kernel (summed over dispatches)
+
+A batch of the persistent pipeline starts with k_setup, so the trace view also lists, for the
+first and last batches, the duration of each stage kernel in launch order.
+"""
+import collections
+import sqlite3
+import sys
+
+
+def _short(name):
+    return name.split("(")[0].replace("void ", "")
+
+
+def trace(db):
+    c = sqlite3.connect(db)
+    rows = list(c.execute("select name, start, end from kernels order by start"))
+    agg = collections.OrderedDict()
+    for n, s, e in rows:
+        a = agg.setdefault(_short(n), [0, 0.0])
+        a[0] += 1
+        a[1] += (e - s) / 1e6
+    tot = sum(v[1] for v in agg.values())
+    out = ["| kernel | calls | total ms | avg ms | share |", "|---|---:|---:|---:|---:|"]
+    for n, (cnt, ms) in sorted(agg.items(), key=lambda kv: -kv[1][1]):
+        out.append("| `%s` | %d | %.3f | %.4f | %.1f%% |" % (n, cnt, ms, ms / cnt, 100 * ms / tot))
+    batches = []
+    for n, s, e in rows:
+        if "k_setup" in n:
+            batches.append([])
+        elif batches and ("k_stage" in n or "k_walk" in n or "k_plain" in n):
+            batches[-1].append((e - s) / 1e6)
+    if batches:
+        out += ["", "Per batch (ms, stage kernels in launch order):", ""]
+        show = list(enumerate(batches))
+        if len(show) > 6:
+            show = show[:3] + show[-3:]
+        for i, b in show:
+            out.append("- batch %d: %s (sum %.2f)" % (i, " ".join("%.2f" % v for v in b), sum(b)))
+        out.append("- batches: %d, mean batch %.3f ms" % (len(batches), sum(sum(b) for b in batches) / len(batches)))
+    return "\n".join(out)
+
+
+def pmc(db):
+    c = sqlite3.connect(db)
+    d = collections.OrderedDict()
+    dur = collections.defaultdict(float)
+    seen = set()
+    for kn, cn, v, disp, du in c.execute(
+            "select kernel_name, counter_name, value, dispatch_id, duration from counters_collection"):
+        k = _short(kn)
+        d.setdefault(k, collections.OrderedDict())
+        d[k][cn] = d[k].get(cn, 0.0) + v
+        if (disp, kn) not in seen:
+            seen.add((disp, kn))
+            dur[k] += du / 1e6
+    names = sorted({cn for v in d.values() for cn in v})
+    out = ["| kernel | ms | " + " | ".join(names) + " |", "|---|---:|" + "---:|" * len(names)]
+    for k, v in d.items():
+        out.append("| `%s` | %.2f | " % (k, dur[k]) + " | ".join("%.4g" % v.get(n, 0) for n in names) + " |")
+    return "\n".join(out)
+
+
+if __name__ == "__main__":
+    if len(sys.argv) != 3 or sys.argv[1] not in ("trace", "pmc"):
+        sys.exit(__doc__)
+    print(trace(sys.argv[2]) if sys.argv[1] == "trace" else pmc(sys.argv[2]))
