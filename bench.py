@@ -81,6 +81,7 @@ def main():
                     help="run one workload only (for profiling)")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--cpu-chunks", type=int, default=64)
+    ap.add_argument("--autotune", action="store_true", help="autotune n=13 full and n=14 canon first")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this engine runs on one GPU; --gpus %d is not supported (the rank space would be "
@@ -92,7 +93,10 @@ def main():
     dev = eng.device_info()
     log("device", dev)
     n13 = math.factorial(13)
-    tun = eng.get_tuning()
+    if a.autotune:
+        for n, mode in ((13, "full"), (14, "canon")):
+            log("autotune", n, mode, eng.autotune(n, mode, budget_ms=60000))
+    tun = eng.tuning_for(13, "full")
 
     if a.only == "main":
         r = eng.run_range(13, "full", hbins=256)
@@ -135,6 +139,9 @@ def main():
                    "parity": "a(13), longest 57, cycle-length histogram equal the published values every step"},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": peak_lane_ops / 1e12,
                      "unit": "Tlane-op/s", "frac": achieved / peak_lane_ops, "traffic": None,
+                     "kernel": "k_stage: the stage kernels of each batch graph (FRESH + queue stages)",
+                     "traffic_note": "HBM bytes per batch from rocprofv3 FETCH_SIZE/WRITE_SIZE are in "
+                                     "profiles/k_stage_n13_full_pmc.md (not measured inside this run)",
                      "turns_per_step": turns // a.steps, "valu_ops_per_turn": VALU_OPS_PER_TURN,
                      "device_ms_per_step": dev_ms / a.steps,
                      "note": "achieved = Brent turns x VALU ops per turn / device time (HIP events)"},
@@ -160,8 +167,8 @@ def main():
         canon = {"workload": "n14_canon: all 13!*13 = 80951270400 positions", "persistent_value": tot14 / cw,
                  "unit": "positions/s", "persistent_device_ms": r["device_ms"],
                  "parity": "cycling 2872129173, longest 465, hist {96,112} equal the published values"}
-        # both kernels on the same sampled chunks (seed 14): batched / plain ratio
-        ch = chunks_spread(tot14, 16, 1 << 24, 14)
+        # both kernels on the same sampled chunks (seed 14, whole q-ranks): batched / plain ratio
+        ch = [(lo - lo % 13, lo - lo % 13 + 13 * (1 << 22)) for lo, _ in chunks_spread(tot14, 8, 13 * (1 << 22), 14)]
         tp = tq = 0.0
         for lo, hi in ch:
             rp = eng.run_range(14, "canon", lo, hi, kernel="plain")
@@ -169,8 +176,9 @@ def main():
             assert (rp["cyc"], rp["maxfin"], rp["turns"]) == (rq["cyc"], rq["maxfin"], rq["turns"])
             tp += rp["device_ms"]; tq += rq["device_ms"]
         npos = sum(hi - lo for lo, hi in ch)
-        canon.update({"sample": "16 chunks of 2^24 positions, seed 14", "plain_sample_value": npos / (tp / 1e3),
-                      "persistent_sample_value": npos / (tq / 1e3), "persistent_over_plain": tp / tq})
+        canon.update({"sample": "8 chunks of 13*2^22 positions (whole q-ranks), seed 14",
+                      "plain_sample_value": npos / (tp / 1e3), "persistent_sample_value": npos / (tq / 1e3),
+                      "persistent_over_plain": tp / tq, "tuning": eng.tuning_for(14, "canon")})
         out["canon14"] = canon
         log("canon14: %.3e pos/s persistent (whole), ratio %.2f on samples" % (tot14 / cw, tp / tq))
 
@@ -192,9 +200,11 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def serving(eng, nreq=64, seed=20261203):
-    """Poisson arrivals of (n, range, mode) queries; p50/p99 latency and aggregate positions/s,
-    beside the same queries one by one through the plain kernel."""
+def serving(eng, nreq=64, seed=20261203, rate=100.0):
+    """Mixed (n, range, mode) queries. (1) Poisson arrivals at `rate`/s through the continuous-batching
+    loop: p50/p99 latency. (2) All submitted at once: aggregate positions/s. (3) The same queries one
+    by one, synchronously, through the plain kernel and through the persistent kernel. Every count
+    must be identical across (1), (2) and (3)."""
     rng = random.Random(seed)
     qs = []
     for _ in range(nreq):
@@ -205,33 +215,45 @@ def serving(eng, nreq=64, seed=20261203):
         lo = rng.randrange(0, total - ln + 1)
         distinct = mode == "canon" and n <= 13 and rng.random() < 0.3
         qs.append((n, mode, lo, lo + ln, distinct))
-    eng.serve_start(max_slots=64, step_positions=1 << 26)
-    ids = []
-    t0 = time.perf_counter()
-    rate = 200.0     # arrivals per second
-    for q in qs:
-        time.sleep(rng.expovariate(rate))
-        n, mode, lo, hi, d = q
-        ids.append(eng.submit(n, mode, lo, hi, hbins=128, distinct=d))
-    res = [eng.wait(i) for i in ids]
-    wall = time.perf_counter() - t0
-    eng.serve_stop()
-    lat = sorted(r["latency_ms"] for r in res)
-    # the same queries alone through the plain kernel: equal counts, summed latency
-    tplain = 0.0
-    for q, r in zip(qs, res):
-        n, mode, lo, hi, d = q
-        t1 = time.perf_counter()
-        p = eng.run_range(n, mode, lo, hi, kernel="plain", hbins=128, distinct=d)
-        tplain += time.perf_counter() - t1
-        assert (p["cyc"], p["done"], p["maxfin"], p["turns"], p["hist"]) == \
-            (r["cyc"], r["done"], r["maxfin"], r["turns"], r["hist"]), q
+    keys = ("cyc", "done", "maxfin", "turns", "hist", "distinct_minkey")
+
+    def run_served(gaps):
+        eng.serve_start(max_slots=64, step_positions=1 << 26)
+        ids = []
+        t0 = time.perf_counter()
+        for q, g in zip(qs, gaps):
+            if g:
+                time.sleep(g)
+            n, mode, lo, hi, d = q
+            ids.append(eng.submit(n, mode, lo, hi, hbins=128, distinct=d))
+        res = [eng.wait(i) for i in ids]
+        wall = time.perf_counter() - t0
+        eng.serve_stop()
+        return res, wall
+
+    arrivals, wall_a = run_served([rng.expovariate(rate) for _ in qs])
+    burst, wall_b = run_served([0.0] * len(qs))
+    lat = sorted(r["latency_ms"] for r in arrivals)
     npos = sum(hi - lo for _, _, lo, hi, _ in qs)
+    alone = {}
+    for kernel in ("plain", "persistent"):
+        t = 0.0
+        for q, r, b in zip(qs, arrivals, burst):
+            n, mode, lo, hi, d = q
+            t1 = time.perf_counter()
+            p = eng.run_range(n, mode, lo, hi, kernel=kernel, hbins=128, distinct=d)
+            t += time.perf_counter() - t1
+            for k in keys:
+                assert p[k] == r[k] == b[k], (q, kernel, k)
+        alone[kernel] = npos / t
     return {"requests": nreq, "arrival_rate_per_s": rate, "p50_ms": lat[len(lat) // 2],
-            "p99_ms": lat[min(len(lat) - 1, int(0.99 * len(lat)))], "positions_per_s": npos / wall,
-            "plain_one_by_one_positions_per_s": npos / tplain,
-            "queries": "n uniform 7..17, full or canon (distinct for 30% of canon n<=13), length log-uniform "
-                       "[1e3, 10^8.5], Poisson arrivals; every count equal to the plain kernel's"}
+            "p99_ms": lat[min(len(lat) - 1, int(0.99 * len(lat)))], "arrivals_wall_s": wall_a,
+            "saturated_positions_per_s": npos / wall_b,
+            "one_by_one_plain_positions_per_s": alone["plain"],
+            "one_by_one_persistent_positions_per_s": alone["persistent"],
+            "queries": "n uniform 7..17, full or canon (distinct for 30%% of canon n<=13), length log-uniform "
+                       "[1e3, 10^8.5] (%d positions in all); counts identical alone, in the mix and in the "
+                       "burst" % npos}
 
 
 if __name__ == "__main__":

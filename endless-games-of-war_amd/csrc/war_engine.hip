@@ -104,10 +104,10 @@ constexpr u64 MIN_TILE = 64, MAX_TILE = 8192;
 constexpr u32 PLAIN_CYC_TILE = 1024;    // home-tile size of the plain kernel's cycle records
 
 struct GraphKey {
-    int W, rule, kernel, wpc, per, cap0, walk, multi, hb, lb;
+    int W, rule, kernel, wpc, per, cap0, walk, multi, hb, lb, kind;
     bool operator<(const GraphKey &o) const {
-        return std::tie(W, rule, kernel, wpc, per, cap0, walk, multi, hb, lb) <
-               std::tie(o.W, o.rule, o.kernel, o.wpc, o.per, o.cap0, o.walk, o.multi, o.hb, o.lb);
+        return std::tie(W, rule, kernel, wpc, per, cap0, walk, multi, hb, lb, kind) <
+               std::tie(o.W, o.rule, o.kernel, o.wpc, o.per, o.cap0, o.walk, o.multi, o.hb, o.lb, o.kind);
     }
 };
 
@@ -118,7 +118,8 @@ struct Engine {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int cus = 0, clock_mhz = 0;
     std::string arch;
-    war_tuning tun{32, 1024, 15, 32};
+    war_tuning tun{0, 0, 0, 0, 0};            // engine-wide override (war_set_tuning); zeros = per-workload default
+    std::map<std::tuple<int, int, int>, war_tuning> tuned;   // (n, mode, rule) -> war_autotune's choice
     u64 batch = 1ull << 28;          // positions per persistent batch (= queue capacity)
     Pipe *pipe = nullptr;
     JobHdr *hdr = nullptr;           // device: JobHdr followed by MAXJOBS Jobs
@@ -163,12 +164,28 @@ int check_request(const war_request *r) {
     return 0;
 }
 
-void resolve_tuning(const Engine &E, const war_tuning &in, war_tuning &t) {
-    t = E.tun;
-    if (in.waves_per_cu > 0) t.waves_per_cu = in.waves_per_cu;
-    if (in.ranks_per_refill > 0) t.ranks_per_refill = in.ranks_per_refill;
-    if (in.brent_cap > 0) t.brent_cap = in.brent_cap;
-    if (in.plain_per > 0) t.plain_per = in.plain_per;
+// Defaults measured on MI355X: deals (long games, ~110 turns at n = 13) want a late first stage
+// boundary; canonical positions (most end within ~30 turns) an early one and larger tiles.
+war_tuning default_tuning(int n, int mode) {
+    war_tuning t{28, 1024, 15, 32, 2};
+    if (mode != WAR_MODE_CANON && n >= 12) t.brent_cap = 63;
+    if (mode == WAR_MODE_CANON) t.ranks_per_refill = 4096;
+    return t;
+}
+
+// Precedence: the request's own fields, then the engine-wide override, then war_autotune's choice
+// for this (n, mode, rule), then the default.
+void resolve_tuning(const Engine &E, const war_request &r, war_tuning &t) {
+    t = default_tuning(r.n, r.mode);
+    auto it = E.tuned.find(std::make_tuple(r.n, r.mode, r.rule));
+    if (it != E.tuned.end()) t = it->second;
+    for (const war_tuning *src : {&E.tun, &r.tuning}) {
+        if (src->waves_per_cu > 0) t.waves_per_cu = src->waves_per_cu;
+        if (src->ranks_per_refill > 0) t.ranks_per_refill = src->ranks_per_refill;
+        if (src->brent_cap > 0) t.brent_cap = src->brent_cap;
+        if (src->plain_per > 0) t.plain_per = src->plain_per;
+        if (src->stage_kind > 0) t.stage_kind = src->stage_kind;
+    }
 }
 
 int check_tuning(const war_tuning &t) {
@@ -178,6 +195,7 @@ int check_tuning(const war_tuning &t) {
     const int c = t.brent_cap;
     if (c < 1 || c > 65535 || ((c + 1) & c)) return fail(11, "brent_cap must be 2^k - 1 in [1, 65535]");
     if (t.plain_per < 1 || t.plain_per > 4096) return fail(11, "plain_per must be in [1, 4096]");
+    if (t.stage_kind != 1 && t.stage_kind != 2) return fail(11, "stage_kind must be 1 (refill) or 2 (lockstep)");
     return 0;
 }
 
@@ -249,7 +267,9 @@ template <int W> int ensure_queues(Engine &E, bool walk) {
     typedef typename Lay<W>::T T;
     QueueBufs &Q = qbufs<W>(E);
     if (!Q.cap) {
-        const u64 cap = E.batch + (u64)MAXJOBS * MAX_TILE;   // T * tile_pos <= batch + jobs * tile_pos
+        // T * tile_pos <= batch + jobs * tile_pos; a refill stage leaves at most one partly filled
+        // region per wave (<= 32 waves/CU x CUs) beyond its records
+        const u64 cap = E.batch + std::max<u64>((u64)MAXJOBS, (u64)E.cus * 32) * MAX_TILE;
         const u64 tmax = cap / MIN_TILE + MAXJOBS;
         for (int i = 0; i < 2; i++) {
             HIPCHK(hipMalloc(&Q.qa[i], cap * sizeof(T)));
@@ -312,6 +332,16 @@ void launch_stage(bool fresh, int flags, int blocks, size_t lds, hipStream_t st,
 }
 
 template <int W, int RULE>
+void launch_refill(int flags, int blocks, size_t lds, hipStream_t st, const StageArgs<W> &A) {
+    switch (flags) {
+        case 0: hipLaunchKernelGGL((k_refill<W, RULE, 0>), dim3(blocks), dim3(256), lds, st, A); break;
+        case F_MULTI: hipLaunchKernelGGL((k_refill<W, RULE, F_MULTI>), dim3(blocks), dim3(256), lds, st, A); break;
+        case F_WALK: hipLaunchKernelGGL((k_refill<W, RULE, F_WALK>), dim3(blocks), dim3(256), lds, st, A); break;
+        default: hipLaunchKernelGGL((k_refill<W, RULE, F_MULTI | F_WALK>), dim3(blocks), dim3(256), lds, st, A); break;
+    }
+}
+
+template <int W, int RULE>
 int build_graph(Engine &E, const GraphKey &key, hipGraphExec_t *out) {
     typedef typename Lay<W>::T T;
     QueueBufs &Q = qbufs<W>(E);
@@ -334,7 +364,14 @@ int build_graph(Engine &E, const GraphKey &key, hipGraphExec_t *out) {
         A.per = (u32)key.per; A.tile_pos = 64u * (u32)key.per;
         A.a0 = a0; A.a1 = caps[s]; A.stage = (int)s;
         A.hb = (u32)key.hb; A.lb = (u32)key.lb;
-        launch_stage<W, RULE>(s == 0, (key.multi ? F_MULTI : 0) | (key.walk ? F_WALK : 0), blocks, lds, E.st, A);
+        A.max_regions = Q.cap / A.tile_pos;
+        const int flags = (key.multi ? F_MULTI : 0) | (key.walk ? F_WALK : 0);
+        // refill stages need a bounded window (no tortoise move inside); the unbounded last stage
+        // and the first (enumerating) stage are lockstep
+        const bool refill = key.kind == 1 && s > 0 && caps[s] != UNBOUNDED;
+        A.in_refill = (key.kind == 1 && s > 1) ? 1u : 0u;
+        if (refill) launch_refill<W, RULE>(flags, blocks, lds, E.st, A);
+        else launch_stage<W, RULE>(s == 0, flags, blocks, lds, E.st, A);
         a0 = caps[s];
     }
     if (key.walk) {
@@ -362,7 +399,7 @@ int run_persistent_batch(Engine &E, std::vector<Job> &jobs, const war_tuning &t,
     u64 tiles = 0, npos = 0;
     for (Job &j : jobs) { j.tile_base = tiles; tiles += (j.hi - j.lo + tile_pos - 1) / tile_pos; npos += j.hi - j.lo; }
     GraphKey key{W, RULE, 1, t.waves_per_cu, t.ranks_per_refill / 64, t.brent_cap, walk ? 1 : 0, multi ? 1 : 0,
-                 round_pow2(hb), round_pow2(lb)};
+                 round_pow2(hb), round_pow2(lb), t.stage_kind};
     if (tiles > qbufs<W>(E).tmax || tiles * tile_pos > qbufs<W>(E).cap) return fail(32, "batch exceeds the queue capacity");
     JobHdr h{};
     h.njobs = (u32)jobs.size(); h.req0 = jobs[0].req; h.ntiles = tiles;
@@ -383,11 +420,14 @@ int run_persistent_batch(Engine &E, std::vector<Job> &jobs, const war_tuning &t,
     HIPCHK(hipEventRecord(E.ev0, E.st));
     HIPCHK(hipGraphLaunch(ge, E.st));
     HIPCHK(hipEventRecord(E.ev1, E.st));
+    u64 ovf = 0;
+    HIPCHK(hipMemcpyAsync(&ovf, &E.pipe->ovf, sizeof ovf, hipMemcpyDeviceToHost, E.st));
     HIPCHK(hipStreamSynchronize(E.st));
     float m = 0;
     HIPCHK(hipEventElapsedTime(&m, E.ev0, E.ev1));
     if (ms) *ms = m;
     (void)npos;
+    if (ovf) return fail(33, "persistent pipeline ran out of queue regions (%llu refused)", ovf);
     return 0;
 }
 
@@ -447,7 +487,7 @@ int alloc_slot(Engine &E) {
 int run_request_locked(Engine &E, ReqHost &h) {
     const war_request &r = h.rq;
     war_tuning t;
-    resolve_tuning(E, r.tuning, t);
+    resolve_tuning(E, r, t);
     int rc = check_tuning(t);
     if (rc) return rc;
     h.slot = alloc_slot(E);
@@ -536,7 +576,9 @@ void serve_loop(Engine *Ep) {
                 std::vector<std::shared_ptr<ReqHost>> grp;
                 for (auto &h : active) if ((h->rq.n <= 16 ? 4 : 5) == W && h->rq.rule == rule && h->next < h->rq.hi) grp.push_back(h);
                 if (grp.empty()) continue;
-                const u64 tile = (u64)E.tun.ranks_per_refill;
+                war_tuning gt;
+                resolve_tuning(E, grp[0]->rq, gt);
+                const u64 tile = (u64)gt.ranks_per_refill;
                 u64 share = std::max<u64>(tile, E.step_positions / grp.size());
                 share = (share + tile - 1) / tile * tile;
                 std::vector<Job> jobs;
@@ -551,8 +593,8 @@ void serve_loop(Engine *Ep) {
                     walk = walk || h->dev.walk;
                 }
                 float ms = 0;
-                rc = (W == 4) ? run_batch_dispatch<4>(E, rule, jobs, E.tun, hb, lb, walk, true, &ms)
-                              : run_batch_dispatch<5>(E, rule, jobs, E.tun, hb, lb, walk, true, &ms);
+                rc = (W == 4) ? run_batch_dispatch<4>(E, rule, jobs, gt, hb, lb, walk, true, &ms)
+                              : run_batch_dispatch<5>(E, rule, jobs, gt, hb, lb, walk, true, &ms);
                 for (auto &h : grp) h->dev_ms += ms;   // device time of the steps the request took part in
                 if (rc) {
                     for (auto &h : grp) { h->status = rc; h->err = t_err; h->next = h->rq.hi; }
@@ -684,12 +726,16 @@ int war_set_tuning(const war_tuning *t) {
     t_err.clear();
     if (!g_eng) return fail(60, "war_init has not been called");
     if (!t) return fail(1, "null tuning");
-    war_tuning r;
-    resolve_tuning(*g_eng, *t, r);
-    int rc = check_tuning(r);
+    war_tuning probe = default_tuning(13, WAR_MODE_FULL);
+    if (t->waves_per_cu > 0) probe.waves_per_cu = t->waves_per_cu;
+    if (t->ranks_per_refill > 0) probe.ranks_per_refill = t->ranks_per_refill;
+    if (t->brent_cap > 0) probe.brent_cap = t->brent_cap;
+    if (t->plain_per > 0) probe.plain_per = t->plain_per;
+    if (t->stage_kind > 0) probe.stage_kind = t->stage_kind;
+    int rc = check_tuning(probe);
     if (rc) return rc;
     std::lock_guard<std::mutex> g(g_eng->mu);
-    g_eng->tun = r;
+    g_eng->tun = *t;
     return 0;
 }
 
@@ -698,6 +744,17 @@ int war_get_tuning(war_tuning *t) {
     if (!g_eng) return fail(60, "war_init has not been called");
     if (!t) return fail(1, "null tuning");
     *t = g_eng->tun;
+    return 0;
+}
+
+int war_tuning_for(int32_t n, int32_t mode, int32_t rule, war_tuning *out) {
+    t_err.clear();
+    if (!g_eng) return fail(60, "war_init has not been called");
+    if (!out) return fail(1, "null tuning");
+    war_request r{};
+    r.n = n; r.mode = mode; r.rule = rule;
+    std::lock_guard<std::mutex> g(g_eng->mu);
+    resolve_tuning(*g_eng, r, *out);
     return 0;
 }
 
@@ -726,17 +783,22 @@ int war_autotune(int32_t n, int32_t mode, int32_t rule, double budget_ms, war_tu
     war_request r{};
     r.n = n; r.mode = mode; r.rule = rule; r.kernel = WAR_KERNEL_PERSISTENT;
     const u64 total = range_total(n, mode);
-    // sample: a slice from the middle of the range, big enough to fill the GPU several times
-    const u64 want = std::min<u64>(total, 1ull << 25);
+    // sample: a slice from the middle of the range, big enough to fill the GPU many times over
+    const u64 want = std::min<u64>(total, 1ull << 26);
     r.lo = total / 2 - std::min(total / 2, want / 2);
     r.hi = r.lo + want;
     int rc = check_request(&r);
     if (rc) return rc;
-    war_tuning cur = E.tun;
+    war_tuning cur;
+    {
+        std::lock_guard<std::mutex> g(E.mu);
+        resolve_tuning(E, r, cur);
+    }
     war_result ref_res{};
-    double best_ms = 1e30;
     bool have_ref = false;
     const double t_start = now_ms();
+    // time one schedule (second run: its graph is then already captured); every schedule must give
+    // the first one's counts exactly
     auto trial = [&](const war_tuning &t, double *ms) -> int {
         war_request q = r;
         q.tuning = t;
@@ -747,7 +809,7 @@ int war_autotune(int32_t n, int32_t mode, int32_t rule, double budget_ms, war_tu
         {
             std::lock_guard<std::mutex> g(E.mu);
             c = run_request_locked(E, h);
-            if (!c) { h.rq.tuning = t; c = run_request_locked(E, h); }   // second run: graph already built
+            if (!c) { ReqHost h2; h2.rq = q; h2.res = &o; c = run_request_locked(E, h2); }
         }
         if (c) return c;
         if (!have_ref) { ref_res = o; have_ref = true; }
@@ -756,24 +818,32 @@ int war_autotune(int32_t n, int32_t mode, int32_t rule, double budget_ms, war_tu
         *ms = o.device_ms;
         return 0;
     };
-    // coordinate descent over (brent_cap, ranks_per_refill, waves_per_cu)
+    double best_ms = 1e30;
+    rc = trial(cur, &best_ms);
+    if (rc) return rc;
+    // coordinate descent over (brent_cap, ranks_per_refill, waves_per_cu, stage_kind)
     const int caps[] = {7, 15, 31, 63, 127};
-    const int rprs[] = {64, 256, 1024, 4096};
-    const int wpcs[] = {4, 8, 16, 24, 32};
+    const int rprs[] = {256, 1024, 2048, 4096};
+    const int wpcs[] = {8, 16, 24, 28, 32};
+    const int kinds[] = {1, 2};
     for (int pass = 0; pass < 2; pass++) {
-        for (int dim = 0; dim < 3; dim++) {
-            const int nc = dim == 0 ? 5 : dim == 1 ? 4 : 5;
+        for (int dim = 0; dim < 4; dim++) {
+            const int nc = dim == 0 ? 5 : dim == 1 ? 4 : dim == 2 ? 5 : 2;
             war_tuning bestt = cur;
             for (int i = 0; i < nc; i++) {
-                if (now_ms() - t_start > budget_ms && have_ref) break;
+                if (now_ms() - t_start > budget_ms) break;
                 war_tuning t = cur;
                 if (dim == 0) t.brent_cap = caps[i];
                 else if (dim == 1) t.ranks_per_refill = rprs[i];
-                else t.waves_per_cu = wpcs[i];
+                else if (dim == 2) t.waves_per_cu = wpcs[i];
+                else t.stage_kind = kinds[i];
+                if (memcmp(&t, &cur, sizeof t) == 0) continue;
                 double ms = 0;
                 rc = trial(t, &ms);
                 if (rc) return rc;
-                if (E.verbose) fprintf(stderr, "autotune n=%d mode=%d cap=%d rpr=%d wpc=%d: %.3f ms\n", n, mode, t.brent_cap, t.ranks_per_refill, t.waves_per_cu, ms);
+                if (E.verbose)
+                    fprintf(stderr, "autotune n=%d mode=%d cap=%d tile=%d wpc=%d kind=%d: %.3f ms\n", n, mode,
+                            t.brent_cap, t.ranks_per_refill, t.waves_per_cu, t.stage_kind, ms);
                 if (ms < best_ms) { best_ms = ms; bestt = t; }
             }
             cur = bestt;
@@ -781,7 +851,7 @@ int war_autotune(int32_t n, int32_t mode, int32_t rule, double budget_ms, war_tu
     }
     {
         std::lock_guard<std::mutex> g(E.mu);
-        E.tun = cur;
+        E.tuned[std::make_tuple(n, mode, rule)] = cur;
     }
     if (best) *best = cur;
     return 0;

@@ -49,9 +49,11 @@ struct JobHdr {
 
 struct Pipe {
     u64 grab[MAXSTAGE];     // grab counter of stage s
-    u64 stotal[MAXSTAGE];   // records written by stage s (for the host's group sizing)
+    u64 stotal[MAXSTAGE];   // records written by stage s (sizes the next stage's grabs)
+    u64 nreg[MAXSTAGE];     // output regions allocated by refill stage s
     u64 wgrab, ctotal;      // walk grab counter; cycle records written
-    u64 pad[6];
+    u64 ovf;                // output regions refused for lack of room (an error; never with the sizing)
+    u64 pad[5];
 };
 
 // Survivor records live in per-tile regions of tile_pos slots: tile t's records are
@@ -113,6 +115,9 @@ template <int W> struct StageArgs {
     u32 a0, a1;            // ages played: (a0, a1]
     int stage;
     u32 hb, lb;            // LDS histogram bins per wave
+    u64 max_regions;       // regions of tile_pos slots each queue buffer holds
+    u32 in_refill;         // the input regions were written by a refill stage (count: nreg[stage-1])
+    u32 pad;
 };
 
 // ------------------------------------------------------------------------------------------
@@ -362,7 +367,7 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs<W> A) {
     acc.init(lds + wid * (A.hb + A.lb), A.hb, A.lb);
     const ReqDev *reqs = A.reqs;
     const bool last = (A.a1 == UNBOUNDED);
-    const u64 T_ = A.hdr->ntiles;
+    const u64 T_ = (!FRESH && A.in_refill) ? A.pipe->nreg[A.stage - 1] : A.hdr->ntiles;
     const u32 req0 = A.hdr->req0;
     const u32 TP = A.tile_pos;
     // tiles per grab: enough for ~256 records, from what the previous stage wrote in this batch
@@ -493,6 +498,243 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs<W> A) {
         }
         if (!last && out_n && lane == 0) atomicAdd((unsigned long long *)&A.pipe->stotal[A.stage], (unsigned long long)out_n);
     }
+    acc.flush(reqs);
+    if (acc.pushed && lane == 0) atomicAdd((unsigned long long *)&A.pipe->ctotal, (unsigned long long)acc.pushed);
+}
+
+// ------------------------------------------------------------------------------------------
+// Refill stage: ages (a0, a1] with a0 = 2^k - 1 and a1 = 2^(k+1) - 1. Brent's tortoise does not move
+// inside such a window (its next reset is exactly at a1), so a lane that finishes can take another
+// record of the same stage at once: the lane keeps its own turn count j (age = a0 + j, cycle length
+// = j), and nothing else about Brent's schedule changes. Records come from the previous stage's
+// regions through a 64-record buffer (loaded one buffer ahead, handed to idle lanes by cross-lane
+// reads); survivors (j reaches a1 - a0) are appended to output regions the wave allocates one at a
+// time (one atomic per region of tile_pos slots). In a multi-request batch a wave holds records of
+// one request at a time (it drains before switching), so all accounting stays per wave.
+template <int W>
+__device__ __forceinline__ typename Lay<W>::T shfl_T(typename Lay<W>::T v, u32 src) {
+    typedef typename Lay<W>::T T;
+    if constexpr (W == 4) {
+        return ((u64)(u32)__shfl((int)(u32)(v >> 32), (int)src) << 32) | (u64)(u32)__shfl((int)(u32)v, (int)src);
+    } else {
+        T r = 0;
+        for (int q = 3; q >= 0; q--) r = (r << 32) | (T)(u32)__shfl((int)(u32)(v >> (32 * q)), (int)src);
+        return r;
+    }
+}
+
+constexpr u32 REFILL_IDLE = 24;    // refill once this many lanes of 64 are idle
+
+template <int W, int RULE, int FLAGS>
+__global__ __launch_bounds__(256) void k_refill(StageArgs<W> A) {
+    typedef typename Lay<W>::T T;
+    extern __shared__ u32 lds[];
+    const u32 wid = threadIdx.x >> 6;
+    const u32 lane = lane_id();
+    WaveAcc acc;
+    acc.init(lds + wid * (A.hb + A.lb), A.hb, A.lb);
+    const ReqDev *reqs = A.reqs;
+    const bool last = (A.a1 == UNBOUNDED);
+    const u32 TP = A.tile_pos;
+    const u32 req0 = A.hdr->req0;
+    const u64 T_in = A.stage == 1 ? A.hdr->ntiles : A.pipe->nreg[A.stage - 1];
+    const u64 recs = A.pipe->stotal[A.stage - 1];
+    if (recs == 0) return;
+    const u32 G = group_size(recs, T_in, TP);
+    const u32 LEN = last ? AGE_CAP - A.a0 : A.a1 - A.a0;
+    const u32 a0 = A.a0;
+    u64 *grab = &A.pipe->grab[A.stage];
+    u64 nxt = 0;
+    if (lane == 0) nxt = atomicAdd((unsigned long long *)grab, 1ull);
+
+    // ---- input stream (wave-uniform cursor over groups of regions)
+    u64 g0 = 0;
+    u32 ng = 0, excl = 0, total = 0, cursor = 0;
+    bool stream_done = false;
+    // next buffer (being loaded) and current buffer (being handed out)
+    T xa = 0, xb = 0; u32 xreq = req0, xhome = 0; u64 xpos = 0; u32 x_n = 0;
+    T ca = 0, cb = 0; u32 creq = req0, chome = 0; u64 cpos = 0; u32 c_n = 0, c_i = 0;
+    auto load_next = [&]() {        // fill the next buffer with up to 64 records (all lanes)
+        x_n = 0;
+        while (cursor >= total) {
+            if (stream_done) return;
+            g0 = uni64(nxt) * G;
+            if (g0 >= T_in) { stream_done = true; return; }
+            if (lane == 0) nxt = atomicAdd((unsigned long long *)grab, 1ull);
+            ng = (u32)((T_in - g0) < G ? (T_in - g0) : G);
+            u32 c = lane < ng ? A.qin.cnt[g0 + lane] : 0u;
+            c = c < TP ? c : TP;
+            const u32 incl = wave_scan(c);
+            excl = incl - c;
+            total = lanev(incl, 63);
+            cursor = 0;
+        }
+        const u32 kk = cursor + lane;
+        u32 ti, off;
+        group_locate(excl, ng, kk < total ? kk : total - 1u, ti, off);
+        if (kk < total && off < TP) {
+            const u64 i = (g0 + ti) * (u64)TP + off;
+            xa = A.qin.a[i]; xb = A.qin.b[i];
+            if constexpr ((FLAGS & F_MULTI) != 0) xreq = A.qin.req[i];
+            if constexpr ((FLAGS & F_WALK) != 0) { xpos = A.qin.pos[i]; xhome = A.qin.home[i]; }
+        }
+        x_n = (total - cursor) < 64u ? (total - cursor) : 64u;
+        cursor += x_n;
+    };
+    load_next();
+
+    // ---- lanes (a finished lane keeps its final state, frozen, until the next refill accounts it)
+    St<W> s, t;
+    s.a = 0; s.b = 0; s.sa = 0; t = s;
+    u32 n = 2, C = 0, j = 0;
+    bool live = false;
+    LaneCtx L;
+    L.req = req0; L.pos = 0; L.home = 0; L.walk = false;
+    u64 pending = 0;    // lanes that finished and are not accounted yet
+    u32 cur_req = req0;
+    acc.select(req0, reqs);
+    u64 lturns = 0;     // per lane: sum of game lengths / detection rounds
+    u32 lmax = 0;       // per lane: longest ending game
+    // ---- output region
+    u32 och = UNBOUNDED, ofill = 0;
+
+    auto close_region = [&]() {
+        if (och != UNBOUNDED && lane == 0) {
+            A.qout.cnt[och] = ofill;
+            if (ofill) atomicAdd((unsigned long long *)&A.pipe->stotal[A.stage], (unsigned long long)ofill);
+        }
+    };
+    auto reduce_lanes = [&]() {     // per-lane totals into the wave accumulator
+        u64 tsum = lturns;
+        u32 tmax = lmax;
+        for (int off = 32; off > 0; off >>= 1) {
+            tsum += __shfl_xor(tsum, off);
+            const u32 o = __shfl_xor(tmax, off);
+            tmax = o > tmax ? o : tmax;
+        }
+        acc.turns += tsum;
+        acc.maxfin = tmax > acc.maxfin ? tmax : acc.maxfin;
+        acc.dirty = true;
+        lturns = 0; lmax = 0;
+    };
+    // Account the finished lanes of `pending` from their frozen states: ended / repeated / reached
+    // the stage end (survivor: appended to the output regions). All lanes call this.
+    auto settle = [&]() {
+        const bool mine = (pending >> lane) & 1ull;
+        const bool e = mine && ended<W>(s, C);
+        const bool d = mine && !e && same<W>(s, t);
+        const u32 age = a0 + j;
+        if (e) {
+            lturns += age; lmax = age > lmax ? age : lmax;
+            if (acc.lb) atomicAdd(&acc.ll[age < acc.lb - 1 ? age : acc.lb - 1], 1u);
+        }
+        if (d) {
+            lturns += age;
+            if (acc.hb) atomicAdd(&acc.lh[j < acc.hb - 1 ? j : acc.hb - 1], 1u);
+        }
+        const u64 em = ballot(e), dm = ballot(d);
+        const u64 sm = pending & ~em & ~dm;
+        acc.done += __popcll(em | dm); acc.cyc += __popcll(dm); acc.dirty = true;
+        if constexpr ((FLAGS & F_WALK) != 0) {
+            const u64 wm = ballot(L.walk) & dm;
+            if (wm) acc.pushed += push_cycles<W>(A.cq, TP, wm, s, n, L.req, j, L.pos, L.home);
+        }
+        if (sm) {
+            const u32 c = (u32)__popcll(sm);
+            const u32 r = rank_in(sm);
+            u32 placed = 0;
+            while (placed < c) {
+                if (och == UNBOUNDED || ofill == TP) {
+                    close_region();
+                    och = (u32)wave_fetch_add(&A.pipe->nreg[A.stage], 1);
+                    ofill = 0;
+                    if (och >= A.max_regions) {       // no room: an error, reported by the host
+                        if (lane == 0) atomicAdd((unsigned long long *)&A.pipe->ovf, 1ull);
+                        och = UNBOUNDED;
+                        break;
+                    }
+                }
+                const u32 kk = (TP - ofill) < (c - placed) ? (TP - ofill) : (c - placed);
+                if (((sm >> lane) & 1ull) && r >= placed && r < placed + kk) {
+                    const u64 i = (u64)och * TP + ofill + (r - placed);
+                    T qa, qb;
+                    pack<W>(s, n, qa, qb);
+                    A.qout.a[i] = qa; A.qout.b[i] = qb;
+                    if constexpr ((FLAGS & F_MULTI) != 0) A.qout.req[i] = L.req;
+                    if constexpr ((FLAGS & F_WALK) != 0) { A.qout.pos[i] = L.pos; A.qout.home[i] = L.home; }
+                }
+                ofill += kk; placed += kk;
+            }
+        }
+        pending = 0;
+    };
+
+    for (;;) {
+        // ---- refill idle lanes (when enough are idle, or all)
+        u64 idle = ~ballot(live);
+        if ((u32)__popcll(idle) >= REFILL_IDLE || idle == ~0ull) {
+            if (pending) settle();
+            for (;;) {
+                if (c_i >= c_n) {
+                    if (x_n == 0) break;                       // stream exhausted
+                    ca = xa; cb = xb; creq = xreq; chome = xhome; cpos = xpos; c_n = x_n; c_i = 0;
+                    load_next();                               // the following buffer, loaded ahead
+                }
+                if (!idle) break;
+                u32 k = (u32)__popcll(idle);
+                k = k < c_n - c_i ? k : c_n - c_i;
+                if constexpr ((FLAGS & F_MULTI) != 0) {
+                    // only records of the wave's current request; switch request once drained
+                    const u64 other = ballot(lane >= c_i && lane < c_n && creq != cur_req);
+                    const u32 first = other ? (u32)(__ffsll((long long)other) - 1) : c_n;
+                    if (first - c_i < k) k = first - c_i;
+                    if (k == 0) {
+                        if (idle != ~0ull) break;
+                        reduce_lanes();
+                        cur_req = lanev(creq, c_i);
+                        acc.select(cur_req, reqs);
+                        continue;
+                    }
+                }
+                const u32 r = rank_in(idle);
+                const u32 src = c_i + (r < k ? r : 0u);
+                const T na = shfl_T<W>(ca, src), nb = shfl_T<W>(cb, src);
+                u32 nreq = req0, nhome = 0;
+                u64 npos = 0;
+                if constexpr ((FLAGS & F_MULTI) != 0) nreq = (u32)__shfl((int)creq, (int)src);
+                if constexpr ((FLAGS & F_WALK) != 0) {
+                    nhome = (u32)__shfl((int)chome, (int)src);
+                    npos = ((u64)(u32)__shfl((int)(u32)(cpos >> 32), (int)src) << 32) | (u64)(u32)__shfl((int)(u32)cpos, (int)src);
+                }
+                const bool take = ((idle >> lane) & 1ull) && r < k;
+                if (take) {
+                    unpack<W>(na, nb, s, n);
+                    t = s; j = 0; C = W * n - 2u * W; live = true;
+                    L.req = nreq; L.pos = npos; L.home = nhome;
+                    if constexpr ((FLAGS & F_WALK) != 0) L.walk = reqs[nreq].walk != 0;
+                }
+                idle &= ~ballot(take);
+                c_i += k;
+            }
+            if (idle == ~0ull) break;                          // nothing left to play
+        }
+
+        // ---- two turns for the playing lanes (finished lanes stay frozen); the refill check and its
+        // mask bookkeeping run once per two turns
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            if (live) {
+                turn<W, RULE>(s, C);
+                ++j;
+                // bitwise, not short-circuit: no branch inside the turn
+                if (ended<W>(s, C) | same<W>(s, t) | (j == LEN)) live = false;
+            }
+        }
+        pending |= ballot(!live) & ~idle;
+    }
+    if (pending) settle();
+    close_region();
+    reduce_lanes();
     acc.flush(reqs);
     if (acc.pushed && lane == 0) atomicAdd((unsigned long long *)&A.pipe->ctotal, (unsigned long long)acc.pushed);
 }

@@ -18,7 +18,7 @@ KERNELS = {"plain": KERNEL_PLAIN, "persistent": KERNEL_PERSISTENT}
 
 class WarTuning(ctypes.Structure):
     _fields_ = [("waves_per_cu", ctypes.c_int32), ("ranks_per_refill", ctypes.c_int32),
-                ("brent_cap", ctypes.c_int32), ("plain_per", ctypes.c_int32)]
+                ("brent_cap", ctypes.c_int32), ("plain_per", ctypes.c_int32), ("stage_kind", ctypes.c_int32)]
 
 
 class WarRequest(ctypes.Structure):
@@ -76,6 +76,7 @@ _SIGS = {
     "war_run_range": ([ctypes.POINTER(WarRequest), ctypes.POINTER(WarResult)], ctypes.c_int),
     "war_set_tuning": ([ctypes.POINTER(WarTuning)], ctypes.c_int),
     "war_get_tuning": ([ctypes.POINTER(WarTuning)], ctypes.c_int),
+    "war_tuning_for": ([ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(WarTuning)], ctypes.c_int),
     "war_autotune": ([ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.POINTER(WarTuning)],
                      ctypes.c_int),
     "war_serve_start": ([ctypes.c_int32, ctypes.c_uint64], ctypes.c_int),

@@ -25,13 +25,22 @@ def total_positions(n, mode):
     return math.factorial(n)
 
 
+STAGE_KINDS = {"refill": 1, "lockstep": 2}
+
+
 def _tuning(t):
     if t is None:
-        return WarTuning(0, 0, 0, 0)
+        return WarTuning(0, 0, 0, 0, 0)
     if isinstance(t, WarTuning):
         return t
+    sk = t.get("stage_kind", 0)
     return WarTuning(int(t.get("waves_per_cu", 0)), int(t.get("ranks_per_refill", 0)),
-                     int(t.get("brent_cap", 0)), int(t.get("plain_per", 0)))
+                     int(t.get("brent_cap", 0)), int(t.get("plain_per", 0)), int(STAGE_KINDS.get(sk, sk)))
+
+
+def _tuning_dict(t):
+    return {"waves_per_cu": t.waves_per_cu, "ranks_per_refill": t.ranks_per_refill, "brent_cap": t.brent_cap,
+            "plain_per": t.plain_per, "stage_kind": {1: "refill", 2: "lockstep"}.get(t.stage_kind, t.stage_kind)}
 
 
 def make_request(n, mode="full", lo=0, hi=None, rule=0, kernel="persistent", hbins=0, lbins=0, dbins=0,
@@ -94,8 +103,13 @@ class Engine:
     def get_tuning(self):
         t = WarTuning()
         check(self.lib.war_get_tuning(ctypes.byref(t)))
-        return {"waves_per_cu": t.waves_per_cu, "ranks_per_refill": t.ranks_per_refill,
-                "brent_cap": t.brent_cap, "plain_per": t.plain_per}
+        return _tuning_dict(t)
+
+    def tuning_for(self, n, mode="full", rule=0):
+        """The tuning a request of (n, mode, rule) runs with when it sets none itself."""
+        t = WarTuning()
+        check(self.lib.war_tuning_for(int(n), int(MODES.get(mode, mode)), int(rule), ctypes.byref(t)))
+        return _tuning_dict(t)
 
     def set_tuning(self, **kw):
         t = _tuning(kw)
@@ -104,8 +118,7 @@ class Engine:
     def autotune(self, n, mode="canon", rule=0, budget_ms=20000.0):
         t = WarTuning()
         check(self.lib.war_autotune(int(n), int(MODES.get(mode, mode)), int(rule), float(budget_ms), ctypes.byref(t)))
-        return {"waves_per_cu": t.waves_per_cu, "ranks_per_refill": t.ranks_per_refill,
-                "brent_cap": t.brent_cap, "plain_per": t.plain_per}
+        return _tuning_dict(t)
 
     # ---- synchronous range query
     def run_range(self, n, mode="full", lo=0, hi=None, **kw):

@@ -42,10 +42,12 @@ extern "C" {
 #define WAR_KERNEL_PERSISTENT 1  /* persistent, age-synchronous stages with survivor compaction */
 
 typedef struct war_tuning {
-    int32_t waves_per_cu;      /* persistent grid = CUs x waves_per_cu wave64s (1..16)          */
-    int32_t ranks_per_refill;  /* positions a wave takes per atomic grab (64..8192, mult of 64)  */
-    int32_t brent_cap;         /* first stage boundary 2^k-1 (7..1023); later stages double     */
+    int32_t waves_per_cu;      /* persistent grid = CUs x waves_per_cu wave64s (1..32)          */
+    int32_t ranks_per_refill;  /* tile: positions per grab / slots per queue region (64..8192)  */
+    int32_t brent_cap;         /* first stage boundary 2^k-1 (1..65535); later stages double    */
     int32_t plain_per;         /* plain kernel: consecutive positions per lane (1..4096)        */
+    int32_t stage_kind;        /* persistent stages after the first: 1 = refill (a lane that
+                                  finishes takes the next record at once), 2 = lockstep; 0 = default */
 } war_tuning;
 
 typedef struct war_request {
@@ -101,12 +103,17 @@ int war_device_info(war_devinfo *out);
 /* Synchronous range query: plays every position of [lo, hi) on the GPU and fills *res. */
 int war_run_range(const war_request *req, war_result *res);
 
-/* Engine-wide default tuning (used when a request's tuning is zero). */
+/* Engine-wide tuning override. A request's non-zero tuning fields win; then this override's
+ * non-zero fields; then war_autotune's choice for the request's (n, mode, rule); then a built-in
+ * default per (n, mode). war_get_tuning returns the override (zeros = none). */
 int war_set_tuning(const war_tuning *t);
 int war_get_tuning(war_tuning *t);
+/* The tuning a request of (n, mode, rule) with zero tuning fields would run with. */
+int war_tuning_for(int32_t n, int32_t mode, int32_t rule, war_tuning *out);
 
-/* Autotuner: times candidate schedules of the persistent kernel on a sample of (n, mode) and
- * installs the fastest as the default. budget_ms bounds the search; *best receives the choice. */
+/* Autotuner: times candidate schedules of the persistent kernel on a 2^26-position slice of
+ * (n, mode, rule), checks that every schedule gives identical counts, and keeps the fastest as the
+ * default for that workload. budget_ms bounds the search; *best receives the choice. */
 int war_autotune(int32_t n, int32_t mode, int32_t rule, double budget_ms, war_tuning *best);
 
 /* Continuous-batching request loop. Requests submitted while others run are admitted into free

@@ -148,6 +148,38 @@ def main():
             key = "n17_%d_%d" % (lo, hi)
             g["long"][key] = {"n": 17, "lo": lo, "hi": hi, "cycling": int(m.group(6)), "longest": int(m.group(7))}
 
+    # war_gpu.py command lines with their logged output (timing fields differ by hardware)
+    def block(lines_, start_pat, stop_pat):
+        out, on = [], False
+        for ln in lines_:
+            if on and (ln.startswith("$") or ln.startswith("==") or not ln.strip() or re.match(stop_pat, ln)):
+                break
+            if on:
+                out.append(ln)
+            if ln.startswith("$") and start_pat in ln:
+                on = True
+        return out
+    g["cli"] = []
+    deals = block(chk, "war_gpu.py 13 --nmin 1 --hist 256", r"^$")
+    g["cli"].append({"args": ["13", "--nmin", "1", "--hist", "256"], "lines": deals, "source": "data/checks.txt:5-24"})
+    canon = block(chk, "war_gpu.py 4 --nmin 1 --canon --distinct --shape", r"^$")
+    g["cli"].append({"args": ["4", "--nmin", "1", "--canon", "--distinct", "--shape"],
+                     "lines": [ln for ln in canon if not re.match(r"n=(\d+)", ln) or int(re.match(r"n=(\d+)", ln).group(1)) <= 4][:12],
+                     "source": "data/checks.txt:146-157"})
+    g["cli"].append({"args": ["13", "--nmin", "5", "--canon", "--distinct", "--shape"], "lines": canon[12:],
+                     "source": "data/checks.txt:158-182"})
+    r1 = block(chk, "war_gpu.py 13 --nmin 2 --rule 1", r"^loser-first")
+    g["cli"].append({"args": ["13", "--nmin", "2", "--rule", "1"], "lines": [ln for ln in r1 if ln.startswith("n=")],
+                     "source": "data/checks.txt:370-381"})
+    r1c = block(chk, "war_gpu.py 12 --nmin 2 --canon --rule 1", r"^n=\d+ rule=")
+    g["cli"].append({"args": ["12", "--nmin", "2", "--canon", "--rule", "1"], "lines": r1c,
+                     "source": "data/checks.txt:424-434"})
+    c14 = block(lr, "war_gpu.py 14 --canon --distinct --shape --hist 256", r"^$")
+    g["cli"].append({"args": ["14", "--canon", "--distinct", "--shape", "--hist", "256"], "lines": c14,
+                     "source": "data/long_runs.txt:19-23"})
+    h14 = block(lr, "war_gpu.py 14 --half --hist 256", r"^n=15")
+    g["cli"].append({"args": ["14", "--half", "--hist", "256"], "lines": h14[:2], "source": "data/long_runs.txt:13-15"})
+
     with open(OUT, "w") as f:
         json.dump(g, f, indent=1, sort_keys=True)
     print("wrote", OUT)

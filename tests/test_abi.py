@@ -28,7 +28,7 @@ def test_no_python_fallback(monkeypatch, tmp_path):
 
 
 STRUCTS = {
-    "war_tuning": (_lib.WarTuning, ["waves_per_cu", "ranks_per_refill", "brent_cap", "plain_per"]),
+    "war_tuning": (_lib.WarTuning, ["waves_per_cu", "ranks_per_refill", "brent_cap", "plain_per", "stage_kind"]),
     "war_request": (_lib.WarRequest, ["n", "mode", "rule", "kernel", "lo", "hi", "hbins", "lbins", "dbins",
                                       "distinct", "shape_blk", "tuning"]),
     "war_result": (_lib.WarResult, ["cyc", "done", "maxfin", "turns", "oncycle", "distinct_minkey",

@@ -77,6 +77,8 @@ def test_distinct_partial_ranges(eng, orc, n):
     {"brent_cap": 1023, "ranks_per_refill": 1024, "waves_per_cu": 16},
     {"brent_cap": 1, "ranks_per_refill": 64, "waves_per_cu": 2},      # sparse tiles: grabs span many tiles
     {"brent_cap": 3, "ranks_per_refill": 8192, "waves_per_cu": 1},
+    {"brent_cap": 15, "ranks_per_refill": 1024, "waves_per_cu": 32, "stage_kind": "lockstep"},
+    {"brent_cap": 1, "ranks_per_refill": 256, "waves_per_cu": 8, "stage_kind": "lockstep"},
 ])
 def test_schedules_are_exact(eng, orc, tuning):
     check(eng, orc, 13, "canon", 123_456_789, 123_456_789 + 2_000_000, tuning=tuning)
