@@ -85,6 +85,9 @@ def run(n, half=False, per=256, threads=256, batch_threads=1 << 18, out=None, re
     while start < total:
         tl = time.time()
         stop = min(total, start + span)
+        if canon and n < 2:      # one order of no cards and no split: nothing to play (gpu/war_gpu.py:209)
+            start = stop
+            continue
         r = eng.run_range(n, mode, start * unit, stop * unit, rule=rule, kernel=kernel, hbins=hbins, lbins=lbins,
                           dbins=dbins, shape_blk=blk, tuning=tuning)
         cyc += r["cyc"]; done += r["done"]; maxfin = max(maxfin, r["maxfin"])

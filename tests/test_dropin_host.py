@@ -101,3 +101,8 @@ def test_cli_lines(fake, capsys):
     assert re.match(r"n=6  a\(n\)=\d+  of 720  longest finite=\d+ rounds  \d+\.\ds$", lines[0])
     assert re.match(r"   loop lengths \(rounds: deals\): 7: \d+$", lines[1])
     assert lines[2].startswith("n=7  a(n)=")
+
+
+def test_canon_n1_plays_nothing(fake):
+    c, d, mx, el, h, sh, dd = war_gpu.run(1, canon=True, shape=True, dbins=4096)
+    assert (c, d, mx) == (0, 0, 0) and fake.calls == 0
