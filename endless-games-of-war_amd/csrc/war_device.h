@@ -47,11 +47,28 @@ template <class T> __device__ __forceinline__ T lowmask_(u32 s) { return s >= 8 
 
 template <int W> __device__ __forceinline__ u32 bits1(const St<W> &s) { return s.sa + W; }   // W * cards of player 1
 
+// Two 64-bit words for the two-card values of a turn, whose high halves stay zero for the whole game
+// loop. Built from a zero the compiler cannot see (a kernel argument), they keep one register per
+// high half instead of a shared zero register that would be copied before every turn.
+// The same zero, OR-ed into sa's low bits when a game loop starts (open_sa), hides from the
+// compiler that sa is a multiple of W; otherwise it narrows the shift-count masks below to a
+// constant the hardware's own count masking does not cover, and emits an AND per shift.
+template <int W> struct Zh {};
+template <> struct Zh<4> { u64 h1, h2; };
+template <int W> __device__ __forceinline__ void open_sa(St<W> &s, u32 zero) { s.sa |= zero & 3u; }
+
+template <int W> __device__ __forceinline__ Zh<W> make_zh(u32 zero) {
+    Zh<W> z;
+    if constexpr (W == 4) { z.h1 = (u64)zero << 32; z.h2 = z.h1; }
+    (void)zero;
+    return z;
+}
+
 // One turn (gpu/war_gpu.py:38-45). Rule 0: the winner puts the winning card, then the losing
 // card, under its pile; rule 1 reverses the two. C = W * (n - 2). Returns true if player 1 won.
 // Branch-free: each pile ORs in its own two-card value (zero for the loser) at its own offset.
 template <int W, int RULE>
-__device__ __forceinline__ bool turn(St<W> &s, u32 C) {
+__device__ __forceinline__ bool turn(St<W> &s, u32 C, Zh<W> &z) {
     typedef typename Lay<W>::T T;
     constexpr u32 M = (1u << W) - 1u;
     const u32 c1 = (u32)s.a & M, c2 = (u32)s.b & M;
@@ -64,8 +81,15 @@ __device__ __forceinline__ bool turn(St<W> &s, u32 C) {
     // the masks only matter for a finished lane (whose state is no longer read); for u64 they are
     // free, the hardware shift already takes the count modulo 64
     constexpr u32 SM = 8 * sizeof(T) - 1;
-    s.a |= (T)x1 << (s.sa & SM);
-    s.b |= (T)x2 << ((C - s.sa) & SM);
+    if constexpr (W == 4) {
+        z.h1 = (z.h1 & 0xFFFFFFFF00000000ull) | x1;
+        z.h2 = (z.h2 & 0xFFFFFFFF00000000ull) | x2;
+        s.a |= z.h1 << (s.sa & SM);
+        s.b |= z.h2 << ((C - s.sa) & SM);
+    } else {
+        s.a |= (T)x1 << (s.sa & SM);
+        s.b |= (T)x2 << ((C - s.sa) & SM);
+    }
     s.sa += p1 ? (u32)W : (u32)(-W);
     return p1;
 }

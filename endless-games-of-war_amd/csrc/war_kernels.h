@@ -117,7 +117,7 @@ template <int W> struct StageArgs {
     u32 hb, lb;            // LDS histogram bins per wave
     u64 max_regions;       // regions of tile_pos slots each queue buffer holds
     u32 in_refill;         // the input regions were written by a refill stage (count: nreg[stage-1])
-    u32 pad;
+    u32 zero;              // always 0 (see Zh)
 };
 
 // ------------------------------------------------------------------------------------------
@@ -286,13 +286,15 @@ __device__ __forceinline__ u64 play(const StageArgs<W> &A, WaveAcc &acc, St<W> &
                                     bool uniform_req, const LaneCtx &L) {
     const ReqDev *reqs = A.reqs;
     const bool last = (A.a1 == UNBOUNDED);
+    open_sa<W>(s, A.zero);
     St<W> t = s;
     u32 R = A.a0;
     u32 age = A.a0;
     const u32 a1 = last ? AGE_CAP : A.a1;
+    Zh<W> z = make_zh<W>(A.zero);
     for (;;) {
         ++age;
-        turn<W, RULE>(s, C);
+        turn<W, RULE>(s, C, z);
         const u64 em = ballot(ended<W>(s, C)) & alive;
         // three ballots of plain compares: each is one v_cmp into an SGPR pair
         const u64 dm = ballot(s.a == t.a) & ballot(s.b == t.b) & ballot(s.sa == t.sa) & alive & ~em;
@@ -669,6 +671,7 @@ __global__ __launch_bounds__(256) void k_refill(StageArgs<W> A) {
         pending = 0;
     };
 
+    Zh<W> z = make_zh<W>(A.zero);
     for (;;) {
         // ---- refill idle lanes (when enough are idle, or all)
         u64 idle = ~ballot(live);
@@ -709,6 +712,7 @@ __global__ __launch_bounds__(256) void k_refill(StageArgs<W> A) {
                 const bool take = ((idle >> lane) & 1ull) && r < k;
                 if (take) {
                     unpack<W>(na, nb, s, n);
+                    open_sa<W>(s, A.zero);
                     t = s; j = 0; C = W * n - 2u * W; live = true;
                     L.req = nreq; L.pos = npos; L.home = nhome;
                     if constexpr ((FLAGS & F_WALK) != 0) L.walk = reqs[nreq].walk != 0;
@@ -724,7 +728,7 @@ __global__ __launch_bounds__(256) void k_refill(StageArgs<W> A) {
 #pragma unroll
         for (int u = 0; u < 2; u++) {
             if (live) {
-                turn<W, RULE>(s, C);
+                turn<W, RULE>(s, C, z);
                 ++j;
                 // bitwise, not short-circuit: no branch inside the turn
                 if (ended<W>(s, C) | same<W>(s, t) | (j == LEN)) live = false;
@@ -745,6 +749,7 @@ __global__ __launch_bounds__(256) void k_refill(StageArgs<W> A) {
 template <int W> struct PlainArgs {
     u64 lo, hi;
     u32 per, hb, lb, tile_pos;   // tile_pos: slots per home tile of the cycle records
+    u32 zero, pad;               // zero: always 0 (see Zh)
     ReqDev R;
     u32 req;
     CycQueue<W> cq;
@@ -775,10 +780,12 @@ __global__ __launch_bounds__(256) void k_plain(PlainArgs<W> P) {
         u32 lam = 0;
         u64 rounds = 0;
         if (take && n >= 2 && !ended<W>(s, C)) {
+            open_sa<W>(s, P.zero);
             St<W> t = s;
             u32 power = 1, l = 0;
+            Zh<W> z = make_zh<W>(P.zero);
             for (;;) {
-                turn<W, RULE>(s, C);
+                turn<W, RULE>(s, C, z);
                 rounds++;
                 if (ended<W>(s, C)) break;
                 l++;
@@ -845,6 +852,7 @@ template <int W> struct WalkArgs {
     const ReqDev *reqs;
     CycQueue<W> cq;
     u32 tile_pos;
+    u32 zero;      // always 0 (see Zh)
 };
 
 // Insert key K into the open-addressing set; returns 1 if it was new. The probe length is bounded;
@@ -866,6 +874,7 @@ template <int W, int RULE>
 __global__ __launch_bounds__(256) void k_walk(WalkArgs<W> A) {
     typedef typename Lay<W>::T T;
     const u32 lane = lane_id();
+    Zh<W> z = make_zh<W>(A.zero);
     const u64 T_ = A.hdr->ntiles;
     const u64 recs = A.pipe->ctotal;
     if (recs == 0) return;
@@ -912,8 +921,9 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs<W> A) {
             u32 maxL = L;
             for (int off = 32; off > 0; off >>= 1) { const u32 o = __shfl_xor(maxL, off); maxL = o > maxL ? o : maxL; }
             St<W> v = m;
+            open_sa<W>(v, A.zero);
             for (u32 t = 0; t < maxL; t++) {
-                const u32 w = turn<W, RULE>(v, C) ? 1u : 0u;
+                const u32 w = turn<W, RULE>(v, C, z) ? 1u : 0u;
                 if (t < L) {
                     if (t == 0) { first = w; cur = w; run = 1; }
                     else if (w == cur) run++;
@@ -950,8 +960,9 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs<W> A) {
                 const u32 L = dist ? lam : 0u;
                 u32 maxL = L;
                 for (int off = 32; off > 0; off >>= 1) { const u32 o = __shfl_xor(maxL, off); maxL = o > maxL ? o : maxL; }
+                open_sa<W>(s0, A.zero);
                 St<W> v = s0;
-                for (u32 t = 0; t < maxL; t++) if (t < L) turn<W, RULE>(v, C);
+                for (u32 t = 0; t < maxL; t++) if (t < L) turn<W, RULE>(v, C, z);
                 const bool onc = dist && same<W>(v, s0);
                 if (ballot(onc)) {
                     // (3) weight, minimum key and hash insert for positions on their cycle
@@ -968,7 +979,7 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs<W> A) {
                             if (b1 > 0 && (v.a & 15ull) == top) { c1n++; const u64 kk = key4(v.a, v.b, b1); K = kk < K ? kk : K; }
                             if (b1 < B && (v.b & 15ull) == top) { c2n++; const u64 kk = key4(v.b, v.a, B - b1); K = kk < K ? kk : K; }
                             if (v.a == s0.b && v.b == s0.a && b1 == B - bits1<W>(s0)) sym = true;
-                            turn<W, RULE>(v, C);
+                            turn<W, RULE>(v, C, z);
                         }
                     }
                     if (onc) {
