@@ -14,7 +14,7 @@ One step = one whole enumeration. Beside it, on the same GPU in the same run:
                  age-synchronous loop) against CUs x 64 lanes x clock;
   cpu_baseline   the CPU oracle (OpenMP) on a bounded sample of the same n = 13 workload.
 
-  python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--only main|canon14] [--no-extras]
+  python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--only main|canon14|plain13|copy] [--no-extras]
 """
 import argparse
 import json
@@ -31,9 +31,13 @@ METRIC = "deals/sec (n=13 full, n=14 canon), bit-exact; batched vs plain kernel;
 PUBLISHED_N13_DEALS_PER_S = 6227020800 / 15.3      # RTX 5080, war_gpu.py 13 (reference data/checks.txt:23)
 A13 = {"cyc": 4880546113, "maxfin": 57, "hist": {42: 1475465743, 56: 2394985450, 70: 1010094920}}
 C14 = {"cyc": 2872129173, "maxfin": 465, "hist": {96: 43669233, 112: 2828459940}}
-# VALU instructions per War turn in the age-synchronous loop (turn + end test + three repeat
-# compares), counted in the gfx950 assembly of k_stage; the algorithmic unit is one Brent turn.
-VALU_OPS_PER_TURN = 24
+# VALU instructions per War turn in the age-synchronous loop (18 for the turn, one end test, one
+# compare of player 1's pile with the tortoise's), counted in the gfx950 assembly of k_stage; the
+# algorithmic unit is one Brent turn.
+VALU_OPS_PER_TURN = 20
+# HBM bytes per k_stage launch on the n = 13 workload (survivor records; 209 GB over 144 launches),
+# from the rocprofv3 byte counters calibrated on a copy of known size.
+K_STAGE_HBM_BYTES_PER_LAUNCH = 1.45e9
 
 
 def log(*a):
@@ -77,8 +81,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--only", choices=["main", "canon14", "plain13"], default=None,
-                    help="run one workload only (for profiling)")
+    ap.add_argument("--only", choices=["main", "canon14", "plain13", "copy"], default=None,
+                    help="run one workload only (for profiling); copy: a 1 GiB device-to-device copy that "
+                         "calibrates the HBM byte counters")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--cpu-chunks", type=int, default=64)
     ap.add_argument("--autotune", action="store_true", help="autotune n=13 full and n=14 canon first")
@@ -87,6 +92,17 @@ def main():
         print("bench.py: this engine runs on one GPU; --gpus %d is not supported (the rank space would be "
               "split across GPUs with one all-reduce of the counters, not built here)" % a.gpus, file=sys.stderr)
         sys.exit(2)
+
+    if a.only == "copy":
+        import torch
+        x = torch.ones(1 << 28, dtype=torch.int32, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        assert int(y[-1]) == 1
+        log("copy: 1 GiB read + 1 GiB written")
+        return
 
     from war_amd import Engine
     eng = Engine(0)
@@ -138,10 +154,11 @@ def main():
                    "n": 13, "mode": "full", "rule": 0, "positions": n13, "kernel": "persistent", "tuning": tun,
                    "parity": "a(13), longest 57, cycle-length histogram equal the published values every step"},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": peak_lane_ops / 1e12,
-                     "unit": "Tlane-op/s", "frac": achieved / peak_lane_ops, "traffic": None,
+                     "unit": "Tlane-op/s", "frac": achieved / peak_lane_ops, "traffic": K_STAGE_HBM_BYTES_PER_LAUNCH,
                      "kernel": "k_stage: the stage kernels of each batch graph (FRESH + queue stages)",
-                     "traffic_note": "HBM bytes per batch from rocprofv3 FETCH_SIZE/WRITE_SIZE are in "
-                                     "profiles/k_stage_n13_full_pmc.md (not measured inside this run)",
+                     "traffic_note": "HBM bytes per k_stage launch at this workload from rocprofv3 FETCH_SIZE and "
+                                     "WRITE_SIZE (scaled by a 1 GiB copy: x2.0 and x1.0), profiles/"
+                                     "k_stage_n13_full_pmc.md; not measured inside this run",
                      "turns_per_step": turns // a.steps, "valu_ops_per_turn": VALU_OPS_PER_TURN,
                      "device_ms_per_step": dev_ms / a.steps,
                      "note": "achieved = Brent turns x VALU ops per turn / device time (HIP events)"},

@@ -8,7 +8,7 @@
 //  k_stage<FRESH>   the persistent, age-synchronous form. Every lane of a wave starts its game at
 //                   the same age, so Brent's tortoise resets (ages 2^k - 1) and every game length
 //                   or cycle length seen at a turn are wave-uniform: the per-turn work is the turn
-//                   plus three compares, and all accounting is scalar. A stage plays ages
+//                   plus two compares, and all accounting is scalar. A stage plays ages
 //                   (a0, a1] with a1 = 2^k - 1; lanes still running at a1 are compacted into the
 //                   next stage's queue (the tortoise equals the hare there, so a record is just the
 //                   state). The FRESH stage enumerates positions from the job list (a global tile
@@ -292,37 +292,47 @@ __device__ __forceinline__ u64 play(const StageArgs<W> &A, WaveAcc &acc, St<W> &
     u32 age = A.a0;
     const u32 a1 = last ? AGE_CAP : A.a1;
     Zh<W> z = make_zh<W>(A.zero);
+    // Brent windows (R, 2R + 1]: the tortoise stays put inside one. The inner loop is the turn, the
+    // end test and one compare of player 1's pile with the tortoise's (a candidate repeat); it leaves
+    // on any candidate or end, and the other two fields are compared only then.
     for (;;) {
-        ++age;
-        turn<W, RULE>(s, C, z);
-        const u64 em = ballot(ended<W>(s, C)) & alive;
-        // three ballots of plain compares: each is one v_cmp into an SGPR pair
-        const u64 dm = ballot(s.a == t.a) & ballot(s.b == t.b) & ballot(s.sa == t.sa) & alive & ~em;
-        if (em | dm) {
-            const u32 lam = age - R;
-            if (uniform_req) {
-                if (em) acc.ended(__popcll(em), age);
-                if (dm) acc.cycled(__popcll(dm), age, lam);
-            } else {
-                u64 rem = em | dm;
-                while (rem) {
-                    const u32 r = lanev(L.req, (u32)(__ffsll((long long)rem) - 1));
-                    const u64 mr = ballot(L.req == r) & rem;
-                    rem &= ~mr;
-                    acc.select(r, reqs);
-                    if (mr & em) acc.ended(__popcll(mr & em), age);
-                    if (mr & dm) acc.cycled(__popcll(mr & dm), age, lam);
+        const u32 wend = (2u * R + 1u) < a1 ? (2u * R + 1u) : a1;
+        while (age < wend) {
+            u64 em, dm;
+            do {
+                ++age;
+                turn<W, RULE>(s, C, z);
+                em = ballot(ended<W>(s, C)) & alive;
+                dm = ballot(s.a == t.a) & alive;
+            } while (!(em | dm) && age < wend);
+            dm &= ~em;
+            if (dm) dm &= ballot(s.b == t.b) & ballot(s.sa == t.sa);
+            if (em | dm) {
+                const u32 lam = age - R;
+                if (uniform_req) {
+                    if (em) acc.ended(__popcll(em), age);
+                    if (dm) acc.cycled(__popcll(dm), age, lam);
+                } else {
+                    u64 rem = em | dm;
+                    while (rem) {
+                        const u32 r = lanev(L.req, (u32)(__ffsll((long long)rem) - 1));
+                        const u64 mr = ballot(L.req == r) & rem;
+                        rem &= ~mr;
+                        acc.select(r, reqs);
+                        if (mr & em) acc.ended(__popcll(mr & em), age);
+                        if (mr & dm) acc.cycled(__popcll(mr & dm), age, lam);
+                    }
                 }
+                if ((FLAGS & F_WALK) && dm) {
+                    const u64 wm = ballot(L.walk) & dm;
+                    if (wm) acc.pushed += push_cycles<W>(A.cq, A.tile_pos, wm, s, n, L.req, lam, L.pos, L.home);
+                }
+                alive &= ~(em | dm);
+                if (!alive) break;
             }
-            if ((FLAGS & F_WALK) && dm) {
-                const u64 wm = ballot(L.walk) & dm;
-                if (wm) acc.pushed += push_cycles<W>(A.cq, A.tile_pos, wm, s, n, L.req, lam, L.pos, L.home);
-            }
-            alive &= ~(em | dm);
-            if (!alive) break;
         }
-        if (age == a1) break;
-        if ((age & (age + 1u)) == 0u) { t = s; R = age; }   // Brent: tortoise := hare
+        if (!alive || age == a1) break;
+        t = s; R = age;                                      // Brent: tortoise := hare at 2^k - 1
     }
     if (alive && last) {
         // safety stop reached: counted as undetermined (an error for the caller)
