@@ -161,6 +161,14 @@ int check_request(const war_request *r) {
         return fail(9, "distinct / dbins need canon mode and n <= 15");
     if (r->shape_blk && r->mode != WAR_MODE_CANON) return fail(10, "shape needs canon mode");
     if (r->shape_blk < 0 || r->shape_blk > 512) return fail(10, "shape_blk must be in [0, 512]");
+    if (r->detector != WAR_DET_BRENT && r->detector != WAR_DET_STREAK) return fail(12, "unknown detector");
+    if (r->detector == WAR_DET_STREAK) {
+        if ((r->n & 1) == 0) return fail(12, "the streak test is valid for odd n only (Spivey Thm 6)");
+        if (r->rule != 0) return fail(12, "the streak test needs rule 0");
+        if (r->kernel != WAR_KERNEL_PLAIN) return fail(12, "the streak test runs in the plain kernel");
+        if (r->hbins || r->dbins || r->distinct || r->shape_blk)
+            return fail(12, "the streak test gives no cycle lengths: hbins, dbins, distinct and shape must be 0");
+    }
     return 0;
 }
 
@@ -453,7 +461,10 @@ int run_plain_batch(Engine &E, ReqHost &h, u64 lo, u64 hi, const war_tuning &t, 
     HIPCHK(hipEventRecord(E.ev0, E.st));
     hipLaunchKernelGGL(k_setup, dim3(256), dim3(256), 0, E.st, E.pipe, (const JobHdr *)E.hdr,
                        walk ? qbufs<W>(E).ccnt : (u32 *)nullptr);
-    hipLaunchKernelGGL((k_plain<W, RULE>), dim3((unsigned)blocks), dim3(256), lds, E.st, P);
+    if (RULE == 0 && h.rq.detector == WAR_DET_STREAK)
+        hipLaunchKernelGGL((k_plain<W, 0, 1>), dim3((unsigned)blocks), dim3(256), lds, E.st, P);
+    else
+        hipLaunchKernelGGL((k_plain<W, RULE, 0>), dim3((unsigned)blocks), dim3(256), lds, E.st, P);
     HIPCHK(hipGetLastError());
     if (walk) {
         WalkArgs<W> WA{};

@@ -766,7 +766,16 @@ template <int W> struct PlainArgs {
     Pipe *pipe;
 };
 
-template <int W, int RULE>
+// The streak test (gpu/war24.py:98-125; odd n, rule 0): anchor at a turn whose winner-to-be holds an
+// even number ws of cards and the other ls >= 2; if the players then alternate for
+// lcm(n+1, ws, ls-1) turns and the state equals the anchor, the game never ends (Spivey Thm 6). A
+// repeated winner drops the anchor. More than 2^22 turns: undetermined (the reference's CAP).
+constexpr u32 STREAK_CAP = 1u << 22;
+__device__ __forceinline__ u32 gcd_u(u32 a, u32 b) { while (b) { const u32 t = a % b; a = b; b = t; } return a; }
+__device__ __forceinline__ u32 lcm_u(u32 a, u32 b) { return a / gcd_u(a, b) * b; }
+
+// DET: 0 = Brent, 1 = the streak test (RULE 0 only; no cycle lengths, so no histogram or walks).
+template <int W, int RULE, int DET>
 __global__ __launch_bounds__(256) void k_plain(PlainArgs<W> P) {
     typedef typename Lay<W>::T T;
     extern __shared__ u32 lds[];
@@ -786,31 +795,64 @@ __global__ __launch_bounds__(256) void k_plain(PlainArgs<W> P) {
     for (u32 j = 0; j < cnt; j++) {
         St<W> s = (mode == 2) ? canon<W>(q, (int)n, k) : deal<W>(q, (int)n);
         const bool take = (mode == 1) ? top_card_to_p1<W>(q, (int)n) : true;
-        bool cy = false;
+        bool cy = false, und = false;
         u32 lam = 0;
         u64 rounds = 0;
         if (take && n >= 2 && !ended<W>(s, C)) {
             open_sa<W>(s, P.zero);
             St<W> t = s;
-            u32 power = 1, l = 0;
             Zh<W> z = make_zh<W>(P.zero);
-            for (;;) {
-                turn<W, RULE>(s, C, z);
-                rounds++;
-                if (ended<W>(s, C)) break;
-                l++;
-                if (same<W>(s, t)) { cy = true; lam = l; break; }
-                if (l == power) { t = s; power <<= 1; l = 0; }
-                if (rounds >= AGE_CAP) { undet++; break; }
+            if constexpr (DET == 0) {
+                u32 power = 1, l = 0;
+                for (;;) {
+                    turn<W, RULE>(s, C, z);
+                    rounds++;
+                    if (ended<W>(s, C)) break;
+                    l++;
+                    if (same<W>(s, t)) { cy = true; lam = l; break; }
+                    if (l == power) { t = s; power <<= 1; l = 0; }
+                    if (rounds >= AGE_CAP) { und = true; break; }
+                }
+            } else {
+                constexpr u32 M = (1u << W) - 1u;
+                bool anch = false;
+                u32 lam_s = 0, since = 0, prevw = 2u;     // prevw 2: no previous turn since the anchor
+                for (;;) {
+                    if (rounds > STREAK_CAP) { und = true; break; }
+                    if (!anch) {
+                        const u32 ta = (u32)s.a & M, tb = (u32)s.b & M;
+                        const u32 l1 = s.sa / (u32)W + 1u;
+                        const u32 ws = ta > tb ? l1 : n - l1, ls = n - ws;
+                        if ((ws & 1u) == 0u && ls >= 2u) {
+                            anch = true; since = 0; prevw = 2u;
+                            lam_s = lcm_u(n + 1u, lcm_u(ws, ls - 1u));
+                            t = s;
+                        }
+                    }
+                    const u32 w = turn<W, 0>(s, C, z) ? 1u : 0u;
+                    rounds++;
+                    if (ended<W>(s, C)) break;
+                    if (anch) {
+                        if (w == prevw) anch = false;
+                        else {
+                            prevw = w;
+                            if (++since == lam_s) {
+                                if (same<W>(s, t)) { cy = true; break; }
+                                anch = false;
+                            }
+                        }
+                    }
+                }
             }
         }
+        if (und) undet++;
         if (take) {
             done++;
             turns += rounds;
             if (cy) {
                 cyc++;
                 if (P.hb) atomicAdd(&lh[lam < P.hb - 1 ? lam : P.hb - 1], 1u);
-            } else if (rounds < AGE_CAP) {
+            } else if (!und) {
                 if (rounds > maxfin) maxfin = rounds;
                 if (P.lb) atomicAdd(&ll[rounds < P.lb - 1 ? (u32)rounds : P.lb - 1], 1u);
             }

@@ -14,6 +14,8 @@ HEADER = os.path.join(os.path.dirname(PKG_DIR), "include", "war_engine.h")
 MODE_FULL, MODE_HALF, MODE_CANON = 0, 1, 2
 KERNEL_PLAIN, KERNEL_PERSISTENT = 0, 1
 KERNELS = {"plain": KERNEL_PLAIN, "persistent": KERNEL_PERSISTENT}
+DET_BRENT, DET_STREAK = 0, 1
+DETECTORS = {"brent": DET_BRENT, "streak": DET_STREAK}
 
 
 class WarTuning(ctypes.Structure):
@@ -25,7 +27,7 @@ class WarRequest(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int32), ("mode", ctypes.c_int32), ("rule", ctypes.c_int32),
                 ("kernel", ctypes.c_int32), ("lo", ctypes.c_uint64), ("hi", ctypes.c_uint64),
                 ("hbins", ctypes.c_int32), ("lbins", ctypes.c_int32), ("dbins", ctypes.c_int32),
-                ("distinct", ctypes.c_int32), ("shape_blk", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("distinct", ctypes.c_int32), ("shape_blk", ctypes.c_int32), ("detector", ctypes.c_int32),
                 ("tuning", WarTuning)]
 
 

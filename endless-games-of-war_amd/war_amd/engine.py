@@ -9,8 +9,8 @@ import math
 import threading
 
 from . import _lib
-from ._lib import (KERNEL_PERSISTENT, KERNEL_PLAIN, KERNELS, MODE_CANON, MODE_FULL, MODE_HALF, WarDevinfo,
-                   WarRequest, WarResult, WarTuning, check)
+from ._lib import (DETECTORS, KERNEL_PERSISTENT, KERNEL_PLAIN, KERNELS, MODE_CANON, MODE_FULL, MODE_HALF,
+                   WarDevinfo, WarRequest, WarResult, WarTuning, check)
 
 MODES = {"full": MODE_FULL, "half": MODE_HALF, "canon": MODE_CANON}
 
@@ -44,12 +44,13 @@ def _tuning_dict(t):
 
 
 def make_request(n, mode="full", lo=0, hi=None, rule=0, kernel="persistent", hbins=0, lbins=0, dbins=0,
-                 distinct=False, shape_blk=0, tuning=None):
+                 distinct=False, shape_blk=0, tuning=None, detector="brent"):
     mode = MODES.get(mode, mode)
     if hi is None:
         hi = total_positions(n, mode)
     return WarRequest(int(n), int(mode), int(rule), int(KERNELS.get(kernel, kernel)), int(lo), int(hi),
-                      int(hbins), int(lbins), int(dbins), int(bool(distinct)), int(shape_blk), 0, _tuning(tuning))
+                      int(hbins), int(lbins), int(dbins), int(bool(distinct)), int(shape_blk),
+                      int(DETECTORS.get(detector, detector)), _tuning(tuning))
 
 
 class _ResultBuf:
@@ -71,7 +72,7 @@ class _ResultBuf:
     def as_dict(self):
         r, q = self.res, self.req
         return {
-            "n": q.n, "mode": q.mode, "rule": q.rule, "lo": q.lo, "hi": q.hi,
+            "n": q.n, "mode": q.mode, "rule": q.rule, "lo": q.lo, "hi": q.hi, "detector": q.detector,
             "cyc": r.cyc, "done": r.done, "maxfin": r.maxfin, "turns": r.turns,
             "oncycle": r.oncycle, "distinct_minkey": r.distinct_minkey, "distinct_hashset": r.distinct_hashset,
             "shape_nonstd": r.shape_nonstd, "shape_unchk": r.shape_unchk, "undetermined": r.undetermined,

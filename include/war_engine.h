@@ -41,6 +41,12 @@ extern "C" {
 #define WAR_KERNEL_PLAIN      0  /* one position per lane, per-lane Brent (reference structure) */
 #define WAR_KERNEL_PERSISTENT 1  /* persistent, age-synchronous stages with survivor compaction */
 
+/* detector (cycle test) */
+#define WAR_DET_BRENT  0   /* Brent (gpu/war_gpu.py:77-96): every output, any n and rule         */
+#define WAR_DET_STREAK 1   /* the odd-n streak test of gpu/war24.py:98-125 (Spivey Thm 6):
+                              odd n, rule 0, plain kernel; counts, game lengths and turns, but
+                              no cycle lengths (hbins, dbins, distinct, shape must be 0)        */
+
 typedef struct war_tuning {
     int32_t waves_per_cu;      /* persistent grid = CUs x waves_per_cu wave64s (1..32)          */
     int32_t ranks_per_refill;  /* tile: positions per grab / slots per queue region (64..8192)  */
@@ -61,7 +67,7 @@ typedef struct war_request {
     int32_t  dbins;        /* distinct weight bins (CANON, n <= 15; 0 = off, <= 4096)           */
     int32_t  distinct;     /* CANON, n <= 15: min-key count + device hash set of cycle keys    */
     int32_t  shape_blk;    /* CANON: Theorem-3 run length to test (0 = off)                      */
-    int32_t  reserved;
+    int32_t  detector;     /* WAR_DET_* (0 = Brent)                                              */
     war_tuning tuning;     /* zeros = engine defaults                                            */
 } war_request;
 
@@ -69,13 +75,15 @@ typedef struct war_result {
     uint64_t cyc;              /* endless deals / cycling positions                              */
     uint64_t done;             /* positions played (must equal the range length; HALF: taken)   */
     uint64_t maxfin;           /* longest ending game, in turns                                  */
-    uint64_t turns;            /* sum of Brent rounds (game length, or turns to the repeat)      */
+    uint64_t turns;            /* sum of the detector's rounds (game length, or turns until the
+                                  repeat is seen)                                                */
     uint64_t oncycle;          /* distinct/dbins: positions lying on their own cycle             */
     uint64_t distinct_minkey;  /* distinct: positions whose key is their cycle's minimum key    */
     uint64_t distinct_hashset; /* distinct: different cycle-minimum keys met (device hash set)  */
     uint64_t shape_nonstd;     /* shape: cycles not made of winner runs of exactly shape_blk    */
     uint64_t shape_unchk;      /* shape: cycles longer than 512 turns (not checked)             */
-    uint64_t undetermined;     /* games still running at the safety cap (2^24 turns); must be 0 */
+    uint64_t undetermined;     /* games still running at the safety cap (2^24 turns; the streak
+                                  test: 2^22, as gpu/war24.py:58); must be 0                     */
     uint64_t *hist;    int32_t hbins;   /* caller-owned, hbins entries (or NULL)               */
     uint64_t *lenhist; int32_t lbins;   /* caller-owned, lbins entries (or NULL)               */
     uint64_t *dhist;   int32_t dbins;   /* caller-owned, dbins entries (or NULL)               */

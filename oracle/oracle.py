@@ -24,7 +24,8 @@ class OrcRes(ctypes.Structure):
                 ("shape_unchk", ctypes.c_uint64),
                 ("hist", _U64P), ("hbins", ctypes.c_int),
                 ("lenhist", _U64P), ("lbins", ctypes.c_int),
-                ("dhist", _U64P), ("dbins", ctypes.c_int)]
+                ("dhist", _U64P), ("dbins", ctypes.c_int),
+                ("undet", ctypes.c_uint64)]
 
 
 _lib = None
@@ -43,6 +44,9 @@ def lib():
         L.orc_run.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64,
                               ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(OrcRes)]
         L.orc_run.restype = ctypes.c_int
+        L.orc_run_det.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64,
+                                  ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(OrcRes)]
+        L.orc_run_det.restype = ctypes.c_int
         L.orc_last_error.restype = ctypes.c_char_p
         L.orc_total.argtypes = [ctypes.c_int, ctypes.c_int]
         L.orc_total.restype = ctypes.c_uint64
@@ -59,8 +63,10 @@ def total(n, mode="full"):
     return int(lib().orc_total(int(n), _MODES.get(mode, mode)))
 
 
-def run(n, mode="full", lo=0, hi=None, rule=0, hbins=0, lbins=0, dbins=0, distinct=False, shape_blk=0, threads=0):
-    """Play every position of [lo, hi) on the CPU; returns a dict shaped like the engine's result."""
+def run(n, mode="full", lo=0, hi=None, rule=0, hbins=0, lbins=0, dbins=0, distinct=False, shape_blk=0, threads=0,
+        detector="brent"):
+    """Play every position of [lo, hi) on the CPU; returns a dict shaped like the engine's result.
+    detector: "brent" (every output) or "streak" (odd n, rule 0, no cycle-length outputs)."""
     L = lib()
     m = _MODES.get(mode, mode)
     if hi is None:
@@ -72,13 +78,14 @@ def run(n, mode="full", lo=0, hi=None, rule=0, hbins=0, lbins=0, dbins=0, distin
     r.hist = ctypes.cast(H, _U64P); r.hbins = hbins
     r.lenhist = ctypes.cast(Ln, _U64P); r.lbins = lbins
     r.dhist = ctypes.cast(D, _U64P); r.dbins = dbins
-    rc = L.orc_run(int(n), m, int(rule), int(lo), int(hi), int(threads), int(bool(distinct)), int(shape_blk),
-                   ctypes.byref(r))
+    det = {"brent": 0, "streak": 1}[detector]
+    rc = L.orc_run_det(int(n), m, int(rule), int(lo), int(hi), int(threads), int(bool(distinct)), int(shape_blk),
+                       det, ctypes.byref(r))
     if rc:
         raise ValueError("oracle: %s" % L.orc_last_error().decode())
     return {"cyc": r.cyc, "done": r.done, "maxfin": r.maxfin, "turns": r.turns, "oncycle": r.oncycle,
             "distinct_minkey": r.distinct_minkey, "distinct_keys": r.distinct_keys,
-            "shape_nonstd": r.shape_nonstd, "shape_unchk": r.shape_unchk,
+            "shape_nonstd": r.shape_nonstd, "shape_unchk": r.shape_unchk, "undetermined": r.undet,
             "hist": [int(H[i]) for i in range(hbins)], "lenhist": [int(Ln[i]) for i in range(lbins)],
             "dhist": [int(D[i]) for i in range(dbins)]}
 

@@ -16,6 +16,7 @@
 //   distinct by minimum key ..... cpu/distinct_cycles.c:24-31 (key), :57-80
 //   loop shape (Theorem-3 test) . gpu/war_gpu.py:51-73, :238-242
 //   5-bit piles (n >= 17) ....... gpu/war24.py:31-47 (only the layout; the game is the same)
+//   streak test (odd n) ......... gpu/war24.py:98-125
 //
 // Extensions the reference does not have (pinned by invariants in the tests, not by goldens):
 // arbitrary rank / position ranges, the histogram of ending-game lengths (lenhist), the total
@@ -54,6 +55,7 @@ struct orc_res {
     u64* hist;  int hbins;    // lambda histogram of endless positions (last bin = overflow)
     u64* lenhist; int lbins;  // histogram of ending-game lengths (last bin = overflow)
     u64* dhist; int dbins;    // distinct weights (canon + distinct)
+    u64 undet;                // streak test: games still undecided after 2^22 turns
 };
 }
 
@@ -107,6 +109,47 @@ template <int W> static inline bool brent(St<W> s, int rule, u64& rounds, u64& l
         lam++;
         if (same<W>(s, t)) { rounds = r; lam_out = lam; if (meet) *meet = s; return true; }
         if (lam == power) { t = s; power <<= 1; lam = 0; }
+    }
+}
+
+// Streak test for odd n (gpu/war24.py:98-125, Spivey Thm 6): anchor the state at a turn whose
+// winner-to-be holds an even number ws of cards while the other holds ls >= 2; if the players then
+// alternate for lcm(n+1, ws, ls-1) turns and the state equals the anchor, the game never ends.
+// A repeated winner drops the anchor. Returns 0 = ends, 1 = endless, 2 = undetermined (more than
+// 2^22 turns, as the reference's CAP); steps = turns played (the game length for a game that ends).
+static inline int gcd_i(int a, int b) { while (b) { int t = a % b; a = b; b = t; } return a; }
+static inline int lcm_i(int a, int b) { return a / gcd_i(a, b) * b; }
+template <int W> static inline int streak(St<W> s, int n, u64& steps) {
+    typedef typename Lay<W>::T T;
+    const T M = (T)((1u << W) - 1u);
+    const u64 CAP = 1ULL << 22;
+    steps = 0;
+    bool anch = false; int lam = 0, since = 0, prevw = -1;
+    St<W> a = s;
+    for (;;) {
+        if (s.la == 0 || s.lb == 0) return 0;
+        if (steps > CAP) return 2;
+        if (!anch) {
+            int ta = (int)(s.a & M), tb = (int)(s.b & M);
+            int ws = ta > tb ? s.la : s.lb, ls = n - ws;
+            if ((ws & 1) == 0 && ls >= 2) {
+                anch = true; since = 0; prevw = -1;
+                lam = lcm_i(n + 1, lcm_i(ws, ls - 1));
+                a = s;
+            }
+        }
+        int w = step<W>(s, 0);
+        steps++;
+        if (anch) {
+            if (w == prevw) anch = false;
+            else {
+                prevw = w;
+                if (++since == lam) {
+                    if (same<W>(s, a)) return 1;
+                    anch = false;
+                }
+            }
+        }
     }
 }
 
@@ -180,16 +223,28 @@ template <int W> static int loop_shape(St<W> s, u64 lam, int blk, int rule) {
 
 // ---------------------------------------------------------------------------------------------
 struct Acc {
-    u64 cyc = 0, done = 0, maxfin = 0, turns = 0, oncycle = 0, minkey = 0, nonstd = 0, unchk = 0;
+    u64 cyc = 0, done = 0, maxfin = 0, turns = 0, oncycle = 0, minkey = 0, nonstd = 0, unchk = 0, undet = 0;
     std::vector<u64> hist, lenhist, dhist;
     std::vector<u64> keys;   // cycle-minimum keys of on-cycle positions (for the distinct-key count)
     Acc(int hb, int lb, int db) : hist(hb > 0 ? hb : 0, 0), lenhist(lb > 0 ? lb : 0, 0), dhist(db > 0 ? db : 0, 0) {}
 };
 
-struct Cfg { int n, mode, rule, hbins, lbins, dbins, distinct, blk; };
+struct Cfg { int n, mode, rule, hbins, lbins, dbins, distinct, blk, det; };
 
 template <int W> static void play(const St<W>& s, const Cfg& c, Acc& A) {
     u64 rounds = 0, lam = 0; St<W> meet;
+    if (c.det == 1) {      // streak detector: counts, game lengths and turns; no cycle lengths
+        int r = streak<W>(s, c.n, rounds);
+        A.done++;
+        A.turns += rounds;
+        if (r == 1) A.cyc++;
+        else if (r == 2) A.undet++;
+        else {
+            if (rounds > A.maxfin) A.maxfin = rounds;
+            if (c.lbins) A.lenhist[rounds < (u64)(c.lbins - 1) ? rounds : (u64)(c.lbins - 1)]++;
+        }
+        return;
+    }
     bool cyc = brent<W>(s, c.rule, rounds, lam, &meet);
     A.done++;
     A.turns += rounds;
@@ -267,8 +322,10 @@ extern "C" const char* orc_last_error(void) { return g_err.c_str(); }
 extern "C" u64 orc_total(int n, int mode) { return range_total(n, mode); }
 
 // Play every position in [lo, hi) (deal ranks, or canon positions). 0 on success.
-extern "C" int orc_run(int n, int mode, int rule, u64 lo, u64 hi, int threads,
-                       int distinct, int shape_blk, orc_res* res) {
+// det: 0 = Brent (every output), 1 = the streak test (odd n, rule 0; no cycle lengths, so no
+// hist / dhist / distinct / shape).
+extern "C" int orc_run_det(int n, int mode, int rule, u64 lo, u64 hi, int threads,
+                           int distinct, int shape_blk, int det, orc_res* res) {
     if (!res) { g_err = "null result"; return 1; }
     if (n < 1 || n > 20) { g_err = "n must be in [1, 20]"; return 2; }
     if (mode < 0 || mode > 2) { g_err = "bad mode"; return 3; }
@@ -278,8 +335,13 @@ extern "C" int orc_run(int n, int mode, int rule, u64 lo, u64 hi, int threads,
     if (mode == ORC_HALF && (n & 1)) { g_err = "half mode needs even n"; return 6; }
     if ((distinct || res->dbins) && (mode != ORC_CANON || n > 15)) { g_err = "distinct needs canon mode and n <= 15"; return 7; }
     if (shape_blk && mode != ORC_CANON) { g_err = "shape needs canon mode"; return 8; }
+    if (det != 0 && det != 1) { g_err = "det must be 0 (Brent) or 1 (streak)"; return 9; }
+    if (det == 1 && ((n % 2) == 0 || rule != 0 || distinct || shape_blk || (res->hist && res->hbins) ||
+                     (res->dhist && res->dbins))) {
+        g_err = "the streak test needs odd n, rule 0 and no cycle-length outputs"; return 10;
+    }
     Cfg c{n, mode, rule, res->hist ? res->hbins : 0, res->lenhist ? res->lbins : 0,
-          res->dhist ? res->dbins : 0, distinct, shape_blk};
+          res->dhist ? res->dbins : 0, distinct, shape_blk, det};
     if (threads > 0) omp_set_num_threads(threads);
     const u64 CH = (mode == ORC_CANON) ? (u64)std::max(1, n - 1) * 512 : 4096;
     u64 nchunks = (hi - lo + CH - 1) / CH;   // 0 for an empty range
@@ -295,7 +357,7 @@ extern "C" int orc_run(int n, int mode, int rule, u64 lo, u64 hi, int threads,
         #pragma omp critical
         {
             G.cyc += A.cyc; G.done += A.done; G.turns += A.turns; G.oncycle += A.oncycle;
-            G.minkey += A.minkey; G.nonstd += A.nonstd; G.unchk += A.unchk;
+            G.minkey += A.minkey; G.nonstd += A.nonstd; G.unchk += A.unchk; G.undet += A.undet;
             G.maxfin = std::max(G.maxfin, A.maxfin);
             for (size_t i = 0; i < G.hist.size(); i++) G.hist[i] += A.hist[i];
             for (size_t i = 0; i < G.lenhist.size(); i++) G.lenhist[i] += A.lenhist[i];
@@ -307,11 +369,16 @@ extern "C" int orc_run(int n, int mode, int rule, u64 lo, u64 hi, int threads,
     u64 nk = (u64)(std::unique(G.keys.begin(), G.keys.end()) - G.keys.begin());
     res->cyc = G.cyc; res->done = G.done; res->maxfin = G.maxfin; res->turns = G.turns;
     res->oncycle = G.oncycle; res->distinct_minkey = G.minkey; res->distinct_keys = nk;
-    res->shape_nonstd = G.nonstd; res->shape_unchk = G.unchk;
+    res->shape_nonstd = G.nonstd; res->shape_unchk = G.unchk; res->undet = G.undet;
     for (int i = 0; i < c.hbins; i++) res->hist[i] = G.hist[(size_t)i];
     for (int i = 0; i < c.lbins; i++) res->lenhist[i] = G.lenhist[(size_t)i];
     for (int i = 0; i < c.dbins; i++) res->dhist[i] = G.dhist[(size_t)i];
     return 0;
+}
+
+extern "C" int orc_run(int n, int mode, int rule, u64 lo, u64 hi, int threads,
+                       int distinct, int shape_blk, orc_res* res) {
+    return orc_run_det(n, mode, rule, lo, hi, threads, distinct, shape_blk, 0, res);
 }
 
 // One position given as two piles (top card first). Returns 1 if endless, 0 if it ends.
@@ -342,11 +409,13 @@ extern "C" void orc_unrank(u64 r, int m, int* p) { unrank(r, m, p); }
 #ifdef ORC_MAIN
 static void usage() {
     fprintf(stderr, "usage: war_oracle n [--mode full|half|canon] [--rule 0|1] [--lo L] [--hi H]\n"
-                    "                  [--hist B] [--lbins B] [--dbins B] [--distinct] [--shape] [--threads T]\n");
+                    "                  [--hist B] [--lbins B] [--dbins B] [--distinct] [--shape] [--threads T]\n"
+                    "                  [--det brent|streak]\n");
 }
 int main(int argc, char** argv) {
     if (argc < 2) { usage(); return 2; }
     int n = atoi(argv[1]), mode = ORC_FULL, rule = 0, threads = 0, hb = 0, lb = 0, db = 0, distinct = 0, shape = 0;
+    int det = 0;
     long long lo = 0, hi = -1;
     for (int i = 2; i < argc; i++) {
         std::string a = argv[i];
@@ -361,6 +430,7 @@ int main(int argc, char** argv) {
         else if (a == "--distinct") distinct = 1;
         else if (a == "--shape") shape = 1;
         else if (a == "--threads") threads = atoi(nxt().c_str());
+        else if (a == "--det") det = nxt() == "streak" ? 1 : 0;
         else { usage(); return 2; }
     }
     u64 total = range_total(n, mode);
@@ -369,12 +439,12 @@ int main(int argc, char** argv) {
     orc_res r{}; r.hist = H.data(); r.hbins = hb; r.lenhist = L.data(); r.lbins = lb; r.dhist = D.data(); r.dbins = db;
     int blk = shape ? (n & -n) : 0;
     double t0 = omp_get_wtime();
-    int rc = orc_run(n, mode, rule, (u64)lo, (u64)hi, threads, distinct, blk, &r);
+    int rc = orc_run_det(n, mode, rule, (u64)lo, (u64)hi, threads, distinct, blk, det, &r);
     double el = omp_get_wtime() - t0;
     if (rc) { fprintf(stderr, "error: %s\n", orc_last_error()); return 1; }
     printf("n=%d mode=%d rule=%d range=[%lld,%lld) done=%llu cycling=%llu longest_finite=%llu turns=%llu "
-           "threads=%d time=%.3fs\n", n, mode, rule, lo, hi, r.done, r.cyc, r.maxfin, r.turns,
-           threads > 0 ? threads : omp_get_max_threads(), el);
+           "undetermined=%llu threads=%d time=%.3fs\n", n, mode, rule, lo, hi, r.done, r.cyc, r.maxfin, r.turns,
+           r.undet, threads > 0 ? threads : omp_get_max_threads(), el);
     if (distinct || db)
         printf("   oncycle=%llu distinct_minkey=%llu distinct_keys=%llu\n", r.oncycle, r.distinct_minkey, r.distinct_keys);
     if (shape) printf("   shape blk=%d nonstd=%llu unchecked=%llu\n", blk, r.shape_nonstd, r.shape_unchk);

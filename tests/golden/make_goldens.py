@@ -180,6 +180,12 @@ def main():
     h14 = block(lr, "war_gpu.py 14 --half --hist 256", r"^n=15")
     g["cli"].append({"args": ["14", "--half", "--hist", "256"], "lines": h14[:2], "source": "data/long_runs.txt:13-15"})
 
+    # war24.py result lines (data/long_runs.txt): the command line is rebuilt from the line itself
+    g["cli24"] = []
+    for i, ln in enumerate(lr):
+        if re.match(r"n=\d+ det=(floyd|streak) range=\[\d+,\d+\) deals_simulated=", ln):
+            g["cli24"].append({"line": ln, "source": "data/long_runs.txt:%d" % (i + 1)})
+
     with open(OUT, "w") as f:
         json.dump(g, f, indent=1, sort_keys=True)
     print("wrote", OUT)

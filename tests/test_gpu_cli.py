@@ -35,3 +35,24 @@ def test_n17_published_ranges(eng, goldens, key):
     r = eng.run_range(17, "full", g["lo"], g["hi"])
     assert r["done"] == g["hi"] - g["lo"]
     assert (r["cyc"], r["maxfin"]) == (g["cycling"], g["longest"])
+
+
+LINE24 = re.compile(r"n=(\d+) det=(floyd|streak) range=\[(\d+),(\d+)\) ")
+TIME24 = re.compile(r"time=[\d.]+s$")
+
+
+@pytest.mark.parametrize("case", [0, 1, 2, 4, 5])
+def test_war24_cli_matches_reference_log(eng, goldens, capsys, monkeypatch, tmp_path, case):
+    """The war24.py drop-in prints the reference's result line for the same range and detector
+    (data/long_runs.txt:62-64,74-75; the whole n = 17 run of :70 took the reference 6 days and
+    is not repeated here). Only the elapsed time differs."""
+    import war24
+    monkeypatch.setattr(war24, "_HERE", str(tmp_path))
+    c = goldens["cli24"][case]
+    n, det, lo, hi = LINE24.match(c["line"]).groups()
+    args = [n, "--det", det, "--start", lo, "--count", str(int(hi) - int(lo))]
+    print("case war24", args, flush=True)
+    capsys.readouterr()
+    war24.main(args)
+    got = capsys.readouterr().out.strip().splitlines()[-1]
+    assert TIME24.sub("time=<t>s", got) == TIME24.sub("time=<t>s", c["line"]), (c["source"], got)

@@ -131,3 +131,45 @@ def test_n14_canon_goldens(eng, goldens):
     assert r["distinct_minkey"] == r["distinct_hashset"] == g["distinct"]
     assert r["oncycle"] == goldens["canon"]["14"]["on_cycle"]
     print("n14 canon persistent device_ms", r["device_ms"], flush=True)
+
+
+# ---- the streak test (gpu/war24.py:98-125): plain kernel, odd n, rule 0
+STREAK_FIELDS = ("cyc", "done", "maxfin", "turns", "lenhist", "undetermined")
+
+
+def check_streak(eng, orc, n, mode, lo, hi):
+    want = orc.run(n, mode, lo, hi, lbins=512, detector="streak")
+    got = eng.run_range(n, mode, lo, hi, kernel="plain", detector="streak", lbins=512)
+    for k in STREAK_FIELDS:
+        assert got[k] == want[k], (n, mode, lo, hi, k, got[k], want[k])
+    return got
+
+
+@pytest.mark.parametrize("n,mode", [(3, "full"), (5, "full"), (7, "full"), (9, "full"), (11, "full"),
+                                    (5, "canon"), (7, "canon"), (9, "canon"), (11, "canon")])
+def test_streak_whole_range(eng, orc, n, mode):
+    print("case streak", n, mode, flush=True)
+    got = check_streak(eng, orc, n, mode, 0, orc.total(n, mode))
+    brent = eng.run_range(n, mode, kernel="persistent")
+    assert (got["cyc"], got["maxfin"]) == (brent["cyc"], brent["maxfin"])
+
+
+def test_streak_sampled_ranges(eng, orc):
+    rng = random.Random(24)
+    for n in (13, 15, 17, 19):
+        total = orc.total(n, "full")
+        for ln in (1, 65, 100_003):
+            lo = rng.randrange(0, total - ln)
+            print("case streak", n, lo, ln, flush=True)
+            check_streak(eng, orc, n, "full", lo, lo + ln)
+    check_streak(eng, orc, 13, "canon", 10**9, 10**9 + 200_000)
+
+
+def test_streak_refusals(eng):
+    for bad in [dict(n=14), dict(n=13, rule=1), dict(n=13, kernel="persistent"), dict(n=13, hbins=16),
+                dict(n=13, mode="canon", dbins=16), dict(n=13, mode="canon", shape_blk=1)]:
+        kw = dict(bad)
+        n = kw.pop("n")
+        kw.setdefault("kernel", "plain")
+        with pytest.raises(WarError, match="streak"):
+            eng.run_range(n, lo=0, hi=100, detector="streak", **kw)
