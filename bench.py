@@ -8,6 +8,8 @@ One step = one whole enumeration. Beside it, on the same GPU in the same run:
   canon14        the n = 14 canonical scan (13! * 13 positions), persistent kernel, against the
                  published counts, and both kernels on the same sampled chunks (the batched/plain
                  ratio);
+  n17_range      the n = 17 rank range [0, 8992587776) the reference logged, in the held-card
+                 layout (4-bit piles, card 16 apart) and in 5-bit piles;
   serving        a continuous-batching run of mixed queries with p50/p99 latency, beside the same
                  queries one by one through the plain kernel;
   roofline       integer VALU: useful lane-ops/s (Brent turns x VALU ops per turn of the
@@ -198,6 +200,20 @@ def main():
                       "persistent_over_plain": tp / tq, "tuning": eng.tuning_for(14, "canon")})
         out["canon14"] = canon
         log("canon14: %.3e pos/s persistent (whole), ratio %.2f on samples" % (tot14 / cw, tp / tq))
+
+        # ---- n = 17, the published rank range [0, 8992587776): held-card layout against 5-bit piles
+        lo17, hi17 = 0, 8992587776
+        n17 = {"workload": "n17_range: deal ranks [0, 8992587776) of 17!", "unit": "deals/s",
+               "parity": "cycling 8016845372, longest 94 (the reference's logged range)",
+               "reference_s": 13.2}
+        for lay in (6, 5):
+            r = eng.run_range(17, "full", lo17, hi17, tuning={"pile_layout": lay})
+            assert (r["cyc"], r["maxfin"], r["done"]) == (8016845372, 94, hi17 - lo17), r
+            n17["layout%d_value" % lay] = (hi17 - lo17) / (r["device_ms"] / 1e3)
+            n17["layout%d_device_ms" % lay] = r["device_ms"]
+        n17["held_over_5bit"] = n17["layout6_value"] / n17["layout5_value"]
+        out["n17_range"] = n17
+        log("n17 range: held layout %.3e deals/s, 5-bit %.3e" % (n17["layout6_value"], n17["layout5_value"]))
 
         # ---- serving: continuous batching of mixed queries
         out["serving"] = serving(eng)

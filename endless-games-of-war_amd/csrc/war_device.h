@@ -27,18 +27,37 @@ __constant__ u64 c_fact[21] = {
     20922789888000ull, 355687428096000ull, 6402373705728000ull, 121645100408832000ull,
     2432902008176640000ull};
 
+// Pile layouts, by id W:
+//   4: 4-bit cards, one u64 per pile (n <= 16);
+//   5: 5-bit cards, one u128 per pile (n <= 20);
+//   6: n = 17 with the top card held apart (the layout of gpu/war17.py:7-16,33-72): card n-1 never
+//      loses a turn, so it never changes hands and its holder never runs out. Both players' other
+//      cards (0..15) are 4-bit in one u64 each, and the holder's pile is its word with card n-1
+//      at depth p. `a` is the holder's word, `b` the other player's; the game is the same, with the
+//      players named by who holds card n-1 instead of by deal order (the counts do not depend on
+//      the names).
+// T: pile word; P / PW: permutation word and its bits per card; CB: bits per card in a pile.
 template <int W> struct Lay;
-template <> struct Lay<4> { typedef u64 T; };
-template <> struct Lay<5> { typedef u128 T; };
+template <> struct Lay<4> { typedef u64 T; typedef u64 P; static constexpr int PW = 4, CB = 4; };
+template <> struct Lay<5> { typedef u128 T; typedef u128 P; static constexpr int PW = 5, CB = 5; };
+template <> struct Lay<6> { typedef u64 T; typedef u128 P; static constexpr int PW = 5, CB = 4; };
+constexpr u32 HELD_N = 17;     // the one n of layout 6
 
-// sa = W * (player 1's cards - 1): the bit index of player 1's bottom card. Player 1 wins a turn:
-// the two cards go to bit sa (after the pop), then sa += W. Player 2 wins: they go to bit
-// C - sa with C = B - 2W, then sa -= W. The game is over iff sa > C as an unsigned number
-// (player 1 empty: sa = -W; player 2 empty: sa = B - W).
+// sa = CB * (player 1's cards - 1): the bit index of player 1's bottom card. Player 1 wins a turn:
+// the two cards go to bit sa (after the pop), then sa += CB. Player 2 wins: they go to bit
+// C - sa with C = CB * (n - 2), then sa -= CB. The game is over iff sa > C as an unsigned number
+// (player 1 empty: sa = -CB; player 2 empty: sa = C + CB). Layout 6: "player 1" is the holder of
+// card n-1 and sa counts only its word's cards (sa = 4 * cards in `a`, the holder's full pile
+// minus one), so the same formulas hold; pb = 4 * the depth of card n-1 in the holder's pile.
 template <int W> struct St {
     typename Lay<W>::T a, b;   // player 1, player 2
     u32 sa;
 };
+template <> struct St<6> {
+    u64 a, b;                  // the holder's other cards, the other player's cards
+    u32 sa, pb;
+};
+template <int W> __host__ __device__ constexpr u32 c_of(u32 n) { return (u32)Lay<W>::CB * (n - 2u); }
 
 // x >> s and x << s that give 0 for s >= bit width (the hardware masks the shift count).
 template <class T> __device__ __forceinline__ T shr_(T x, u32 s) { return s >= 8 * sizeof(T) ? (T)0 : x >> s; }
@@ -55,11 +74,12 @@ template <int W> __device__ __forceinline__ u32 bits1(const St<W> &s) { return s
 // constant the hardware's own count masking does not cover, and emits an AND per shift.
 template <int W> struct Zh {};
 template <> struct Zh<4> { u64 h1, h2; };
+template <> struct Zh<6> { u64 h1, h2; };
 template <int W> __device__ __forceinline__ void open_sa(St<W> &s, u32 zero) { s.sa |= zero & 3u; }
 
 template <int W> __device__ __forceinline__ Zh<W> make_zh(u32 zero) {
     Zh<W> z;
-    if constexpr (W == 4) { z.h1 = (u64)zero << 32; z.h2 = z.h1; }
+    if constexpr (W == 4 || W == 6) { z.h1 = (u64)zero << 32; z.h2 = z.h1; }
     (void)zero;
     return z;
 }
@@ -67,8 +87,12 @@ template <int W> __device__ __forceinline__ Zh<W> make_zh(u32 zero) {
 // One turn (gpu/war_gpu.py:38-45). Rule 0: the winner puts the winning card, then the losing
 // card, under its pile; rule 1 reverses the two. C = W * (n - 2). Returns true if player 1 won.
 // Branch-free: each pile ORs in its own two-card value (zero for the loser) at its own offset.
+template <int RULE>
+__device__ __forceinline__ bool turn_held(St<6> &s, u32 C, Zh<6> &z);
+
 template <int W, int RULE>
 __device__ __forceinline__ bool turn(St<W> &s, u32 C, Zh<W> &z) {
+    if constexpr (W == 6) { return turn_held<RULE>(s, C, z); } else {
     typedef typename Lay<W>::T T;
     constexpr u32 M = (1u << W) - 1u;
     const u32 c1 = (u32)s.a & M, c2 = (u32)s.b & M;
@@ -92,13 +116,48 @@ __device__ __forceinline__ bool turn(St<W> &s, u32 C, Zh<W> &z) {
     }
     s.sa += p1 ? (u32)W : (u32)(-W);
     return p1;
+    }
+}
+
+// Layout 6 (held top card). The holder plays card n-1 when it is on top (pb == 0): it wins, the
+// other player's card goes under its word, and card n-1 goes under its pile (gpu/war17.py:36-41).
+// Otherwise both play their words' top cards and the holder's card n-1 moves up one. The holder
+// never runs out, so the game is over iff the other player is out: sa > C. Returns true if the
+// holder won.
+template <int RULE>
+__device__ __forceinline__ bool turn_held(St<6> &s, u32 C, Zh<6> &z) {
+    const u32 ch = (u32)s.a & 15u, co = (u32)s.b & 15u;
+    const bool top = s.pb == 0u;
+    const bool hw = top | (ch > co);
+    const u32 sh = top ? 0u : 4u;
+    s.a >>= sh;
+    s.b >>= 4;
+    const u32 hoff = s.sa - sh;                                     // the holder's word, after the pop
+    const u32 vp = RULE ? (co | (ch << 4)) : (ch | (co << 4));      // holder won with a word card
+    const u32 vh = top ? co : vp;                                   // with card n-1: only the other card
+    const u32 vo = RULE ? (ch | (co << 4)) : (co | (ch << 4));      // the other player won
+    const u32 xh = hw ? vh : 0u, xo = hw ? 0u : vo;
+    z.h1 = (z.h1 & 0xFFFFFFFF00000000ull) | xh;
+    z.h2 = (z.h2 & 0xFFFFFFFF00000000ull) | xo;
+    s.a |= z.h1 << (hoff & 63u);
+    s.b |= z.h2 << ((C - s.sa) & 63u);
+    // card n-1 went under: rule 0 puts it before the captured card (depth = cards in the word
+    // before the capture), rule 1 after it
+    s.pb = top ? (RULE ? s.sa + 4u : s.sa) : s.pb - 4u;
+    s.sa += hw ? 4u : (u32)-4;
+    return hw;
 }
 
 template <int W> __device__ __forceinline__ bool ended(const St<W> &s, u32 C) { return s.sa > C; }
 
 // The repeat test compares (w1, w2, l1): card 0 and an empty slot look alike (gpu/war_gpu.py:91).
+// Layout 6 adds the depth of card n-1.
+template <int W> __device__ __forceinline__ bool rest_same(const St<W> &x, const St<W> &y) {
+    if constexpr (W == 6) return (x.b == y.b) & (x.sa == y.sa) & (x.pb == y.pb);
+    else return (x.b == y.b) & (x.sa == y.sa);
+}
 template <int W> __device__ __forceinline__ bool same(const St<W> &x, const St<W> &y) {
-    return (x.a == y.a) & (x.b == y.b) & (x.sa == y.sa);
+    return (x.a == y.a) & rest_same<W>(x, y);
 }
 
 template <int W> __device__ __forceinline__ u32 card(typename Lay<W>::T q, int i) {
@@ -177,11 +236,47 @@ __device__ __forceinline__ u64 even_fields4(u64 x) {
     return x;
 }
 
+// Layout 6 from a 5-bit order of the 17 cards: each player's cards in deal order, card n-1 left out
+// and remembered as (holder, depth) (gpu/war17.py:50-58).
+__device__ __forceinline__ St<6> deal_held(u128 q) {
+    u64 w0 = 0, w1 = 0;
+    u32 l0 = 0, l1 = 0, holder = 0, p = 0;
+#pragma unroll
+    for (int i = 0; i < (int)HELD_N; i++) {
+        const u32 c = (u32)(q >> (5 * i)) & 31u;
+        const bool top = c == HELD_N - 1u;
+        if ((i & 1) == 0) {
+            holder = top ? 0u : holder; p = top ? l0 : p;
+            w0 |= top ? 0ull : (u64)c << (4u * l0); l0 += top ? 0u : 1u;
+        } else {
+            holder = top ? 1u : holder; p = top ? l1 : p;
+            w1 |= top ? 0ull : (u64)c << (4u * l1); l1 += top ? 0u : 1u;
+        }
+    }
+    St<6> s;
+    s.a = holder ? w1 : w0;
+    s.b = holder ? w0 : w1;
+    s.sa = 4u * (holder ? l1 : l0);
+    s.pb = 4u * p;
+    return s;
+}
+
+// 5-bit packed cards 0..15 (16 fields) to 4-bit packed.
+__device__ __forceinline__ u64 fields5to4(u128 q) {
+    u64 r = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) r |= (u64)((u32)(q >> (5 * i)) & 15u) << (4 * i);
+    return r;
+}
+
 template <int W>
-__device__ __forceinline__ St<W> deal(typename Lay<W>::T q, int n) {
+__device__ __forceinline__ St<W> deal(typename Lay<W>::P q, int n) {
     typedef typename Lay<W>::T T;
     St<W> s;
-    if constexpr (W == 4) {
+    if constexpr (W == 6) {
+        (void)n;
+        return deal_held(q);
+    } else if constexpr (W == 4) {
         s.a = even_fields4(q);
         s.b = even_fields4(q >> 4);
     } else {
@@ -197,14 +292,25 @@ __device__ __forceinline__ St<W> deal(typename Lay<W>::T q, int n) {
 }
 
 // Canonical position (gpu/war_gpu.py:210-213): player 1 = [n-1, q0..q(k-1)], player 2 = [qk..q(n-2)].
+// Layout 6: player 1 holds card n-1 on top (depth 0), its word is q0..q(k-1).
 template <int W>
-__device__ __forceinline__ St<W> canon(typename Lay<W>::T q, int n, int k) {
+__device__ __forceinline__ St<W> canon(typename Lay<W>::P q, int n, int k) {
     typedef typename Lay<W>::T T;
     St<W> s;
-    const u32 sk = W * (u32)k;
-    s.a = (T)(u32)(n - 1) | ((q & lowmask_<T>(sk)) << W);
-    s.b = shr_<T>(q, sk);
-    s.sa = W * (u32)k;
+    if constexpr (W == 6) {
+        (void)n;
+        const u64 q4 = fields5to4(q);
+        const u32 sk = 4u * (u32)k;                 // k <= 15
+        s.a = q4 & lowmask_<u64>(sk);
+        s.b = q4 >> sk;
+        s.sa = sk;
+        s.pb = 0;
+    } else {
+        const u32 sk = W * (u32)k;
+        s.a = (T)(u32)(n - 1) | ((q & lowmask_<T>(sk)) << W);
+        s.b = shr_<T>(q, sk);
+        s.sa = W * (u32)k;
+    }
     return s;
 }
 

@@ -83,6 +83,7 @@ struct ReqHost {
     int status = 0;
     std::string err;
     size_t bytes = 0;
+    int layout = 4;                 // pile layout the request runs in (4, 5 or 6)
 };
 
 // Device buffers of one pile layout: two region buffers (ping-pong between stages) of `cap` slots
@@ -172,10 +173,17 @@ int check_request(const war_request *r) {
     return 0;
 }
 
+// Layout 6 applies to n = 17 persistent Brent requests without cycle walks (its records do not
+// keep the players' deal order).
+bool held_ok(const war_request &r) {
+    return r.n == (int)HELD_N && r.kernel == WAR_KERNEL_PERSISTENT && r.detector == WAR_DET_BRENT &&
+           !r.dbins && !r.distinct && !r.shape_blk;
+}
+
 // Defaults measured on MI355X: deals (long games, ~110 turns at n = 13) want a late first stage
 // boundary; canonical positions (most end within ~30 turns) an early one and larger tiles.
 war_tuning default_tuning(int n, int mode) {
-    war_tuning t{28, 1024, 15, 32, 2};
+    war_tuning t{28, 1024, 15, 32, 2, 0};
     if (mode != WAR_MODE_CANON && n >= 12) t.brent_cap = 63;
     if (mode == WAR_MODE_CANON) t.ranks_per_refill = 4096;
     return t;
@@ -186,13 +194,17 @@ war_tuning default_tuning(int n, int mode) {
 void resolve_tuning(const Engine &E, const war_request &r, war_tuning &t) {
     t = default_tuning(r.n, r.mode);
     auto it = E.tuned.find(std::make_tuple(r.n, r.mode, r.rule));
-    if (it != E.tuned.end()) t = it->second;
+    if (it != E.tuned.end()) {
+        t = it->second;
+        if (t.pile_layout == 6 && !held_ok(r)) t.pile_layout = 0;   // tuned on requests without walks
+    }
     for (const war_tuning *src : {&E.tun, &r.tuning}) {
         if (src->waves_per_cu > 0) t.waves_per_cu = src->waves_per_cu;
         if (src->ranks_per_refill > 0) t.ranks_per_refill = src->ranks_per_refill;
         if (src->brent_cap > 0) t.brent_cap = src->brent_cap;
         if (src->plain_per > 0) t.plain_per = src->plain_per;
         if (src->stage_kind > 0) t.stage_kind = src->stage_kind;
+        if (src->pile_layout > 0) t.pile_layout = src->pile_layout;
     }
 }
 
@@ -204,6 +216,22 @@ int check_tuning(const war_tuning &t) {
     if (c < 1 || c > 65535 || ((c + 1) & c)) return fail(11, "brent_cap must be 2^k - 1 in [1, 65535]");
     if (t.plain_per < 1 || t.plain_per > 4096) return fail(11, "plain_per must be in [1, 4096]");
     if (t.stage_kind != 1 && t.stage_kind != 2) return fail(11, "stage_kind must be 1 (refill) or 2 (lockstep)");
+    if (t.pile_layout != 0 && t.pile_layout != 4 && t.pile_layout != 5 && t.pile_layout != 6)
+        return fail(11, "pile_layout must be 0, 4, 5 or 6");
+    return 0;
+}
+
+// The pile layout a request runs in. Layout 6 (n = 17, card 16 held apart, 4-bit piles) plays a
+// turn in about half the instructions of the 5-bit layout; it runs in the persistent pipeline
+// with Brent and without cycle walks (its records do not keep the players' deal order).
+int layout_of(const war_request &r, const war_tuning &t, int *out) {
+    int L = t.pile_layout;
+    if (L == 0) L = held_ok(r) ? 6 : (r.n <= 16 ? 4 : 5);
+    if (L == 4 && r.n > 16) return fail(11, "pile_layout 4 needs n <= 16");
+    if (L == 6 && !held_ok(r))
+        return fail(11, "pile_layout 6 needs n = 17, the persistent kernel, Brent and no dbins / distinct / shape");
+    if (L == 5 && (r.distinct || r.dbins)) return fail(11, "distinct / dbins need 4-bit piles (n <= 15)");
+    *out = L;
     return 0;
 }
 
@@ -269,7 +297,8 @@ void req_free(Engine &E, ReqHost &h) {
 }
 
 // ---- queue buffers of one layout, allocated on first use
-template <int W> QueueBufs &qbufs(Engine &E) { return W == 4 ? E.qb4 : E.qb5; }
+// layout 6 records are 16-byte u64 pairs like layout 4's, so the two share buffers
+template <int W> QueueBufs &qbufs(Engine &E) { return W == 5 ? E.qb5 : E.qb4; }
 
 template <int W> int ensure_queues(Engine &E, bool walk) {
     typedef typename Lay<W>::T T;
@@ -325,6 +354,11 @@ std::vector<u32> stage_caps(int cap0) {
 
 template <int W, int RULE, bool FRESH>
 void launch_stage_f(int flags, int blocks, size_t lds, hipStream_t st, const StageArgs<W> &A) {
+    if constexpr (W == 6) {            // no cycle walks in layout 6
+        if (flags & F_MULTI) hipLaunchKernelGGL((k_stage<W, RULE, FRESH, F_MULTI>), dim3(blocks), dim3(256), lds, st, A);
+        else hipLaunchKernelGGL((k_stage<W, RULE, FRESH, 0>), dim3(blocks), dim3(256), lds, st, A);
+        return;
+    }
     switch (flags) {
         case 0: hipLaunchKernelGGL((k_stage<W, RULE, FRESH, 0>), dim3(blocks), dim3(256), lds, st, A); break;
         case F_MULTI: hipLaunchKernelGGL((k_stage<W, RULE, FRESH, F_MULTI>), dim3(blocks), dim3(256), lds, st, A); break;
@@ -341,6 +375,10 @@ void launch_stage(bool fresh, int flags, int blocks, size_t lds, hipStream_t st,
 
 template <int W, int RULE>
 void launch_refill(int flags, int blocks, size_t lds, hipStream_t st, const StageArgs<W> &A) {
+    if constexpr (W == 6) {            // layout 6 runs lockstep stages only
+        launch_stage<W, RULE>(false, flags, blocks, lds, st, A);
+        return;
+    }
     switch (flags) {
         case 0: hipLaunchKernelGGL((k_refill<W, RULE, 0>), dim3(blocks), dim3(256), lds, st, A); break;
         case F_MULTI: hipLaunchKernelGGL((k_refill<W, RULE, F_MULTI>), dim3(blocks), dim3(256), lds, st, A); break;
@@ -376,16 +414,18 @@ int build_graph(Engine &E, const GraphKey &key, hipGraphExec_t *out) {
         const int flags = (key.multi ? F_MULTI : 0) | (key.walk ? F_WALK : 0);
         // refill stages need a bounded window (no tortoise move inside); the unbounded last stage
         // and the first (enumerating) stage are lockstep
-        const bool refill = key.kind == 1 && s > 0 && caps[s] != UNBOUNDED;
-        A.in_refill = (key.kind == 1 && s > 1) ? 1u : 0u;
+        const bool refill = W != 6 && key.kind == 1 && s > 0 && caps[s] != UNBOUNDED;
+        A.in_refill = (W != 6 && key.kind == 1 && s > 1) ? 1u : 0u;
         if (refill) launch_refill<W, RULE>(flags, blocks, lds, E.st, A);
         else launch_stage<W, RULE>(s == 0, flags, blocks, lds, E.st, A);
         a0 = caps[s];
     }
-    if (key.walk) {
-        WalkArgs<W> WA{};
-        WA.pipe = E.pipe; WA.hdr = E.hdr; WA.reqs = E.reqs; WA.cq = cyc_queue<W>(E); WA.tile_pos = 64u * (u32)key.per;
-        hipLaunchKernelGGL((k_walk<W, RULE>), dim3(blocks), dim3(256), 0, E.st, WA);
+    if constexpr (W != 6) {
+        if (key.walk) {
+            WalkArgs<W> WA{};
+            WA.pipe = E.pipe; WA.hdr = E.hdr; WA.reqs = E.reqs; WA.cq = cyc_queue<W>(E); WA.tile_pos = 64u * (u32)key.per;
+            hipLaunchKernelGGL((k_walk<W, RULE>), dim3(blocks), dim3(256), 0, E.st, WA);
+        }
     }
     hipGraph_t g = nullptr;
     hipError_t ce = hipStreamEndCapture(E.st, &g);
@@ -501,11 +541,14 @@ int run_request_locked(Engine &E, ReqHost &h) {
     resolve_tuning(E, r, t);
     int rc = check_tuning(t);
     if (rc) return rc;
+    int W = 4;
+    rc = layout_of(r, t, &W);
+    if (rc) return rc;
+    h.layout = W;
     h.slot = alloc_slot(E);
     if (h.slot < 0) return fail(40, "no free request slot");
     rc = req_alloc(E, h);
     if (rc) { req_free(E, h); return rc; }
-    const int W = r.n <= 16 ? 4 : 5;
     double dev = 0;
     if (r.kernel == WAR_KERNEL_PLAIN) {
         const u64 chunk = 1ull << 26;   // contiguous chunks of 2^26 positions per launch
@@ -522,8 +565,9 @@ int run_request_locked(Engine &E, ReqHost &h) {
             std::vector<Job> jobs(1);
             jobs[0].lo = lo; jobs[0].hi = hi; jobs[0].req = (u32)h.slot;
             float ms = 0;
-            if (W == 4) rc = run_batch_dispatch<4>(E, r.rule, jobs, t, r.hbins, r.lbins, h.dev.walk != 0, false, &ms);
-            else        rc = run_batch_dispatch<5>(E, r.rule, jobs, t, r.hbins, r.lbins, h.dev.walk != 0, false, &ms);
+            if (W == 4)      rc = run_batch_dispatch<4>(E, r.rule, jobs, t, r.hbins, r.lbins, h.dev.walk != 0, false, &ms);
+            else if (W == 6) rc = run_batch_dispatch<6>(E, r.rule, jobs, t, r.hbins, r.lbins, false, false, &ms);
+            else             rc = run_batch_dispatch<5>(E, r.rule, jobs, t, r.hbins, r.lbins, h.dev.walk != 0, false, &ms);
             dev += ms;
         }
     }
@@ -566,8 +610,16 @@ void serve_loop(Engine *Ep) {
                 E.pending.pop_front();
             }
             int rc = 0;
-            h->slot = alloc_slot(E);
-            if (h->slot < 0) rc = fail(40, "no free request slot");
+            {
+                war_tuning tt;
+                resolve_tuning(E, h->rq, tt);
+                rc = check_tuning(tt);
+                if (!rc) rc = layout_of(h->rq, tt, &h->layout);
+            }
+            if (!rc) {
+                h->slot = alloc_slot(E);
+                if (h->slot < 0) rc = fail(40, "no free request slot");
+            }
             if (!rc) rc = req_alloc(E, *h);
             if (rc) {
                 h->status = rc; h->err = t_err; req_free(E, *h);
@@ -582,10 +634,10 @@ void serve_loop(Engine *Ep) {
         if (active.empty()) continue;
         // one step per (layout, rule) group
         int rc = 0;
-        for (int W : {4, 5}) {
+        for (int W : {4, 5, 6}) {
             for (int rule : {0, 1}) {
                 std::vector<std::shared_ptr<ReqHost>> grp;
-                for (auto &h : active) if ((h->rq.n <= 16 ? 4 : 5) == W && h->rq.rule == rule && h->next < h->rq.hi) grp.push_back(h);
+                for (auto &h : active) if (h->layout == W && h->rq.rule == rule && h->next < h->rq.hi) grp.push_back(h);
                 if (grp.empty()) continue;
                 war_tuning gt;
                 resolve_tuning(E, grp[0]->rq, gt);
@@ -605,6 +657,7 @@ void serve_loop(Engine *Ep) {
                 }
                 float ms = 0;
                 rc = (W == 4) ? run_batch_dispatch<4>(E, rule, jobs, gt, hb, lb, walk, true, &ms)
+                   : (W == 6) ? run_batch_dispatch<6>(E, rule, jobs, gt, hb, lb, false, true, &ms)
                               : run_batch_dispatch<5>(E, rule, jobs, gt, hb, lb, walk, true, &ms);
                 for (auto &h : grp) h->dev_ms += ms;   // device time of the steps the request took part in
                 if (rc) {
@@ -832,13 +885,16 @@ int war_autotune(int32_t n, int32_t mode, int32_t rule, double budget_ms, war_tu
     double best_ms = 1e30;
     rc = trial(cur, &best_ms);
     if (rc) return rc;
-    // coordinate descent over (brent_cap, ranks_per_refill, waves_per_cu, stage_kind)
+    // coordinate descent over (brent_cap, ranks_per_refill, waves_per_cu, stage_kind, pile layout);
+    // the layout dimension exists for n = 17 (5-bit piles, or 4-bit with card 16 held apart)
     const int caps[] = {7, 15, 31, 63, 127};
     const int rprs[] = {256, 1024, 2048, 4096};
     const int wpcs[] = {8, 16, 24, 28, 32};
     const int kinds[] = {1, 2};
+    const int lays[] = {5, 6};
+    const int ndim = held_ok(r) ? 5 : 4;
     for (int pass = 0; pass < 2; pass++) {
-        for (int dim = 0; dim < 4; dim++) {
+        for (int dim = 0; dim < ndim; dim++) {
             const int nc = dim == 0 ? 5 : dim == 1 ? 4 : dim == 2 ? 5 : 2;
             war_tuning bestt = cur;
             for (int i = 0; i < nc; i++) {
@@ -847,14 +903,15 @@ int war_autotune(int32_t n, int32_t mode, int32_t rule, double budget_ms, war_tu
                 if (dim == 0) t.brent_cap = caps[i];
                 else if (dim == 1) t.ranks_per_refill = rprs[i];
                 else if (dim == 2) t.waves_per_cu = wpcs[i];
-                else t.stage_kind = kinds[i];
+                else if (dim == 3) t.stage_kind = kinds[i];
+                else t.pile_layout = lays[i];
                 if (memcmp(&t, &cur, sizeof t) == 0) continue;
                 double ms = 0;
                 rc = trial(t, &ms);
                 if (rc) return rc;
                 if (E.verbose)
-                    fprintf(stderr, "autotune n=%d mode=%d cap=%d tile=%d wpc=%d kind=%d: %.3f ms\n", n, mode,
-                            t.brent_cap, t.ranks_per_refill, t.waves_per_cu, t.stage_kind, ms);
+                    fprintf(stderr, "autotune n=%d mode=%d cap=%d tile=%d wpc=%d kind=%d layout=%d: %.3f ms\n", n,
+                            mode, t.brent_cap, t.ranks_per_refill, t.waves_per_cu, t.stage_kind, t.pile_layout, ms);
                 if (ms < best_ms) { best_ms = ms; bestt = t; }
             }
             cur = bestt;

@@ -71,9 +71,16 @@ template <int W> struct Queue {
     u32 *cnt;    // records per tile
 };
 
+// Layout 6 (n = 17 only, so n is not stored): an alive holder's word has <= 15 cards, so its top
+// nibble takes the word's length; when that length is >= 1 the other player holds <= 15 cards and
+// the other word's top nibble takes the depth of card n-1 (length 0 means depth 0).
 template <int W> __device__ __forceinline__ void pack(const St<W> &s, u32 n, typename Lay<W>::T &qa, typename Lay<W>::T &qb) {
     typedef typename Lay<W>::T T;
-    if constexpr (W == 4) {
+    if constexpr (W == 6) {
+        (void)n;
+        qa = s.a | ((u64)(s.sa >> 2) << 60);
+        qb = s.b | ((u64)(s.pb >> 2) << 60);
+    } else if constexpr (W == 4) {
         qa = s.a | ((u64)(s.sa >> 2) << 60);
         qb = s.b | ((u64)(n - 1) << 60);
     } else {
@@ -83,7 +90,13 @@ template <int W> __device__ __forceinline__ void pack(const St<W> &s, u32 n, typ
 }
 template <int W> __device__ __forceinline__ void unpack(typename Lay<W>::T qa, typename Lay<W>::T qb, St<W> &s, u32 &n) {
     typedef typename Lay<W>::T T;
-    if constexpr (W == 4) {
+    if constexpr (W == 6) {
+        const u32 lh = (u32)(qa >> 60);
+        s.a = qa & 0x0FFFFFFFFFFFFFFFull; s.sa = lh << 2;
+        s.b = lh ? (qb & 0x0FFFFFFFFFFFFFFFull) : qb;
+        s.pb = lh ? (u32)(qb >> 60) << 2 : 0u;
+        n = HELD_N;
+    } else if constexpr (W == 4) {
         s.a = qa & 0x0FFFFFFFFFFFFFFFull; s.sa = (u32)(qa >> 60) << 2;
         s.b = qb & 0x0FFFFFFFFFFFFFFFull; n = (u32)(qb >> 60) + 1u;
     } else {
@@ -306,7 +319,7 @@ __device__ __forceinline__ u64 play(const StageArgs<W> &A, WaveAcc &acc, St<W> &
                 dm = ballot(s.a == t.a) & alive;
             } while (!(em | dm) && age < wend);
             dm &= ~em;
-            if (dm) dm &= ballot(s.b == t.b) & ballot(s.sa == t.sa);
+            if (dm) dm &= ballot(rest_same<W>(s, t));
             if (em | dm) {
                 const u32 lam = age - R;
                 if (uniform_req) {
@@ -408,7 +421,8 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs<W> A) {
             const Job J = A.jobs[lo];
             const ReqDev &Rq = reqs[J.req];
             const u32 n = Rq.n, mode = Rq.mode;
-            const u32 C = W * n - 2u * W;
+            const u32 C = c_of<W>(n);
+            constexpr int PW = Lay<W>::PW;         // the permutation's bits per card
             const u32 per = A.per;
             acc.select(J.req, reqs);
             LaneCtx L;
@@ -416,20 +430,19 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs<W> A) {
             const u64 first = J.lo + (g0 - J.tile_base) * (u64)TP + (u64)lane * per;
             const u32 cnt = first < J.hi ? (u32)((J.hi - first) < per ? (J.hi - first) : per) : 0u;
             L.pos = first;
-            T q = 0;
+            typename Lay<W>::P q = 0;
             int k = 0;
             if (cnt) {
-                if (mode == 2) { const u32 m = n - 1; const u64 r = first / m; k = (int)(first - r * m); q = unrank<W>(r, (int)m); }
-                else q = unrank<W>(first, (int)n);
+                if (mode == 2) { const u32 m = n - 1; const u64 r = first / m; k = (int)(first - r * m); q = unrank<PW>(r, (int)m); }
+                else q = unrank<PW>(first, (int)n);
             }
             for (u32 j = 0; j < per; j++) {
                 if (ballot(j < cnt) == 0) break;
                 bool act = j < cnt;
-                St<W> s;
-                s.a = 0; s.b = 0; s.sa = 0;
+                St<W> s = {};
                 if (act) {
                     s = (mode == 2) ? canon<W>(q, (int)n, k) : deal<W>(q, (int)n);
-                    if (mode == 1) act = top_card_to_p1<W>(q, (int)n);
+                    if (mode == 1) act = top_card_to_p1<PW>(q, (int)n);
                 }
                 const u64 act_m = ballot(act);
                 // a position that is over before it starts: rounds 0 (gpu/war_gpu.py:80)
@@ -442,8 +455,8 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs<W> A) {
                     out_n += __popcll(alive);
                 }
                 if (j + 1 < cnt) {
-                    if (mode == 2) { if (++k == (int)n - 1) { k = 0; q = next_perm<W>(q, (int)n - 1); } }
-                    else q = next_perm<W>(q, (int)n);
+                    if (mode == 2) { if (++k == (int)n - 1) { k = 0; q = next_perm<PW>(q, (int)n - 1); } }
+                    else q = next_perm<PW>(q, (int)n);
                 }
                 L.pos++;
             }
@@ -475,18 +488,16 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs<W> A) {
             for (u32 b = 0; b < total; b += 64) {
                 const u32 kk = b + lane;
                 const bool act = kk < total;
-                St<W> s;
+                St<W> s = {};
                 u32 n = 2;
                 LaneCtx L;
                 L.req = req0; L.pos = 0; L.home = 0; L.walk = false;
                 if (act) {
                     unpack<W>(na, nb, s, n);
                     L.req = nreq; L.pos = npos; L.home = nhome;
-                } else {
-                    s.a = 0; s.b = 0; s.sa = 0;
                 }
                 if (b + 64 < total) fetch(b + 64 + lane);
-                const u32 C = W * n - 2u * W;
+                const u32 C = c_of<W>(n);
                 u64 alive = ballot(act);
                 bool uniform_req = true;
                 if constexpr ((FLAGS & F_MULTI) != 0) {

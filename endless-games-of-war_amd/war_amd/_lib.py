@@ -20,7 +20,8 @@ DETECTORS = {"brent": DET_BRENT, "streak": DET_STREAK}
 
 class WarTuning(ctypes.Structure):
     _fields_ = [("waves_per_cu", ctypes.c_int32), ("ranks_per_refill", ctypes.c_int32),
-                ("brent_cap", ctypes.c_int32), ("plain_per", ctypes.c_int32), ("stage_kind", ctypes.c_int32)]
+                ("brent_cap", ctypes.c_int32), ("plain_per", ctypes.c_int32), ("stage_kind", ctypes.c_int32),
+                ("pile_layout", ctypes.c_int32)]
 
 
 class WarRequest(ctypes.Structure):

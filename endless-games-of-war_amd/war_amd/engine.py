@@ -30,17 +30,19 @@ STAGE_KINDS = {"refill": 1, "lockstep": 2}
 
 def _tuning(t):
     if t is None:
-        return WarTuning(0, 0, 0, 0, 0)
+        return WarTuning(0, 0, 0, 0, 0, 0)
     if isinstance(t, WarTuning):
         return t
     sk = t.get("stage_kind", 0)
     return WarTuning(int(t.get("waves_per_cu", 0)), int(t.get("ranks_per_refill", 0)),
-                     int(t.get("brent_cap", 0)), int(t.get("plain_per", 0)), int(STAGE_KINDS.get(sk, sk)))
+                     int(t.get("brent_cap", 0)), int(t.get("plain_per", 0)), int(STAGE_KINDS.get(sk, sk)),
+                     int(t.get("pile_layout", 0)))
 
 
 def _tuning_dict(t):
     return {"waves_per_cu": t.waves_per_cu, "ranks_per_refill": t.ranks_per_refill, "brent_cap": t.brent_cap,
-            "plain_per": t.plain_per, "stage_kind": {1: "refill", 2: "lockstep"}.get(t.stage_kind, t.stage_kind)}
+            "plain_per": t.plain_per, "stage_kind": {1: "refill", 2: "lockstep"}.get(t.stage_kind, t.stage_kind),
+            "pile_layout": t.pile_layout}
 
 
 def make_request(n, mode="full", lo=0, hi=None, rule=0, kernel="persistent", hbins=0, lbins=0, dbins=0,

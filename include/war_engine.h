@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define WAR_ABI_VERSION 1
+#define WAR_ABI_VERSION 2
 
 /* mode */
 #define WAR_MODE_FULL  0   /* deal ranks r in [0, n!), deck positions 0,2,4.. to player 1     */
@@ -54,6 +54,10 @@ typedef struct war_tuning {
     int32_t plain_per;         /* plain kernel: consecutive positions per lane (1..4096)        */
     int32_t stage_kind;        /* persistent stages after the first: 1 = refill (a lane that
                                   finishes takes the next record at once), 2 = lockstep; 0 = default */
+    int32_t pile_layout;       /* 0 = default; 4 = 4-bit piles (n <= 16); 5 = 5-bit piles (any n);
+                                  6 = n = 17 with card 16 held apart and 4-bit piles (the layout
+                                  of gpu/war17.py; persistent kernel, Brent, no cycle walks).
+                                  The default is 6 where it applies, else 4 for n <= 16, else 5 */
 } war_tuning;
 
 typedef struct war_request {

@@ -19,7 +19,7 @@ def test_library_exports_header():
     assert len(names) >= 14
     for nm in names:
         assert hasattr(lib, nm), nm
-    assert lib.war_abi_version() == 1
+    assert lib.war_abi_version() == 2
 
 
 def test_no_python_fallback(monkeypatch, tmp_path):

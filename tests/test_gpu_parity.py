@@ -173,3 +173,39 @@ def test_streak_refusals(eng):
         kw.setdefault("kernel", "plain")
         with pytest.raises(WarError, match="streak"):
             eng.run_range(n, lo=0, hi=100, detector="streak", **kw)
+
+
+# ---- pile layouts: n = 17 with card 16 held apart (layout 6) against the 5-bit layout and the oracle
+@pytest.mark.parametrize("rule", [0, 1])
+def test_held_layout_n17(eng, orc, rule):
+    rng = random.Random(17 + rule)
+    cases = [("full", 0, 70_000), ("full", orc.total(17, "full") - 50_000, 50_000), ("canon", 0, 40_000)]
+    for _ in range(3):
+        mode = rng.choice(["full", "canon"])
+        ln = rng.choice([1, 64, 65, 100_000])
+        cases.append((mode, rng.randrange(0, orc.total(17, mode) - ln), ln))
+    for mode, lo, ln in cases:
+        print("case held", rule, mode, lo, ln, flush=True)
+        g6, want = check(eng, orc, 17, mode, lo, lo + ln, rule=rule, tuning={"pile_layout": 6})
+        g5, _ = check(eng, orc, 17, mode, lo, lo + ln, rule=rule, tuning={"pile_layout": 5})
+        assert g6["turns"] == g5["turns"] == want["turns"]
+    assert eng.tuning_for(17, "full")["pile_layout"] == 0           # auto: 6 where it applies
+
+
+def test_layout_5_for_small_n(eng, orc):
+    """The 5-bit layout is exact for any n: the same counts as the 4-bit one."""
+    for n, mode, lo, ln in [(13, "full", 10**9, 300_000), (14, "canon", 7 * 10**9, 200_000), (9, "full", 0, 362_880)]:
+        print("case layout5", n, mode, lo, ln, flush=True)
+        check(eng, orc, n, mode, lo, lo + ln, tuning={"pile_layout": 5})
+
+
+def test_layout_refusals(eng):
+    for n, kw in [(16, {}), (17, {"kernel": "plain"}), (17, {"mode": "canon", "shape_blk": 1}),
+                  (18, {})]:
+        kw = dict(kw)
+        with pytest.raises(WarError, match="pile_layout 6"):
+            eng.run_range(n, lo=0, hi=100, tuning={"pile_layout": 6}, **kw)
+    with pytest.raises(WarError, match="pile_layout 4"):
+        eng.run_range(17, lo=0, hi=100, tuning={"pile_layout": 4})
+    with pytest.raises(WarError, match="4-bit"):
+        eng.run_range(11, "canon", lo=0, hi=100, distinct=True, tuning={"pile_layout": 5})
