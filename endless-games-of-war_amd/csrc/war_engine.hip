@@ -26,6 +26,7 @@
 
 #include "war_engine.h"
 #include "war_kernels.h"
+#include "war_tools.h"
 
 using namespace war;
 
@@ -84,6 +85,9 @@ struct ReqHost {
     std::string err;
     size_t bytes = 0;
     int layout = 4;                 // pile layout the request runs in (4, 5 or 6)
+    bool reach = false;             // war_reach: search each distinct cycle's predecessors for a deal
+    u64 reach_cap = 0;
+    war_reach_result *rres = nullptr;
 };
 
 // Device buffers of one pile layout: two region buffers (ping-pong between stages) of `cap` slots
@@ -128,6 +132,7 @@ struct Engine {
     void *stage_host = nullptr;      // pinned staging for the job list
     ReqDev *reqs = nullptr;          // device table of MAXSLOTS
     QueueBufs qb4, qb5;
+    u64 *reach_stack = nullptr;     // reach searches: REACH_DEPTH entries per thread of a walk grid
     std::map<GraphKey, hipGraphExec_t> graphs;
     std::vector<int> free_slots;
     std::mutex mu;                   // serialises all GPU work of the engine
@@ -246,7 +251,8 @@ int req_alloc(Engine &E, ReqHost &h) {
         hlog2 = r.n <= 13 ? 21 : (r.n == 14 ? 17 : 27);
         nt = (size_t)1 << hlog2;
     }
-    h.bytes = (nc + nh + nl + nd + nt) * sizeof(u64);
+    const size_t nr = h.reach ? 4096 : 0;
+    h.bytes = (nc + nh + nl + nd + nt + nr) * sizeof(u64);
     void *p = nullptr;
     HIPCHK(hipMalloc(&p, h.bytes));
     HIPCHK(hipMemsetAsync(p, 0, h.bytes, E.st));
@@ -263,6 +269,9 @@ int req_alloc(Engine &E, ReqHost &h) {
     d.lenhist = d.hist + nh;
     d.dhist = d.lenhist + nl;
     d.htab = nt ? d.dhist + nd : nullptr;
+    d.reach = h.reach ? 1u : 0u;
+    d.reach_cap = h.reach_cap;
+    d.nolen = nr ? d.dhist + nd + nt : nullptr;
     HIPCHK(hipMemcpyAsync(&E.reqs[h.slot], &d, sizeof(ReqDev), hipMemcpyHostToDevice, E.st));
     return 0;
 }
@@ -277,6 +286,17 @@ int req_collect(Engine &E, ReqHost &h) {
         o->cyc = buf[C_CYC]; o->done = buf[C_DONE]; o->maxfin = buf[C_MAXFIN]; o->turns = buf[C_TURNS];
         o->oncycle = buf[C_ONCYCLE]; o->distinct_minkey = buf[C_MINKEY]; o->distinct_hashset = buf[C_HASH];
         o->shape_nonstd = buf[C_NONSTD]; o->shape_unchk = buf[C_UNCHK]; o->undetermined = buf[C_UNDET];
+        if (h.rres) {
+            war_reach_result *q = h.rres;
+            q->classes = buf[C_MINKEY]; q->reached = buf[C_RYES]; q->not_reached = buf[C_RNO];
+            q->undetermined = buf[C_RUND]; q->device_ms = h.dev_ms;
+            if (q->nolen && q->nbins > 0) {
+                std::vector<u64> nl(4096);
+                HIPCHK(hipMemcpy(nl.data(), h.dev.nolen, 4096 * sizeof(u64), hipMemcpyDeviceToHost));
+                for (int i = 0; i < q->nbins; i++) q->nolen[i] = 0;
+                for (int i = 0; i < 4096; i++) q->nolen[std::min(i, q->nbins - 1)] += nl[(size_t)i];
+            }
+        }
         const u64 *hp = buf.data() + C_N, *lp = hp + r.hbins, *dp = lp + r.lbins;
         if (o->hist) for (int i = 0; i < std::min(o->hbins, r.hbins); i++) o->hist[i] = hp[i];
         if (o->lenhist) for (int i = 0; i < std::min(o->lbins, r.lbins); i++) o->lenhist[i] = lp[i];
@@ -318,6 +338,10 @@ template <int W> int ensure_queues(Engine &E, bool walk) {
         }
         Q.cap = cap;
         Q.tmax = tmax;
+    }
+    if (walk && !E.reach_stack) {
+        // the walk grids have at most CUs x 8 blocks of 256 threads
+        HIPCHK(hipMalloc((void **)&E.reach_stack, (size_t)E.cus * 8 * 256 * REACH_DEPTH * 2 * sizeof(u64)));
     }
     if (walk && !Q.have_c) {
         HIPCHK(hipMalloc(&Q.ca, Q.cap * sizeof(T)));
@@ -424,6 +448,7 @@ int build_graph(Engine &E, const GraphKey &key, hipGraphExec_t *out) {
         if (key.walk) {
             WalkArgs<W> WA{};
             WA.pipe = E.pipe; WA.hdr = E.hdr; WA.reqs = E.reqs; WA.cq = cyc_queue<W>(E); WA.tile_pos = 64u * (u32)key.per;
+            WA.stack = E.reach_stack;
             hipLaunchKernelGGL((k_walk<W, RULE>), dim3(blocks), dim3(256), 0, E.st, WA);
         }
     }
@@ -509,6 +534,7 @@ int run_plain_batch(Engine &E, ReqHost &h, u64 lo, u64 hi, const war_tuning &t, 
     if (walk) {
         WalkArgs<W> WA{};
         WA.pipe = E.pipe; WA.hdr = E.hdr; WA.reqs = E.reqs; WA.cq = cyc_queue<W>(E); WA.tile_pos = PLAIN_CYC_TILE;
+        WA.stack = E.reach_stack;
         hipLaunchKernelGGL((k_walk<W, RULE>), dim3(std::max(1, E.cus * 8)), dim3(256), 0, E.st, WA);
         HIPCHK(hipGetLastError());
     }
@@ -838,6 +864,99 @@ int war_run_range(const war_request *req, war_result *res) {
     }
     res->latency_ms = now_ms() - t0;
     return rc;
+}
+
+int war_reach(int32_t n, uint64_t lo, uint64_t hi, uint64_t cap, war_reach_result *out) {
+    t_err.clear();
+    if (!g_eng) return fail(60, "war_init has not been called");
+    if (!out) return fail(1, "null result");
+    if (n < 2 || n > 15) return fail(2, "reach needs 2 <= n <= 15 (got %d)", n);
+    if (cap < 1) return fail(2, "cap must be >= 1");
+    war_request r{};
+    r.n = n; r.mode = WAR_MODE_CANON; r.rule = 0; r.kernel = WAR_KERNEL_PERSISTENT;
+    r.lo = lo; r.hi = hi; r.distinct = 1;
+    int rc = check_request(&r);
+    if (rc) return rc;
+    war_result res{};
+    ReqHost h;
+    h.rq = r; h.res = &res; h.reach = true; h.reach_cap = cap; h.rres = out;
+    {
+        std::lock_guard<std::mutex> g(g_eng->mu);
+        rc = run_request_locked(*g_eng, h);
+    }
+    return rc;
+}
+
+int war_cycle_info_batch(const war_pile_position *pos, int32_t count, int32_t rule, uint64_t cap,
+                         war_cycle_info *out) {
+    t_err.clear();
+    if (!g_eng) return fail(60, "war_init has not been called");
+    if (count < 0 || count > 65536) return fail(2, "count must be in [0, 65536]");
+    if (count == 0) return 0;
+    if (!pos || !out) return fail(1, "null positions or results");
+    if (rule != 0 && rule != 1) return fail(4, "rule must be 0 or 1");
+    if (cap < 1) return fail(2, "cap must be >= 1");
+    std::vector<CInfoDev> H((size_t)count);
+    for (int q = 0; q < count; q++) {
+        const war_pile_position &P = pos[q];
+        if (P.n1 < 1 || P.n2 < 1 || P.n1 + P.n2 > 255 || !P.p1 || !P.p2)
+            return fail(2, "position %d: piles must be non-empty with at most 255 cards in all", q);
+        bool seen[256] = {false};
+        CInfoDev &d = H[(size_t)q];
+        memset(&d, 0, sizeof d);
+        d.l1 = (u32)P.n1; d.l2 = (u32)P.n2;
+        for (int p = 0; p < 2; p++) {
+            const int L = p ? P.n2 : P.n1;
+            const uint8_t *c = p ? P.p2 : P.p1;
+            u64 sm = 0, ws = 0;
+            for (int i = 0; i < L; i++) {
+                if (seen[c[i]]) return fail(2, "position %d: card %d appears twice", q, (int)c[i]);
+                seen[c[i]] = true;
+                d.start[p][i] = c[i];
+                d.ring[p][i] = c[i]; d.ring[p + 2][i] = c[i];
+                sm += c[i]; ws += (u64)c[i] * (u64)(i + 1);
+            }
+            d.len[p] = d.len[p + 2] = (u32)L;
+            d.sum[p] = d.sum[p + 2] = sm;
+            d.wsum[p] = d.wsum[p + 2] = ws;
+        }
+        d.power = 1; d.phase = CI_BRENT;
+    }
+    Engine &E = *g_eng;
+    std::lock_guard<std::mutex> g(E.mu);
+    CInfoDev *D = nullptr;
+    HIPCHK(hipMalloc((void **)&D, H.size() * sizeof(CInfoDev)));
+    int rc = 0;
+    do {
+        if (hipMemcpy(D, H.data(), H.size() * sizeof(CInfoDev), hipMemcpyHostToDevice) != hipSuccess) {
+            rc = fail(50, "copy to the device failed"); break;
+        }
+        const unsigned blocks = (unsigned)((count + CI_LANES - 1) / CI_LANES);
+        for (;;) {
+            hipLaunchKernelGGL(k_cycinfo, dim3(blocks), dim3(CI_LANES), 0, E.st, D, (u32)count, (u32)rule, (u64)cap,
+                               (u64)1 << 22);
+            if (hipGetLastError() != hipSuccess || hipStreamSynchronize(E.st) != hipSuccess) {
+                rc = fail(50, "cycle-info kernel failed"); break;
+            }
+            if (hipMemcpy(H.data(), D, H.size() * sizeof(CInfoDev), hipMemcpyDeviceToHost) != hipSuccess) {
+                rc = fail(50, "copy from the device failed"); break;
+            }
+            bool all = true;
+            for (const CInfoDev &d : H) all = all && d.phase == CI_DONE;
+            if (all) break;
+        }
+    } while (false);
+    (void)hipFree(D);
+    if (rc) return rc;
+    for (int q = 0; q < count; q++) {
+        const CInfoDev &d = H[(size_t)q];
+        war_cycle_info &o = out[q];
+        o.status = (int32_t)d.status; o.pad = 0;
+        o.turns = d.rounds;
+        o.lam = d.status == 1 ? d.lam : 0;
+        o.mu = d.status == 1 ? d.mu : 0;
+    }
+    return 0;
 }
 
 int war_autotune(int32_t n, int32_t mode, int32_t rule, double budget_ms, war_tuning *best) {

@@ -27,7 +27,7 @@ namespace war {
 
 enum {
     C_CYC = 0, C_DONE, C_MAXFIN, C_TURNS, C_ONCYCLE, C_MINKEY, C_NONSTD, C_UNCHK, C_UNDET,
-    C_HASH, C_HOVF, C_QOVF, C_N = 16
+    C_HASH, C_HOVF, C_QOVF, C_RYES, C_RNO, C_RUND, C_N = 16
 };
 
 constexpr u32 AGE_CAP = 1u << 24;      // safety stop of the last stage (never reached by a valid game)
@@ -36,8 +36,10 @@ constexpr int MAXSTAGE = 16;
 
 struct ReqDev {
     u32 n, mode, rule, hbins, lbins, dbins, distinct, shape_blk;
-    u32 walk, hlog2, pad0, pad1;
+    u32 walk, hlog2, reach, pad1;
     u64 *cnt, *hist, *lenhist, *dhist, *htab;
+    u64 reach_cap;      // reach: positions a cycle's backward search may visit
+    u64 *nolen;         // reach: cycle lengths of the cycles no deal enters (4096 bins)
 };
 
 struct Job { u64 lo, hi, tile_base; u32 req, pad; };
@@ -916,7 +918,67 @@ template <int W> struct WalkArgs {
     CycQueue<W> cq;
     u32 tile_pos;
     u32 zero;      // always 0 (see Zh)
+    u64 *stack;    // reach: REACH_DEPTH 16-byte entries per thread of the grid
 };
+
+// ------------------------------------------------------------------------------------------
+// Is a distinct cycle entered by an ordinary deal? (cpu/reach_cycles.c:32-69) Positions of deal
+// size hold ceil(n/2) and floor(n/2) cards. A position has at most two predecessors: undo a win of
+// player w, whose bottom two cards must be (winner, loser) with winner > loser (rule 0): they go
+// back on top of the two piles. Off the loop the predecessors form a finite tree, searched depth
+// first from each loop position (skipping the loop's own predecessor) with a bounded per-thread
+// stack in device memory. 1 = reached, 0 = not, -1 = undetermined (more than `cap` positions
+// visited, or the stack is full).
+constexpr u32 REACH_DEPTH = 64;
+
+__device__ __forceinline__ bool deal_sized4(const St<4> &s, u32 n) {
+    const u32 la = (s.sa >> 2) + 1u;
+    return la == (n + 1u) / 2u || la == n / 2u;
+}
+__device__ __forceinline__ bool undo_win4(const St<4> &s, u32 w, u32 n, St<4> &o) {
+    const u32 la = (s.sa >> 2) + 1u;
+    const u32 L = w ? n - la : la;
+    if (L < 2u) return false;
+    const u64 word = w ? s.b : s.a;
+    const u64 cw = (word >> (4u * (L - 2u))) & 15ull, cl = (word >> (4u * (L - 1u))) & 15ull;
+    if (cw <= cl) return false;
+    const u64 back = ((word & lowmask_<u64>(4u * (L - 2u))) << 4) | cw;
+    if (w == 0u) { o.a = back; o.b = (s.b << 4) | cl; o.sa = s.sa - 4u; }
+    else         { o.b = back; o.a = (s.a << 4) | cl; o.sa = s.sa + 4u; }
+    return true;
+}
+template <int RULE>
+__device__ int reach_search(const St<4> &s0, u32 lam, u32 n, u32 C, u64 cap, u64 *stk, Zh<4> &z) {
+    St<4> prev = s0;
+    for (u32 i = 0; i + 1u < lam; i++) turn<4, RULE>(prev, C, z);
+    St<4> t = s0;
+    u64 visited = 0;
+    for (u32 i = 0; i < lam; i++) {
+        if (deal_sized4(t, n)) return 1;
+        for (u32 w = 0; w < 2u; w++) {
+            St<4> q;
+            if (!undo_win4(t, w, n, q) || same<4>(q, prev)) continue;
+            u32 sp = 0;
+            stk[0] = q.a | ((u64)(q.sa >> 2) << 60); stk[1] = q.b; sp = 1;
+            while (sp) {
+                sp--;
+                St<4> u;
+                u.a = stk[2 * sp] & 0x0FFFFFFFFFFFFFFFull; u.sa = (u32)(stk[2 * sp] >> 60) << 2; u.b = stk[2 * sp + 1];
+                if (deal_sized4(u, n)) return 1;
+                if (++visited > cap) return -1;
+                for (u32 w2 = 0; w2 < 2u; w2++) {
+                    St<4> r;
+                    if (!undo_win4(u, w2, n, r)) continue;
+                    if (sp == REACH_DEPTH) return -1;
+                    stk[2 * sp] = r.a | ((u64)(r.sa >> 2) << 60); stk[2 * sp + 1] = r.b; sp++;
+                }
+            }
+        }
+        prev = t;
+        turn<4, RULE>(t, C, z);
+    }
+    return 0;
+}
 
 // Insert key K into the open-addressing set; returns 1 if it was new. The probe length is bounded;
 // a full table raises C_HOVF instead of looping.
@@ -1052,8 +1114,16 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs<W> A) {
                             atomicAdd((unsigned long long *)&R.dhist[d < (u64)(R.dbins - 1) ? d : (u64)(R.dbins - 1)], 1ull);
                         }
                         if (R.distinct) {
-                            if (key4(s0.a, s0.b, bits1<W>(s0)) == K) atomicAdd((unsigned long long *)&R.cnt[C_MINKEY], 1ull);
+                            const bool rep = key4(s0.a, s0.b, bits1<W>(s0)) == K;
+                            if (rep) atomicAdd((unsigned long long *)&R.cnt[C_MINKEY], 1ull);
                             if (hash_insert(R, K)) atomicAdd((unsigned long long *)&R.cnt[C_HASH], 1ull);
+                            if (rep && R.reach && RULE == 0) {
+                                const u64 gtid = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+                                const int res = reach_search<RULE>(s0, lam, n, C, R.reach_cap,
+                                                                   A.stack + gtid * (2u * REACH_DEPTH), z);
+                                atomicAdd((unsigned long long *)&R.cnt[res == 1 ? C_RYES : res == 0 ? C_RNO : C_RUND], 1ull);
+                                if (res == 0) atomicAdd((unsigned long long *)&R.nolen[lam < 4095u ? lam : 4095u], 1ull);
+                            }
                         }
                     }
                 }

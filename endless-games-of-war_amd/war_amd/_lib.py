@@ -46,6 +46,22 @@ class WarResult(ctypes.Structure):
                 ("device_ms", ctypes.c_double), ("latency_ms", ctypes.c_double)]
 
 
+class WarReachResult(ctypes.Structure):
+    _fields_ = [("classes", ctypes.c_uint64), ("reached", ctypes.c_uint64), ("not_reached", ctypes.c_uint64),
+                ("undetermined", ctypes.c_uint64), ("nolen", ctypes.POINTER(ctypes.c_uint64)), ("nbins", ctypes.c_int32),
+                ("device_ms", ctypes.c_double)]
+
+
+class WarPilePosition(ctypes.Structure):
+    _fields_ = [("n1", ctypes.c_int32), ("n2", ctypes.c_int32), ("p1", ctypes.POINTER(ctypes.c_uint8)),
+                ("p2", ctypes.POINTER(ctypes.c_uint8))]
+
+
+class WarCycleInfo(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("pad", ctypes.c_int32), ("turns", ctypes.c_uint64),
+                ("lam", ctypes.c_uint64), ("mu", ctypes.c_uint64)]
+
+
 class WarDevinfo(ctypes.Structure):
     _fields_ = [("cus", ctypes.c_int32), ("max_clock_mhz", ctypes.c_int32), ("wave_size", ctypes.c_int32),
                 ("reserved", ctypes.c_int32), ("free_bytes", ctypes.c_uint64), ("total_bytes", ctypes.c_uint64),
@@ -82,6 +98,10 @@ _SIGS = {
     "war_tuning_for": ([ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(WarTuning)], ctypes.c_int),
     "war_autotune": ([ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.POINTER(WarTuning)],
                      ctypes.c_int),
+    "war_reach": ([ctypes.c_int32, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(WarReachResult)],
+                  ctypes.c_int),
+    "war_cycle_info_batch": ([ctypes.POINTER(WarPilePosition), ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64,
+                              ctypes.POINTER(WarCycleInfo)], ctypes.c_int),
     "war_serve_start": ([ctypes.c_int32, ctypes.c_uint64], ctypes.c_int),
     "war_serve_stop": ([], ctypes.c_int),
     "war_submit": ([ctypes.POINTER(WarRequest), ctypes.POINTER(WarResult), ctypes.POINTER(ctypes.c_uint64)],

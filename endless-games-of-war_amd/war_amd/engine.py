@@ -10,7 +10,8 @@ import threading
 
 from . import _lib
 from ._lib import (DETECTORS, KERNEL_PERSISTENT, KERNEL_PLAIN, KERNELS, MODE_CANON, MODE_FULL, MODE_HALF,
-                   WarDevinfo, WarRequest, WarResult, WarTuning, check)
+                   WarCycleInfo, WarDevinfo, WarPilePosition, WarReachResult, WarRequest, WarResult, WarTuning,
+                   check)
 
 MODES = {"full": MODE_FULL, "half": MODE_HALF, "canon": MODE_CANON}
 
@@ -129,6 +130,38 @@ class Engine:
         buf = _ResultBuf(req)
         check(self.lib.war_run_range(ctypes.byref(req), ctypes.byref(buf.res)))
         return buf.as_dict()
+
+    # ---- tools around the enumeration
+    def reach(self, n, lo=0, hi=None, cap=200_000_000, nbins=4096):
+        """Distinct cycles of canonical positions [lo, hi) and whether ordinary deals enter them."""
+        if hi is None:
+            hi = total_positions(n, MODE_CANON)
+        nl = (ctypes.c_uint64 * max(1, nbins))()
+        r = WarReachResult()
+        r.nolen = ctypes.cast(nl, ctypes.POINTER(ctypes.c_uint64))
+        r.nbins = nbins
+        check(self.lib.war_reach(int(n), int(lo), int(hi), int(cap), ctypes.byref(r)))
+        return {"classes": r.classes, "reached": r.reached, "not_reached": r.not_reached,
+                "undetermined": r.undetermined, "nolen": [int(nl[i]) for i in range(nbins)],
+                "device_ms": r.device_ms}
+
+    def cycle_info(self, positions, rule=0, cap=10**12):
+        """positions: [(pile1, pile2), ...] of card values, top card first. Returns per position
+        {"status": "ends"|"cycles"|"unsettled", "turns", "lam", "mu"}."""
+        bufs = []
+        arr = (WarPilePosition * max(1, len(positions)))()
+        for i, (p1, p2) in enumerate(positions):
+            b1 = (ctypes.c_uint8 * max(1, len(p1)))(*p1)
+            b2 = (ctypes.c_uint8 * max(1, len(p2)))(*p2)
+            bufs.append((b1, b2))
+            arr[i].n1, arr[i].n2 = len(p1), len(p2)
+            arr[i].p1 = ctypes.cast(b1, ctypes.POINTER(ctypes.c_uint8))
+            arr[i].p2 = ctypes.cast(b2, ctypes.POINTER(ctypes.c_uint8))
+        out = (WarCycleInfo * max(1, len(positions)))()
+        check(self.lib.war_cycle_info_batch(arr, len(positions), int(rule), int(cap), out))
+        names = {0: "ends", 1: "cycles", 2: "unsettled"}
+        return [{"status": names[o.status], "turns": o.turns, "lam": o.lam, "mu": o.mu}
+                for o in list(out)[:len(positions)]]
 
     # ---- serving loop
     def serve_start(self, max_slots=64, step_positions=0):

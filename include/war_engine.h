@@ -95,6 +95,35 @@ typedef struct war_result {
     double   latency_ms;       /* serving: submit -> completion, host clock                     */
 } war_result;
 
+/* war_reach: the distinct cycles (up to swapping the players) met by the canonical positions of a
+ * range, each taken once at its position of minimum key, and whether an ordinary deal enters it
+ * (cpu/reach_cycles.c): a backward search from the cycle for a position of ceil(n/2) and
+ * floor(n/2) cards. */
+typedef struct war_reach_result {
+    uint64_t classes;          /* distinct cycles (positions at their cycle's minimum key)       */
+    uint64_t reached;          /* entered by a deal                                               */
+    uint64_t not_reached;
+    uint64_t undetermined;     /* the search visited more than `cap` positions (or its stack of
+                                  64 entries per search was full)                                */
+    uint64_t *nolen; int32_t nbins;  /* caller-owned: cycle lengths of the cycles not reached
+                                        (last bin = overflow), or NULL                           */
+    double   device_ms;
+} war_reach_result;
+
+/* war_cycle_info: one position of up to 255 cards (cpu/cycle_length.c). */
+typedef struct war_pile_position {
+    int32_t n1, n2;            /* cards of player 1 and player 2: >= 1 each, n1 + n2 <= 255     */
+    const uint8_t *p1, *p2;    /* card values, top card first; distinct                          */
+} war_pile_position;
+
+typedef struct war_cycle_info {
+    int32_t  status;           /* 0: the game ends; 1: it cycles; 2: not settled within cap turns */
+    int32_t  pad;
+    uint64_t turns;            /* 0: the game's length; 1: the turn Brent saw the repeat; 2: cap */
+    uint64_t lam;              /* 1: cycle length                                                */
+    uint64_t mu;               /* 1: turns played before the game enters its cycle               */
+} war_cycle_info;
+
 typedef struct war_devinfo {
     int32_t  cus;              /* compute units                                                  */
     int32_t  max_clock_mhz;
@@ -114,6 +143,19 @@ int war_device_info(war_devinfo *out);
 
 /* Synchronous range query: plays every position of [lo, hi) on the GPU and fills *res. */
 int war_run_range(const war_request *req, war_result *res);
+
+/* Reachability of the distinct cycles of canonical positions [lo, hi) of n cards (2 <= n <= 15,
+ * rule 0) by ordinary deals: replaces cpu/reach_cycles.c's main loop (:73-107) and search
+ * (:48-69). cap: positions one cycle's backward search may visit (the reference's default is
+ * 200000000). Synchronous. */
+int war_reach(int32_t n, uint64_t lo, uint64_t hi, uint64_t cap, war_reach_result *out);
+
+/* Cycle length and pre-period of `count` positions (one GPU lane each): replaces
+ * cpu/cycle_length.c:114-126 (Brent, then two copies lam apart stepped together). rule 0 or 1.
+ * cap bounds Brent's turns per position (the reference's default is 10^12). Synchronous; runs as
+ * a series of launches of at most 2^22 turns per lane. count <= 65536. */
+int war_cycle_info_batch(const war_pile_position *pos, int32_t count, int32_t rule, uint64_t cap,
+                         war_cycle_info *out);
 
 /* Engine-wide tuning override. A request's non-zero tuning fields win; then this override's
  * non-zero fields; then war_autotune's choice for the request's (n, mode, rule); then a built-in

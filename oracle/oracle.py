@@ -105,3 +105,37 @@ def unrank(r, m):
     p = (ctypes.c_int * max(1, m))()
     lib().orc_unrank(int(r), int(m), p)
     return [p[i] for i in range(m)]
+
+
+def cycle_info(p1, p2, rule=0, cap=10**12):
+    """One position of up to 255 byte cards (cpu/cycle_length.c). Returns
+    {"status": "ends"|"cycles"|"unsettled", "turns", "lam", "mu"}."""
+    L = lib()
+    f = L.orc_cycle_info
+    f.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int,
+                  ctypes.c_uint64, _U64P, _U64P, _U64P]
+    f.restype = ctypes.c_int
+    a = (ctypes.c_int * max(1, len(p1)))(*p1)
+    b = (ctypes.c_int * max(1, len(p2)))(*p2)
+    t, lam, mu = ctypes.c_uint64(0), ctypes.c_uint64(0), ctypes.c_uint64(0)
+    rc = f(a, len(p1), b, len(p2), int(rule), int(cap), ctypes.byref(t), ctypes.byref(lam), ctypes.byref(mu))
+    if rc < 0:
+        raise ValueError("oracle: %s" % L.orc_last_error().decode())
+    return {"status": {0: "ends", 1: "cycles", 2: "unsettled"}[rc], "turns": t.value,
+            "lam": lam.value if rc == 1 else 0, "mu": mu.value if rc == 1 else 0}
+
+
+def reach(n, threads=0, cap=200000000):
+    """Distinct cycles of n cards and whether ordinary deals enter them (cpu/reach_cycles.c)."""
+    L = lib()
+    f = L.orc_reach
+    f.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, _U64P, _U64P, _U64P, _U64P, _U64P]
+    f.restype = ctypes.c_int
+    c, y, no, u = (ctypes.c_uint64(0) for _ in range(4))
+    nl = (ctypes.c_uint64 * 4096)()
+    rc = f(int(n), int(threads), int(cap), ctypes.byref(c), ctypes.byref(y), ctypes.byref(no), ctypes.byref(u),
+           ctypes.cast(nl, _U64P))
+    if rc:
+        raise ValueError("oracle: %s" % L.orc_last_error().decode())
+    return {"classes": c.value, "reached": y.value, "not_reached": no.value, "undetermined": u.value,
+            "nolen": [int(nl[i]) for i in range(4096)]}

@@ -403,6 +403,153 @@ extern "C" int orc_play(int n, const int* p1, int l1, const int* p2, int l2, int
     return cyc ? 1 : 0;
 }
 
+// ---------------------------------------------------------------------------------------------
+// One position of up to 255 cards (cpu/cycle_length.c:16-30,114-126): byte cards in two rings.
+// Brent finds the cycle length lam; then one copy is advanced lam turns and both are stepped
+// together until they meet: the number of turns before the cycle (mu). Card values are any
+// distinct bytes. Returns 0 = the game ends (turns = its length), 1 = it cycles (turns = the
+// round Brent saw the repeat), 2 = not settled within cap turns.
+struct Ring {
+    unsigned char c[2][256];
+    int h[2], len[2];
+};
+static inline int ring_step(Ring& s, int rule) {
+    int x = s.c[0][s.h[0]], y = s.c[1][s.h[1]];
+    s.h[0] = (s.h[0] + 1) & 255; s.h[1] = (s.h[1] + 1) & 255;
+    s.len[0]--; s.len[1]--;
+    int w = x > y ? 0 : 1;
+    int hi = x > y ? x : y, lo = x > y ? y : x;
+    int first = rule ? lo : hi, second = rule ? hi : lo;
+    s.c[w][(s.h[w] + s.len[w]) & 255] = (unsigned char)first; s.len[w]++;
+    s.c[w][(s.h[w] + s.len[w]) & 255] = (unsigned char)second; s.len[w]++;
+    return s.len[0] == 0 || s.len[1] == 0;
+}
+static inline bool ring_same(const Ring& x, const Ring& y) {
+    for (int p = 0; p < 2; p++) {
+        if (x.len[p] != y.len[p]) return false;
+        for (int i = 0; i < x.len[p]; i++)
+            if (x.c[p][(x.h[p] + i) & 255] != y.c[p][(y.h[p] + i) & 255]) return false;
+    }
+    return true;
+}
+extern "C" int orc_cycle_info(const int* p1, int l1, const int* p2, int l2, int rule, u64 cap,
+                              u64* turns, u64* lam_out, u64* mu_out) {
+    if (l1 < 1 || l2 < 1 || l1 + l2 > 255) { g_err = "piles must be non-empty, at most 255 cards"; return -1; }
+    Ring s0; memset(&s0, 0, sizeof s0);
+    for (int i = 0; i < l1; i++) s0.c[0][i] = (unsigned char)p1[i];
+    for (int i = 0; i < l2; i++) s0.c[1][i] = (unsigned char)p2[i];
+    s0.len[0] = l1; s0.len[1] = l2;
+    Ring s = s0, t = s0;
+    u64 power = 1, lam = 0, r = 0;
+    for (;;) {
+        if (ring_step(s, rule)) { *turns = r + 1; return 0; }
+        r++; lam++;
+        if (ring_same(s, t)) break;
+        if (lam == power) { t = s; power <<= 1; lam = 0; }
+        if (r >= cap) { *turns = r; return 2; }
+    }
+    Ring x = s0, y = s0;
+    for (u64 i = 0; i < lam; i++) ring_step(y, rule);
+    u64 mu = 0;
+    while (!ring_same(x, y)) { ring_step(x, rule); ring_step(y, rule); mu++; }
+    *turns = r; *lam_out = lam; *mu_out = mu;
+    return 1;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Are the distinct cycles entered by ordinary deals? (cpu/reach_cycles.c:2-11,32-69,73-107)
+// Each distinct cycle (up to swapping the players) is taken once, at its canonical position of
+// minimum key (as in distinct_cycles.c); then the positions that lead into it are searched
+// backwards for one of deal size (ceil(n/2) and floor(n/2) cards, either way round). A position
+// has at most two predecessors: undo a win of player w, whose bottom two cards must be (winner,
+// loser) with winner > loser (rule 0). Off the loop the predecessors form a finite tree, searched
+// depth first. More than cap positions visited for one cycle: undetermined.
+static inline bool undo_win(const St<4>& s, int w, St<4>& out) {
+    const int L = w ? s.lb : s.la;
+    if (L < 2) return false;
+    const u64 word = w ? s.b : s.a;
+    const u64 cw = (word >> (4 * (L - 2))) & 15ULL, cl = (word >> (4 * (L - 1))) & 15ULL;
+    if (cw <= cl) return false;
+    const u64 rest = word & ((1ULL << (4 * (L - 2))) - 1ULL);
+    out = s;
+    if (w == 0) { out.a = (rest << 4) | cw; out.la = L - 1; out.b = (s.b << 4) | cl; out.lb = s.lb + 1; }
+    else        { out.b = (rest << 4) | cw; out.lb = L - 1; out.a = (s.a << 4) | cl; out.la = s.la + 1; }
+    return true;
+}
+static inline bool deal_sized(const St<4>& s, int n) {
+    const int hi = (n + 1) / 2, lo = n / 2;
+    return (s.la == hi && s.lb == lo) || (s.la == lo && s.lb == hi);
+}
+// 1 = some deal-size position leads into the cycle through s0, 0 = none, -1 = capped
+static int reached_by_deal(const St<4>& s0, u64 lam, u64 cap, int n) {
+    St<4> prev = s0;                              // the loop predecessor of t
+    for (u64 i = 0; i + 1 < lam; i++) step<4>(prev, 0);
+    St<4> t = s0;
+    std::vector<St<4>> todo;
+    u64 visited = 0;
+    for (u64 i = 0; i < lam; i++) {
+        if (deal_sized(t, n)) return 1;
+        for (int w = 0; w < 2; w++) {
+            St<4> q;
+            if (!undo_win(t, w, q) || same<4>(q, prev)) continue;
+            todo.clear();
+            todo.push_back(q);
+            while (!todo.empty()) {
+                St<4> u = todo.back(); todo.pop_back();
+                if (deal_sized(u, n)) return 1;
+                if (++visited > cap) return -1;
+                for (int w2 = 0; w2 < 2; w2++) { St<4> r; if (undo_win(u, w2, r)) todo.push_back(r); }
+            }
+        }
+        prev = t;
+        step<4>(t, 0);
+    }
+    return 0;
+}
+extern "C" int orc_reach(int n, int threads, u64 cap, u64* classes, u64* yes, u64* no, u64* und, u64* nolen) {
+    if (n < 2 || n > 15) { g_err = "reach needs 2 <= n <= 15"; return 1; }
+    if (threads > 0) omp_set_num_threads(threads);
+    const int m = n - 1;
+    const u64 F = fact_u64(m);
+    u64 C = 0, Y = 0, N = 0, U = 0;
+    std::vector<u64> NL(4096, 0);
+    #pragma omp parallel
+    {
+        std::vector<u64> nl(4096, 0);
+        #pragma omp for schedule(dynamic, 64) reduction(+ : C, Y, N, U)
+        for (long long r = 0; r < (long long)F; r++) {
+            int p[16];
+            unrank((u64)r, m, p);
+            for (int k = 0; k < m; k++) {
+                St<4> s = canon_pos<4>(p, n, k);
+                u64 rounds = 0, lam = 0;
+                if (!brent<4>(s, 0, rounds, lam, nullptr)) continue;
+                St<4> v = s;
+                for (u64 t = 0; t < lam; t++) step<4>(v, 0);
+                if (!same<4>(v, s)) continue;                     // not on its own cycle
+                u64 K = ~0ULL;
+                v = s;
+                for (u64 t = 0; t < lam; t++) {
+                    if (v.la > 0 && (int)(v.a & 15ULL) == n - 1) { u64 kk = key4(v); if (kk < K) K = kk; }
+                    if (v.lb > 0 && (int)(v.b & 15ULL) == n - 1) { u64 kk = key4(swapst(v)); if (kk < K) K = kk; }
+                    step<4>(v, 0);
+                }
+                if (key4(s) != K) continue;                       // not the class's representative
+                C++;
+                const int res = reached_by_deal(s, lam, cap, n);
+                if (res == 1) Y++;
+                else if (res == 0) { N++; nl[lam < 4095 ? lam : 4095]++; }
+                else U++;
+            }
+        }
+        #pragma omp critical
+        for (int i = 0; i < 4096; i++) NL[(size_t)i] += nl[(size_t)i];
+    }
+    *classes = C; *yes = Y; *no = N; *und = U;
+    if (nolen) for (int i = 0; i < 4096; i++) nolen[i] = NL[(size_t)i];
+    return 0;
+}
+
 // Lexicographic unrank, exposed for the permutation tests.
 extern "C" void orc_unrank(u64 r, int m, int* p) { unrank(r, m, p); }
 

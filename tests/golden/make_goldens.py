@@ -186,6 +186,29 @@ def main():
         if re.match(r"n=\d+ det=(floyd|streak) range=\[\d+,\d+\) deals_simulated=", ln):
             g["cli24"].append({"line": ln, "source": "data/long_runs.txt:%d" % (i + 1)})
 
+    # cycle_length doubled j k (data/checks.txt, "Cycle lengths of every family member with n <= 255")
+    g["cycle_length"] = []
+    i0 = next(i for i, ln in enumerate(chk) if "./cycle_length doubled $j $k" in ln)
+    cur = None
+    for i in range(i0 + 1, len(chk)):
+        ln = chk[i]
+        if not ln.strip() or ln.startswith("=="):
+            break
+        m = re.match(r"doubled family j=(\d+) k=(\d+)$", ln)
+        if m:
+            cur = {"args": ["doubled", m.group(1), m.group(2)], "lines": [ln], "source": "data/checks.txt:%d" % (i + 1)}
+            g["cycle_length"].append(cur)
+        else:
+            cur["lines"].append(ln)
+
+    # reach_cycles n threads (data/checks.txt, "Cycles entered by an ordinary deal")
+    g["reach"] = {}
+    for i, ln in enumerate(chk):
+        m = re.match(r"n=(\d+): distinct cycles up to swap (\d+) \| reached by a deal (\d+) \| NOT reached (\d+) \| "
+                     r"undetermined \(cap (\d+)\) (\d+)$", ln)
+        if m:
+            g["reach"][m.group(1)] = {"line": ln, "source": "data/checks.txt:%d" % (i + 1)}
+
     with open(OUT, "w") as f:
         json.dump(g, f, indent=1, sort_keys=True)
     print("wrote", OUT)

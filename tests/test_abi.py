@@ -28,9 +28,14 @@ def test_no_python_fallback(monkeypatch, tmp_path):
 
 
 STRUCTS = {
-    "war_tuning": (_lib.WarTuning, ["waves_per_cu", "ranks_per_refill", "brent_cap", "plain_per", "stage_kind"]),
+    "war_tuning": (_lib.WarTuning, ["waves_per_cu", "ranks_per_refill", "brent_cap", "plain_per", "stage_kind",
+                                    "pile_layout"]),
     "war_request": (_lib.WarRequest, ["n", "mode", "rule", "kernel", "lo", "hi", "hbins", "lbins", "dbins",
-                                      "distinct", "shape_blk", "tuning"]),
+                                      "distinct", "shape_blk", "detector", "tuning"]),
+    "war_reach_result": (_lib.WarReachResult, ["classes", "reached", "not_reached", "undetermined", "nolen",
+                                               "nbins", "device_ms"]),
+    "war_pile_position": (_lib.WarPilePosition, ["n1", "n2", "p1", "p2"]),
+    "war_cycle_info": (_lib.WarCycleInfo, ["status", "turns", "lam", "mu"]),
     "war_result": (_lib.WarResult, ["cyc", "done", "maxfin", "turns", "oncycle", "distinct_minkey",
                                     "distinct_hashset", "shape_nonstd", "shape_unchk", "undetermined", "hist",
                                     "hbins", "lenhist", "lbins", "dhist", "dbins", "device_ms", "latency_ms"]),
