@@ -190,7 +190,7 @@ bool held_ok(const war_request &r) {
 war_tuning default_tuning(int n, int mode) {
     war_tuning t{28, 1024, 15, 32, 2, 0};
     if (mode != WAR_MODE_CANON && n >= 12) t.brent_cap = 63;
-    if (mode == WAR_MODE_CANON) t.ranks_per_refill = 4096;
+    if (mode == WAR_MODE_CANON) t.ranks_per_refill = 2048;
     return t;
 }
 
