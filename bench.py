@@ -201,6 +201,16 @@ def main():
         out["canon14"] = canon
         log("canon14: %.3e pos/s persistent (whole), ratio %.2f on samples" % (tot14 / cw, tp / tq))
 
+        # ---- the alternation-checkpoint detector (odd n; exact, fewer turns) on the same n = 13 run
+        ra = eng.run_range(13, "full", hbins=256, detector="alt")
+        check_n13(ra)
+        rb = eng.run_range(13, "full", hbins=256)
+        out["alt_detector"] = {"workload": "n13_full", "value": n13 / (ra["device_ms"] / 1e3), "unit": "deals/s",
+                               "device_ms": ra["device_ms"], "brent_device_ms": rb["device_ms"],
+                               "turns_per_deal": ra["turns"] / n13, "brent_turns_per_deal": rb["turns"] / n13,
+                               "note": "same exact counts; fewer turns, more per-lane work per turn"}
+        log("n13 alt detector: %.1f ms (Brent %.1f ms)" % (ra["device_ms"], rb["device_ms"]))
+
         # ---- n = 17, the published rank range [0, 8992587776): held-card layout against 5-bit piles
         lo17, hi17 = 0, 8992587776
         n17 = {"workload": "n17_range: deal ranks [0, 8992587776) of 17!", "unit": "deals/s",
@@ -212,6 +222,9 @@ def main():
             n17["layout%d_value" % lay] = (hi17 - lo17) / (r["device_ms"] / 1e3)
             n17["layout%d_device_ms" % lay] = r["device_ms"]
         n17["held_over_5bit"] = n17["layout6_value"] / n17["layout5_value"]
+        r = eng.run_range(17, "full", lo17, hi17, detector="alt")
+        assert (r["cyc"], r["maxfin"], r["done"]) == (8016845372, 94, hi17 - lo17), r
+        n17["layout6_alt_value"] = (hi17 - lo17) / (r["device_ms"] / 1e3)
         out["n17_range"] = n17
         log("n17 range: held layout %.3e deals/s, 5-bit %.3e" % (n17["layout6_value"], n17["layout5_value"]))
 

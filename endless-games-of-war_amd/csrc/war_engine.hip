@@ -101,18 +101,20 @@ struct QueueBufs {
     void *ca = nullptr, *cb = nullptr;
     u32 *cmeta = nullptr, *creq = nullptr, *clam = nullptr, *ccnt = nullptr;
     u64 *cpos = nullptr;
+    void *qta[2] = {nullptr, nullptr}, *qtb[2] = {nullptr, nullptr};   // alternation detector: tortoise
+    u32 *qmeta[2] = {nullptr, nullptr};
     u64 cap = 0, tmax = 0;
-    bool have_c = false;
+    bool have_c = false, have_t = false;
 };
 
 constexpr u64 MIN_TILE = 64, MAX_TILE = 8192;
 constexpr u32 PLAIN_CYC_TILE = 1024;    // home-tile size of the plain kernel's cycle records
 
 struct GraphKey {
-    int W, rule, kernel, wpc, per, cap0, walk, multi, hb, lb, kind;
+    int W, rule, kernel, wpc, per, cap0, walk, multi, hb, lb, kind, alt;
     bool operator<(const GraphKey &o) const {
-        return std::tie(W, rule, kernel, wpc, per, cap0, walk, multi, hb, lb, kind) <
-               std::tie(o.W, o.rule, o.kernel, o.wpc, o.per, o.cap0, o.walk, o.multi, o.hb, o.lb, o.kind);
+        return std::tie(W, rule, kernel, wpc, per, cap0, walk, multi, hb, lb, kind, alt) <
+               std::tie(o.W, o.rule, o.kernel, o.wpc, o.per, o.cap0, o.walk, o.multi, o.hb, o.lb, o.kind, o.alt);
     }
 };
 
@@ -167,7 +169,10 @@ int check_request(const war_request *r) {
         return fail(9, "distinct / dbins need canon mode and n <= 15");
     if (r->shape_blk && r->mode != WAR_MODE_CANON) return fail(10, "shape needs canon mode");
     if (r->shape_blk < 0 || r->shape_blk > 512) return fail(10, "shape_blk must be in [0, 512]");
-    if (r->detector != WAR_DET_BRENT && r->detector != WAR_DET_STREAK) return fail(12, "unknown detector");
+    if (r->detector != WAR_DET_BRENT && r->detector != WAR_DET_STREAK && r->detector != WAR_DET_ALT)
+        return fail(12, "unknown detector");
+    if (r->detector == WAR_DET_ALT && ((r->n & 1) == 0 || r->rule != 0))
+        return fail(12, "the alternation detector needs odd n and rule 0 (odd-n cycles alternate: Spivey Thm 6)");
     if (r->detector == WAR_DET_STREAK) {
         if ((r->n & 1) == 0) return fail(12, "the streak test is valid for odd n only (Spivey Thm 6)");
         if (r->rule != 0) return fail(12, "the streak test needs rule 0");
@@ -181,15 +186,16 @@ int check_request(const war_request *r) {
 // Layout 6 applies to n = 17 persistent Brent requests without cycle walks (its records do not
 // keep the players' deal order).
 bool held_ok(const war_request &r) {
-    return r.n == (int)HELD_N && r.kernel == WAR_KERNEL_PERSISTENT && r.detector == WAR_DET_BRENT &&
-           !r.dbins && !r.distinct && !r.shape_blk;
+    return r.n == (int)HELD_N && r.kernel == WAR_KERNEL_PERSISTENT &&
+           (r.detector == WAR_DET_BRENT || r.detector == WAR_DET_ALT) && !r.dbins && !r.distinct && !r.shape_blk;
 }
 
 // Defaults measured on MI355X: deals (long games, ~110 turns at n = 13) want a late first stage
 // boundary; canonical positions (most end within ~30 turns) an early one and larger tiles.
-war_tuning default_tuning(int n, int mode) {
+war_tuning default_tuning(int n, int mode, int det) {
     war_tuning t{28, 1024, 15, 32, 2, 0};
     if (mode != WAR_MODE_CANON && n >= 12) t.brent_cap = 63;
+    if (det == WAR_DET_ALT) t.brent_cap = mode == WAR_MODE_CANON ? 31 : 63;   // stage windows of 32 / 64 turns
     if (mode == WAR_MODE_CANON) t.ranks_per_refill = 2048;
     return t;
 }
@@ -197,7 +203,7 @@ war_tuning default_tuning(int n, int mode) {
 // Precedence: the request's own fields, then the engine-wide override, then war_autotune's choice
 // for this (n, mode, rule), then the default.
 void resolve_tuning(const Engine &E, const war_request &r, war_tuning &t) {
-    t = default_tuning(r.n, r.mode);
+    t = default_tuning(r.n, r.mode, r.detector);
     auto it = E.tuned.find(std::make_tuple(r.n, r.mode, r.rule));
     if (it != E.tuned.end()) {
         t = it->second;
@@ -236,6 +242,8 @@ int layout_of(const war_request &r, const war_tuning &t, int *out) {
     if (L == 6 && !held_ok(r))
         return fail(11, "pile_layout 6 needs n = 17, the persistent kernel, Brent and no dbins / distinct / shape");
     if (L == 5 && (r.distinct || r.dbins)) return fail(11, "distinct / dbins need 4-bit piles (n <= 15)");
+    if (r.detector == WAR_DET_ALT && t.brent_cap < 3)
+        return fail(11, "the alternation detector needs brent_cap >= 3 (stage windows of brent_cap + 1 turns)");
     *out = L;
     return 0;
 }
@@ -320,7 +328,7 @@ void req_free(Engine &E, ReqHost &h) {
 // layout 6 records are 16-byte u64 pairs like layout 4's, so the two share buffers
 template <int W> QueueBufs &qbufs(Engine &E) { return W == 5 ? E.qb5 : E.qb4; }
 
-template <int W> int ensure_queues(Engine &E, bool walk) {
+template <int W> int ensure_queues(Engine &E, bool walk, bool alt = false) {
     typedef typename Lay<W>::T T;
     QueueBufs &Q = qbufs<W>(E);
     if (!Q.cap) {
@@ -338,6 +346,14 @@ template <int W> int ensure_queues(Engine &E, bool walk) {
         }
         Q.cap = cap;
         Q.tmax = tmax;
+    }
+    if (alt && !Q.have_t) {
+        for (int i = 0; i < 2; i++) {
+            HIPCHK(hipMalloc(&Q.qta[i], Q.cap * sizeof(T)));
+            HIPCHK(hipMalloc(&Q.qtb[i], Q.cap * sizeof(T)));
+            HIPCHK(hipMalloc((void **)&Q.qmeta[i], Q.cap * sizeof(u32)));
+        }
+        Q.have_t = true;
     }
     if (walk && !E.reach_stack) {
         // the walk grids have at most CUs x 8 blocks of 256 threads
@@ -376,19 +392,38 @@ std::vector<u32> stage_caps(int cap0) {
     return c;
 }
 
-template <int W, int RULE, bool FRESH>
-void launch_stage_f(int flags, int blocks, size_t lds, hipStream_t st, const StageArgs<W> &A) {
+// Stage boundaries of the alternation detector: multiples of ALT_K, so every lane meets the same
+// checkpoints: w = cap0 + 1, 2w, ..., 8w, then doubling while below 1024, then unbounded.
+std::vector<u32> stage_caps_alt(int cap0) {
+    std::vector<u32> c;
+    const u32 w = (u32)cap0 + 1u;
+    for (u32 i = 1; i <= 8 && (int)c.size() < MAXSTAGE - 1; i++) c.push_back(i * w);
+    u32 a = c.back();
+    while (a < 1024u && (int)c.size() < MAXSTAGE - 1) { a *= 2; c.push_back(a); }
+    c.push_back(UNBOUNDED);
+    return c;
+}
+
+template <int W, int RULE, bool FRESH, int AF>
+void launch_stage_a(int flags, int blocks, size_t lds, hipStream_t st, const StageArgs<W> &A) {
     if constexpr (W == 6) {            // no cycle walks in layout 6
-        if (flags & F_MULTI) hipLaunchKernelGGL((k_stage<W, RULE, FRESH, F_MULTI>), dim3(blocks), dim3(256), lds, st, A);
-        else hipLaunchKernelGGL((k_stage<W, RULE, FRESH, 0>), dim3(blocks), dim3(256), lds, st, A);
+        if (flags & F_MULTI) hipLaunchKernelGGL((k_stage<W, RULE, FRESH, F_MULTI | AF>), dim3(blocks), dim3(256), lds, st, A);
+        else hipLaunchKernelGGL((k_stage<W, RULE, FRESH, AF>), dim3(blocks), dim3(256), lds, st, A);
         return;
     }
-    switch (flags) {
-        case 0: hipLaunchKernelGGL((k_stage<W, RULE, FRESH, 0>), dim3(blocks), dim3(256), lds, st, A); break;
-        case F_MULTI: hipLaunchKernelGGL((k_stage<W, RULE, FRESH, F_MULTI>), dim3(blocks), dim3(256), lds, st, A); break;
-        case F_WALK: hipLaunchKernelGGL((k_stage<W, RULE, FRESH, F_WALK>), dim3(blocks), dim3(256), lds, st, A); break;
-        default: hipLaunchKernelGGL((k_stage<W, RULE, FRESH, F_MULTI | F_WALK>), dim3(blocks), dim3(256), lds, st, A); break;
+    switch (flags & (F_MULTI | F_WALK)) {
+        case 0: hipLaunchKernelGGL((k_stage<W, RULE, FRESH, AF>), dim3(blocks), dim3(256), lds, st, A); break;
+        case F_MULTI: hipLaunchKernelGGL((k_stage<W, RULE, FRESH, F_MULTI | AF>), dim3(blocks), dim3(256), lds, st, A); break;
+        case F_WALK: hipLaunchKernelGGL((k_stage<W, RULE, FRESH, F_WALK | AF>), dim3(blocks), dim3(256), lds, st, A); break;
+        default: hipLaunchKernelGGL((k_stage<W, RULE, FRESH, F_MULTI | F_WALK | AF>), dim3(blocks), dim3(256), lds, st, A); break;
     }
+}
+
+template <int W, int RULE, bool FRESH>
+void launch_stage_f(int flags, int blocks, size_t lds, hipStream_t st, const StageArgs<W> &A) {
+    // the alternation detector exists for rule 0 only
+    if (RULE == 0 && (flags & F_ALT)) launch_stage_a<W, RULE, FRESH, (RULE == 0 ? F_ALT : 0)>(flags, blocks, lds, st, A);
+    else launch_stage_a<W, RULE, FRESH, 0>(flags, blocks, lds, st, A);
 }
 
 template <int W, int RULE>
@@ -417,7 +452,7 @@ int build_graph(Engine &E, const GraphKey &key, hipGraphExec_t *out) {
     QueueBufs &Q = qbufs<W>(E);
     const int blocks = std::max(1, E.cus * key.wpc / 4);
     const size_t lds = (size_t)4 * (key.hb + key.lb) * sizeof(u32);
-    const std::vector<u32> caps = stage_caps(key.cap0);
+    const std::vector<u32> caps = key.alt ? stage_caps_alt(key.cap0) : stage_caps(key.cap0);
     HIPCHK(hipStreamBeginCapture(E.st, hipStreamCaptureModeThreadLocal));
     hipLaunchKernelGGL(k_setup, dim3(256), dim3(256), 0, E.st, E.pipe, (const JobHdr *)E.hdr,
                        key.walk ? Q.ccnt : (u32 *)nullptr);
@@ -430,16 +465,18 @@ int build_graph(Engine &E, const GraphKey &key, hipGraphExec_t *out) {
         A.qin.home = Q.qhome[in]; A.qin.cnt = Q.qcnt[in];
         A.qout.a = (T *)Q.qa[out]; A.qout.b = (T *)Q.qb[out]; A.qout.req = Q.qreq[out]; A.qout.pos = Q.qpos[out];
         A.qout.home = Q.qhome[out]; A.qout.cnt = Q.qcnt[out];
+        A.qin.ta = (T *)Q.qta[in]; A.qin.tb = (T *)Q.qtb[in]; A.qin.meta = Q.qmeta[in];
+        A.qout.ta = (T *)Q.qta[out]; A.qout.tb = (T *)Q.qtb[out]; A.qout.meta = Q.qmeta[out];
         A.cq = cyc_queue<W>(E);
         A.per = (u32)key.per; A.tile_pos = 64u * (u32)key.per;
         A.a0 = a0; A.a1 = caps[s]; A.stage = (int)s;
         A.hb = (u32)key.hb; A.lb = (u32)key.lb;
         A.max_regions = Q.cap / A.tile_pos;
-        const int flags = (key.multi ? F_MULTI : 0) | (key.walk ? F_WALK : 0);
+        const int flags = (key.multi ? F_MULTI : 0) | (key.walk ? F_WALK : 0) | (key.alt ? F_ALT : 0);
         // refill stages need a bounded window (no tortoise move inside); the unbounded last stage
         // and the first (enumerating) stage are lockstep
-        const bool refill = W != 6 && key.kind == 1 && s > 0 && caps[s] != UNBOUNDED;
-        A.in_refill = (W != 6 && key.kind == 1 && s > 1) ? 1u : 0u;
+        const bool refill = W != 6 && !key.alt && key.kind == 1 && s > 0 && caps[s] != UNBOUNDED;
+        A.in_refill = (W != 6 && !key.alt && key.kind == 1 && s > 1) ? 1u : 0u;
         if (refill) launch_refill<W, RULE>(flags, blocks, lds, E.st, A);
         else launch_stage<W, RULE>(s == 0, flags, blocks, lds, E.st, A);
         a0 = caps[s];
@@ -463,16 +500,16 @@ int build_graph(Engine &E, const GraphKey &key, hipGraphExec_t *out) {
 // One batch of jobs (all of layout W and rule RULE) through the persistent pipeline.
 template <int W, int RULE>
 int run_persistent_batch(Engine &E, std::vector<Job> &jobs, const war_tuning &t, int hb, int lb, bool walk,
-                         bool multi, float *ms) {
+                         bool multi, float *ms, bool alt) {
     if (jobs.empty()) return 0;
     if ((int)jobs.size() > MAXJOBS) return fail(31, "too many jobs in one batch");
-    int rc = ensure_queues<W>(E, walk);
+    int rc = ensure_queues<W>(E, walk, alt);
     if (rc) return rc;
     const u64 tile_pos = (u64)t.ranks_per_refill;
     u64 tiles = 0, npos = 0;
     for (Job &j : jobs) { j.tile_base = tiles; tiles += (j.hi - j.lo + tile_pos - 1) / tile_pos; npos += j.hi - j.lo; }
     GraphKey key{W, RULE, 1, t.waves_per_cu, t.ranks_per_refill / 64, t.brent_cap, walk ? 1 : 0, multi ? 1 : 0,
-                 round_pow2(hb), round_pow2(lb), t.stage_kind};
+                 round_pow2(hb), round_pow2(lb), t.stage_kind, alt ? 1 : 0};
     if (tiles > qbufs<W>(E).tmax || tiles * tile_pos > qbufs<W>(E).cap) return fail(32, "batch exceeds the queue capacity");
     JobHdr h{};
     h.njobs = (u32)jobs.size(); h.req0 = jobs[0].req; h.ntiles = tiles;
@@ -528,6 +565,8 @@ int run_plain_batch(Engine &E, ReqHost &h, u64 lo, u64 hi, const war_tuning &t, 
                        walk ? qbufs<W>(E).ccnt : (u32 *)nullptr);
     if (RULE == 0 && h.rq.detector == WAR_DET_STREAK)
         hipLaunchKernelGGL((k_plain<W, 0, 1>), dim3((unsigned)blocks), dim3(256), lds, E.st, P);
+    else if (RULE == 0 && h.rq.detector == WAR_DET_ALT)
+        hipLaunchKernelGGL((k_plain<W, 0, 2>), dim3((unsigned)blocks), dim3(256), lds, E.st, P);
     else
         hipLaunchKernelGGL((k_plain<W, RULE, 0>), dim3((unsigned)blocks), dim3(256), lds, E.st, P);
     HIPCHK(hipGetLastError());
@@ -548,9 +587,9 @@ int run_plain_batch(Engine &E, ReqHost &h, u64 lo, u64 hi, const war_tuning &t, 
 
 template <int W>
 int run_batch_dispatch(Engine &E, int rule, std::vector<Job> &jobs, const war_tuning &t, int hb, int lb, bool walk,
-                       bool multi, float *ms) {
-    return rule ? run_persistent_batch<W, 1>(E, jobs, t, hb, lb, walk, multi, ms)
-                : run_persistent_batch<W, 0>(E, jobs, t, hb, lb, walk, multi, ms);
+                       bool multi, float *ms, bool alt = false) {
+    return rule ? run_persistent_batch<W, 1>(E, jobs, t, hb, lb, walk, multi, ms, false)
+                : run_persistent_batch<W, 0>(E, jobs, t, hb, lb, walk, multi, ms, alt);
 }
 
 int alloc_slot(Engine &E) {
@@ -591,9 +630,10 @@ int run_request_locked(Engine &E, ReqHost &h) {
             std::vector<Job> jobs(1);
             jobs[0].lo = lo; jobs[0].hi = hi; jobs[0].req = (u32)h.slot;
             float ms = 0;
-            if (W == 4)      rc = run_batch_dispatch<4>(E, r.rule, jobs, t, r.hbins, r.lbins, h.dev.walk != 0, false, &ms);
-            else if (W == 6) rc = run_batch_dispatch<6>(E, r.rule, jobs, t, r.hbins, r.lbins, false, false, &ms);
-            else             rc = run_batch_dispatch<5>(E, r.rule, jobs, t, r.hbins, r.lbins, h.dev.walk != 0, false, &ms);
+            const bool alt = r.detector == WAR_DET_ALT;
+            if (W == 4)      rc = run_batch_dispatch<4>(E, r.rule, jobs, t, r.hbins, r.lbins, h.dev.walk != 0, false, &ms, alt);
+            else if (W == 6) rc = run_batch_dispatch<6>(E, r.rule, jobs, t, r.hbins, r.lbins, false, false, &ms, alt);
+            else             rc = run_batch_dispatch<5>(E, r.rule, jobs, t, r.hbins, r.lbins, h.dev.walk != 0, false, &ms, alt);
             dev += ms;
         }
     }
@@ -661,9 +701,13 @@ void serve_loop(Engine *Ep) {
         // one step per (layout, rule) group
         int rc = 0;
         for (int W : {4, 5, 6}) {
-            for (int rule : {0, 1}) {
+            for (int rule : {0, 1})
+            for (int alt : {0, 1}) {
                 std::vector<std::shared_ptr<ReqHost>> grp;
-                for (auto &h : active) if (h->layout == W && h->rq.rule == rule && h->next < h->rq.hi) grp.push_back(h);
+                for (auto &h : active)
+                    if (h->layout == W && h->rq.rule == rule && (h->rq.detector == WAR_DET_ALT) == (alt != 0) &&
+                        h->next < h->rq.hi)
+                        grp.push_back(h);
                 if (grp.empty()) continue;
                 war_tuning gt;
                 resolve_tuning(E, grp[0]->rq, gt);
@@ -682,9 +726,9 @@ void serve_loop(Engine *Ep) {
                     walk = walk || h->dev.walk;
                 }
                 float ms = 0;
-                rc = (W == 4) ? run_batch_dispatch<4>(E, rule, jobs, gt, hb, lb, walk, true, &ms)
-                   : (W == 6) ? run_batch_dispatch<6>(E, rule, jobs, gt, hb, lb, false, true, &ms)
-                              : run_batch_dispatch<5>(E, rule, jobs, gt, hb, lb, walk, true, &ms);
+                rc = (W == 4) ? run_batch_dispatch<4>(E, rule, jobs, gt, hb, lb, walk, true, &ms, alt != 0)
+                   : (W == 6) ? run_batch_dispatch<6>(E, rule, jobs, gt, hb, lb, false, true, &ms, alt != 0)
+                              : run_batch_dispatch<5>(E, rule, jobs, gt, hb, lb, walk, true, &ms, alt != 0);
                 for (auto &h : grp) h->dev_ms += ms;   // device time of the steps the request took part in
                 if (rc) {
                     for (auto &h : grp) { h->status = rc; h->err = t_err; h->next = h->rq.hi; }
@@ -816,12 +860,13 @@ int war_set_tuning(const war_tuning *t) {
     t_err.clear();
     if (!g_eng) return fail(60, "war_init has not been called");
     if (!t) return fail(1, "null tuning");
-    war_tuning probe = default_tuning(13, WAR_MODE_FULL);
+    war_tuning probe = default_tuning(13, WAR_MODE_FULL, WAR_DET_BRENT);
     if (t->waves_per_cu > 0) probe.waves_per_cu = t->waves_per_cu;
     if (t->ranks_per_refill > 0) probe.ranks_per_refill = t->ranks_per_refill;
     if (t->brent_cap > 0) probe.brent_cap = t->brent_cap;
     if (t->plain_per > 0) probe.plain_per = t->plain_per;
     if (t->stage_kind > 0) probe.stage_kind = t->stage_kind;
+    if (t->pile_layout > 0) probe.pile_layout = t->pile_layout;
     int rc = check_tuning(probe);
     if (rc) return rc;
     std::lock_guard<std::mutex> g(g_eng->mu);

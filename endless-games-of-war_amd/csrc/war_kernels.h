@@ -71,6 +71,9 @@ template <int W> struct Queue {
     u64 *pos;
     u32 *home;   // the tile the position was enumerated in (cycle records go to its region)
     u32 *cnt;    // records per tile
+    T *ta, *tb;  // alternation detector: the tortoise (packed like the state)
+    u32 *meta;   // alternation detector: age - tortoise age (bits 0-23), log2(wait / ALT_L0) (24-28),
+                 // player 1 won the last turn (29)
 };
 
 // Layout 6 (n = 17 only, so n is not stored): an alive holder's word has <= 15 cards, so its top
@@ -252,6 +255,23 @@ struct WaveAcc {
         cyc += c; done += c; turns += (u64)c * age; dirty = true;
         if (hb && lane_id() == 0) atomicAdd(&lh[lam < hb - 1 ? lam : hb - 1], c);
     }
+    // Lanes with per-lane values: em ended at age `fin`, cm repeated at age `fin` with cycle length
+    // `lam`. Every lane of the wave calls this (wave reductions, then one LDS add per lane).
+    __device__ void lanes(u64 em, u64 cm, u32 fin, u32 lam) {
+        const u32 lane = lane_id();
+        const bool ie = (em >> lane) & 1ull, ic = (cm >> lane) & 1ull;
+        u32 sum = (ie || ic) ? fin : 0u, mx = ie ? fin : 0u;
+        for (int off = 32; off > 0; off >>= 1) {
+            sum += (u32)__shfl_xor((int)sum, off);
+            const u32 o = (u32)__shfl_xor((int)mx, off);
+            mx = o > mx ? o : mx;
+        }
+        const u32 ne = (u32)__popcll(em), nc = (u32)__popcll(cm);
+        done += ne + nc; cyc += nc; turns += sum; dirty = true;
+        maxfin = (u64)mx > maxfin ? (u64)mx : maxfin;
+        if (lb && ie) atomicAdd(&ll[fin < lb - 1 ? fin : lb - 1], 1u);
+        if (hb && ic) atomicAdd(&lh[lam < hb - 1 ? lam : hb - 1], 1u);
+    }
 };
 
 // Append the lanes of `mask` to the cycle records of their home tiles (one atomic per home tile
@@ -293,8 +313,17 @@ struct LaneCtx {
 };
 
 // FLAGS: bit 0 = the batch holds several requests (records carry req), bit 1 = some request needs
-// cycle walks (records carry pos and home). Compile-time, so the record loads are unconditional.
-enum { F_MULTI = 1, F_WALK = 2 };
+// cycle walks (records carry pos and home), bit 2 = the alternation detector (records carry the
+// tortoise). Compile-time, so the record loads are unconditional.
+enum { F_MULTI = 1, F_WALK = 2, F_ALT = 4 };
+
+// The alternation detector (odd n, rule 0): Brent's tortoise also jumps at ages that are multiples
+// of ALT_K when the last ALT_K turns held a repeated winner (in a cycle of odd-n War the players
+// alternate: Spivey Thm 6, gpu/war24.py:9-17), and after waiting ALT_L0 turns (then twice as long,
+// ...). Before the first turn, player 1 counts as the last winner. Repeats are only declared on
+// exact state matches, and the first match after a jump is the cycle length, so every output is
+// exact; games are seen to repeat in about 30 % fewer turns.
+constexpr u32 ALT_K = 4, ALT_L0 = 128;
 
 template <int W, int RULE, bool FRESH, int FLAGS>
 __device__ __forceinline__ u64 play(const StageArgs<W> &A, WaveAcc &acc, St<W> &s, u32 C, u32 n, u64 alive,
@@ -364,10 +393,113 @@ __device__ __forceinline__ u64 play(const StageArgs<W> &A, WaveAcc &acc, St<W> &
     return alive;
 }
 
+// Per-lane state of the alternation detector beside the game state.
+template <int W> struct AltLane {
+    St<W> t;         // tortoise
+    u32 R, L;        // tortoise age, wait before the next timed jump
+};
+
+// The lockstep game of one wave for ages (a0, a1] under the alternation detector. a0 and a1 are
+// multiples of ALT_K, so the checkpoints fall at the same ages for every lane. prevm: lanes whose
+// previous turn player 1 won (before the first turn, player 1 counts as the last winner). Cycle
+// lengths differ per lane (age - R); the accounting groups the lanes of an event by (request,
+// cycle length).
+template <int W, int RULE, bool FRESH, int FLAGS>
+__device__ __forceinline__ u64 play_alt(const StageArgs<W> &A, WaveAcc &acc, St<W> &s, AltLane<W> &T_, u64 &prevm,
+                                        u32 C, u32 n, u64 alive, bool uniform_req, const LaneCtx &L) {
+    const ReqDev *reqs = A.reqs;
+    const bool last = (A.a1 == UNBOUNDED);
+    open_sa<W>(s, A.zero);
+    open_sa<W>(T_.t, A.zero);
+    u32 age = A.a0;
+    const u32 a1 = last ? AGE_CAP : A.a1;
+    Zh<W> z = make_zh<W>(A.zero);
+    u32 due = T_.R + T_.L;                 // per lane: the age of the next timed jump
+    u64 streakm = 0;
+    // Finished lanes are recorded as they finish (their age, and which masks they are in) and
+    // accounted once, when the wave leaves the stage: detections are spread over many ages under
+    // this detector, so the accounting works on per-lane values (wave sums, one LDS add per lane)
+    // instead of one scalar update per distinct age.
+    u64 endm = 0, cycm = 0;
+    u32 fin = 0;                           // per lane: the age its game ended or repeated
+    auto settle = [&]() {
+        const u32 lam = fin - T_.R;
+        if (uniform_req) {
+            acc.lanes(endm, cycm, fin, lam);
+        } else {
+            u64 rem = endm | cycm;
+            while (rem) {
+                const u32 r = lanev(L.req, (u32)(__ffsll((long long)rem) - 1));
+                const u64 mr = ballot(L.req == r) & rem;
+                rem &= ~mr;
+                acc.select(r, reqs);
+                acc.lanes(endm & mr, cycm & mr, fin, lam);
+            }
+        }
+        endm = cycm = 0;
+    };
+#define WAR_ALT_TURN()                                                                     \
+    {                                                                                      \
+        ++age;                                                                             \
+        const u64 pm = ballot(turn<W, RULE>(s, C, z));                                     \
+        const u64 em = ballot(ended<W>(s, C)) & alive;                                     \
+        u64 dm = ballot(s.a == T_.t.a) & alive;                                            \
+        streakm |= ~(pm ^ prevm);                                                          \
+        prevm = pm;                                                                        \
+        if (em | dm) {                                                                     \
+            dm &= ~em;                                                                     \
+            if (dm) dm &= ballot(rest_same<W>(s, T_.t));                                   \
+            const u64 ev = em | dm;                                                        \
+            if (((ev >> lane_id()) & 1ull) != 0ull) fin = age;                             \
+            endm |= em; cycm |= dm; alive &= ~ev;                                          \
+            if ((FLAGS & F_WALK) && dm) {                                                  \
+                const u64 wm = ballot(L.walk) & dm;                                        \
+                if (wm) acc.pushed += push_cycles<W>(A.cq, A.tile_pos, wm, s, n, L.req,    \
+                                                     age - T_.R, L.pos, L.home);           \
+            }                                                                              \
+        }                                                                                  \
+    }
+    for (;;) {
+        // one block of ALT_K turns up to the next checkpoint (ages a0, a1 are multiples of ALT_K)
+        static_assert(ALT_K == 4, "the unrolled block plays 4 turns");
+        WAR_ALT_TURN();
+        WAR_ALT_TURN();
+        WAR_ALT_TURN();
+        WAR_ALT_TURN();
+        if (!alive) break;
+        // checkpoint: timed jumps (the wait doubles), then jumps after a repeated winner
+        const u64 ex = ballot(age >= due) & alive;
+        const u64 js = (ex | streakm) & alive;
+        streakm = 0;
+        if (js) {
+            if ((js >> lane_id()) & 1ull) {
+                T_.t = s; T_.R = age;
+                if ((ex >> lane_id()) & 1ull) T_.L <<= 1;
+                due = age + T_.L;
+            }
+        }
+        if (age == a1) break;
+    }
+#undef WAR_ALT_TURN
+    if (endm | cycm) settle();
+    if (alive && last) {
+        u64 rem = alive;
+        while (rem) {
+            const u32 r = lanev(L.req, (u32)(__ffsll((long long)rem) - 1));
+            const u64 mr = ballot(L.req == r) & rem;
+            rem &= ~mr;
+            acc.select(r, reqs);
+            acc.undet += __popcll(mr); acc.done += __popcll(mr); acc.dirty = true;
+        }
+        alive = 0;
+    }
+    return alive;
+}
+
 // Write the survivors (lanes of `alive`) densely at span + out_n.
 template <int W, int FLAGS>
 __device__ __forceinline__ void put_survivors(const StageArgs<W> &A, u64 span, u32 out_n, u64 alive, const St<W> &s,
-                                              u32 n, const LaneCtx &L) {
+                                              u32 n, const LaneCtx &L, const AltLane<W> *T_ = nullptr, u64 prevm = 0) {
     typedef typename Lay<W>::T T;
     if ((alive >> lane_id()) & 1ull) {
         const u64 i = span + out_n + rank_in(alive);
@@ -376,6 +508,13 @@ __device__ __forceinline__ void put_survivors(const StageArgs<W> &A, u64 span, u
         A.qout.a[i] = qa; A.qout.b[i] = qb;
         if (FLAGS & F_MULTI) A.qout.req[i] = L.req;
         if (FLAGS & F_WALK) { A.qout.pos[i] = L.pos; A.qout.home[i] = L.home; }
+        if constexpr ((FLAGS & F_ALT) != 0) {
+            T ta, tb;
+            pack<W>(T_->t, n, ta, tb);
+            A.qout.ta[i] = ta; A.qout.tb[i] = tb;
+            const u32 e = (u32)(__builtin_ctz(T_->L) - __builtin_ctz(ALT_L0));
+            A.qout.meta[i] = (A.a1 - T_->R) | (e << 24) | ((u32)((prevm >> lane_id()) & 1ull) << 29);
+        }
     }
 }
 
@@ -451,10 +590,21 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs<W> A) {
                 const u64 e0 = ballot(act && (n < 2 || ended<W>(s, C))) & act_m;
                 if (e0) acc.ended(__popcll(e0), 0);
                 u64 alive = act_m & ~e0;
-                if (alive) alive = play<W, RULE, FRESH, FLAGS>(A, acc, s, C, n, alive, true, L);
-                if (alive) {
-                    put_survivors<W, FLAGS>(A, span, out_n, alive, s, n, L);
-                    out_n += __popcll(alive);
+                if constexpr ((FLAGS & F_ALT) != 0) {
+                    AltLane<W> al;
+                    al.t = s; al.R = 0; al.L = ALT_L0;
+                    u64 prevm = ~0ull;
+                    if (alive) alive = play_alt<W, RULE, FRESH, FLAGS>(A, acc, s, al, prevm, C, n, alive, true, L);
+                    if (alive) {
+                        put_survivors<W, FLAGS>(A, span, out_n, alive, s, n, L, &al, prevm);
+                        out_n += __popcll(alive);
+                    }
+                } else {
+                    if (alive) alive = play<W, RULE, FRESH, FLAGS>(A, acc, s, C, n, alive, true, L);
+                    if (alive) {
+                        put_survivors<W, FLAGS>(A, span, out_n, alive, s, n, L);
+                        out_n += __popcll(alive);
+                    }
                 }
                 if (j + 1 < cnt) {
                     if (mode == 2) { if (++k == (int)n - 1) { k = 0; q = next_perm<PW>(q, (int)n - 1); } }
@@ -471,8 +621,8 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs<W> A) {
             const u32 excl = incl - c;
             const u32 total = lanev(incl, 63);
             // records of the next batch are loaded one batch ahead (their latency hides behind play)
-            T na = 0, nb = 0;
-            u32 nreq = req0, nhome = 0;
+            T na = 0, nb = 0, nta = 0, ntb = 0;
+            u32 nreq = req0, nhome = 0, nmeta = 0;
             u64 npos = 0;
             // called with every lane active: group_locate reads other lanes (ds_bpermute), and a
             // read from an inactive lane does not return its value
@@ -484,6 +634,7 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs<W> A) {
                     na = A.qin.a[i]; nb = A.qin.b[i];
                     if constexpr ((FLAGS & F_MULTI) != 0) nreq = A.qin.req[i];
                     if constexpr ((FLAGS & F_WALK) != 0) { npos = A.qin.pos[i]; nhome = A.qin.home[i]; }
+                    if constexpr ((FLAGS & F_ALT) != 0) { nta = A.qin.ta[i]; ntb = A.qin.tb[i]; nmeta = A.qin.meta[i]; }
                 }
             };
             fetch(lane);
@@ -494,9 +645,19 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs<W> A) {
                 u32 n = 2;
                 LaneCtx L;
                 L.req = req0; L.pos = 0; L.home = 0; L.walk = false;
+                AltLane<W> al;
+                al.t = St<W>{}; al.R = 0; al.L = ALT_L0;
+                u32 prevbit = 0;
                 if (act) {
                     unpack<W>(na, nb, s, n);
                     L.req = nreq; L.pos = npos; L.home = nhome;
+                    if constexpr ((FLAGS & F_ALT) != 0) {
+                        u32 tn;
+                        unpack<W>(nta, ntb, al.t, tn);
+                        al.R = A.a0 - (nmeta & 0xFFFFFFu);
+                        al.L = ALT_L0 << ((nmeta >> 24) & 31u);
+                        prevbit = (nmeta >> 29) & 1u;
+                    }
                 }
                 if (b + 64 < total) fetch(b + 64 + lane);
                 const u32 C = c_of<W>(n);
@@ -510,10 +671,19 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs<W> A) {
                     acc.select(req0, reqs);
                 }
                 if constexpr ((FLAGS & F_WALK) != 0) L.walk = act && reqs[L.req].walk != 0;
-                alive = play<W, RULE, FRESH, FLAGS>(A, acc, s, C, n, alive, uniform_req, L);
-                if (alive) {
-                    put_survivors<W, FLAGS>(A, span, out_n, alive, s, n, L);
-                    out_n += __popcll(alive);
+                if constexpr ((FLAGS & F_ALT) != 0) {
+                    u64 prevm = ballot(prevbit != 0u) & alive;
+                    alive = play_alt<W, RULE, FRESH, FLAGS>(A, acc, s, al, prevm, C, n, alive, uniform_req, L);
+                    if (alive) {
+                        put_survivors<W, FLAGS>(A, span, out_n, alive, s, n, L, &al, prevm);
+                        out_n += __popcll(alive);
+                    }
+                } else {
+                    alive = play<W, RULE, FRESH, FLAGS>(A, acc, s, C, n, alive, uniform_req, L);
+                    if (alive) {
+                        put_survivors<W, FLAGS>(A, span, out_n, alive, s, n, L);
+                        out_n += __popcll(alive);
+                    }
                 }
             }
             if (!last && lane < ng) {
@@ -787,7 +957,8 @@ constexpr u32 STREAK_CAP = 1u << 22;
 __device__ __forceinline__ u32 gcd_u(u32 a, u32 b) { while (b) { const u32 t = a % b; a = b; b = t; } return a; }
 __device__ __forceinline__ u32 lcm_u(u32 a, u32 b) { return a / gcd_u(a, b) * b; }
 
-// DET: 0 = Brent, 1 = the streak test (RULE 0 only; no cycle lengths, so no histogram or walks).
+// DET: 0 = Brent, 1 = the streak test (RULE 0 only; no cycle lengths, so no histogram or walks),
+// 2 = the alternation detector (RULE 0 only; every output).
 template <int W, int RULE, int DET>
 __global__ __launch_bounds__(256) void k_plain(PlainArgs<W> P) {
     typedef typename Lay<W>::T T;
@@ -824,6 +995,24 @@ __global__ __launch_bounds__(256) void k_plain(PlainArgs<W> P) {
                     l++;
                     if (same<W>(s, t)) { cy = true; lam = l; break; }
                     if (l == power) { t = s; power <<= 1; l = 0; }
+                    if (rounds >= AGE_CAP) { und = true; break; }
+                }
+            } else if constexpr (DET == 2) {
+                // the alternation detector, one lane at a time (same schedule as the persistent form)
+                u32 R = 0, Lw = ALT_L0, prevw = 1u;
+                bool streak = false;
+                for (;;) {
+                    const u32 w = turn<W, RULE>(s, C, z) ? 1u : 0u;
+                    rounds++;
+                    if (ended<W>(s, C)) break;
+                    if (same<W>(s, t)) { cy = true; lam = (u32)rounds - R; break; }
+                    streak = streak || w == prevw;
+                    prevw = w;
+                    if ((rounds & (ALT_K - 1u)) == 0) {
+                        if ((u32)rounds - R >= Lw) { t = s; R = (u32)rounds; Lw <<= 1; }
+                        else if (streak) { t = s; R = (u32)rounds; }
+                        streak = false;
+                    }
                     if (rounds >= AGE_CAP) { und = true; break; }
                 }
             } else {

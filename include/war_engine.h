@@ -46,6 +46,11 @@ extern "C" {
 #define WAR_DET_STREAK 1   /* the odd-n streak test of gpu/war24.py:98-125 (Spivey Thm 6):
                               odd n, rule 0, plain kernel; counts, game lengths and turns, but
                               no cycle lengths (hbins, dbins, distinct, shape must be 0)        */
+#define WAR_DET_ALT    2   /* Brent with alternation checkpoints (this engine's; odd n, rule 0,
+                              either kernel): the tortoise also jumps, at ages that are multiples
+                              of 4, after a repeated winner (odd-n cycles alternate, Spivey
+                              Thm 6), and after waiting 128, 256, ... turns. Every output is
+                              exact and equals Brent's; only `turns` (its own) is smaller.     */
 
 typedef struct war_tuning {
     int32_t waves_per_cu;      /* persistent grid = CUs x waves_per_cu wave64s (1..32)          */

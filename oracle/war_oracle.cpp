@@ -17,6 +17,9 @@
 //   loop shape (Theorem-3 test) . gpu/war_gpu.py:51-73, :238-242
 //   5-bit piles (n >= 17) ....... gpu/war24.py:31-47 (only the layout; the game is the same)
 //   streak test (odd n) ......... gpu/war24.py:98-125
+//   cycle length / pre-period ... cpu/cycle_length.c:16-30,114-126
+//   reach by a deal ............. cpu/reach_cycles.c:32-69,73-107
+// and one detector of this engine's own (alt_brent, odd n), pinned against Brent.
 //
 // Extensions the reference does not have (pinned by invariants in the tests, not by goldens):
 // arbitrary rank / position ranges, the histogram of ending-game lengths (lenhist), the total
@@ -153,6 +156,36 @@ template <int W> static inline int streak(St<W> s, int n, u64& steps) {
     }
 }
 
+// Brent with alternation checkpoints (this engine's detector for odd n, rule 0; not in the
+// reference). In a cycle of odd-n War the players alternate winning (Spivey Thm 6, the fact the
+// reference's streak test rests on, gpu/war24.py:9-17). Every ALT_K turns (ages multiple of
+// ALT_K), a game whose last ALT_K turns held a repeated winner cannot have been on its cycle for
+// those turns, so its tortoise jumps to the present state; a tortoise that has waited ALT_L0
+// turns (then twice as long, and so on) jumps too, as in Brent. A repeat is only ever declared
+// on an exact state match, and the first match after a jump is the cycle length, so every output
+// is exact; the turns to the repeat are fewer than Brent's (about 30 % at n = 13).
+constexpr u64 ALT_K = 4, ALT_L0 = 128;
+template <int W> static inline bool alt_brent(St<W> s, u64& rounds, u64& lam_out, St<W>* meet) {
+    if (s.la == 0 || s.lb == 0) { rounds = 0; return false; }
+    St<W> t = s;
+    u64 R = 0, age = 0, L = ALT_L0;
+    bool streak = false;
+    int prevw = 1;          // before the first turn, player 1 counts as the last winner
+    for (;;) {
+        const int w = step<W>(s, 0);
+        age++;
+        if (s.la == 0 || s.lb == 0) { rounds = age; return false; }
+        if (same<W>(s, t)) { rounds = age; lam_out = age - R; if (meet) *meet = s; return true; }
+        if (w == prevw) streak = true;
+        prevw = w;
+        if (age % ALT_K == 0) {
+            if (age - R >= L) { t = s; R = age; L *= 2; }
+            else if (streak) { t = s; R = age; }
+            streak = false;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------------------------
 // Permutations (lexicographic, p[0] most significant), war_gpu.py:135-146 and :109-119.
 static u64 fact_u64(int k) { u64 f = 1; for (int i = 2; i <= k; i++) f *= (u64)i; return f; }
@@ -231,8 +264,22 @@ struct Acc {
 
 struct Cfg { int n, mode, rule, hbins, lbins, dbins, distinct, blk, det; };
 
+template <int W> static void play_cycle(const St<W>& s, const Cfg& c, Acc& A, u64 lam, const St<W>& meet);
+
 template <int W> static void play(const St<W>& s, const Cfg& c, Acc& A) {
     u64 rounds = 0, lam = 0; St<W> meet;
+    if (c.det == 2) {      // alternation checkpoints: every output, its own turn count
+        const bool cy = alt_brent<W>(s, rounds, lam, &meet);
+        A.done++;
+        A.turns += rounds;
+        if (!cy) {
+            if (rounds > A.maxfin) A.maxfin = rounds;
+            if (c.lbins) A.lenhist[rounds < (u64)(c.lbins - 1) ? rounds : (u64)(c.lbins - 1)]++;
+            return;
+        }
+        play_cycle<W>(s, c, A, lam, meet);
+        return;
+    }
     if (c.det == 1) {      // streak detector: counts, game lengths and turns; no cycle lengths
         int r = streak<W>(s, c.n, rounds);
         A.done++;
@@ -253,6 +300,11 @@ template <int W> static void play(const St<W>& s, const Cfg& c, Acc& A) {
         if (c.lbins) A.lenhist[rounds < (u64)(c.lbins - 1) ? rounds : (u64)(c.lbins - 1)]++;
         return;
     }
+    play_cycle<W>(s, c, A, lam, meet);
+}
+
+// An endless position s with cycle length lam, seen at state meet (on the cycle).
+template <int W> static void play_cycle(const St<W>& s, const Cfg& c, Acc& A, u64 lam, const St<W>& meet) {
     A.cyc++;
     if (c.hbins) A.hist[lam < (u64)(c.hbins - 1) ? lam : (u64)(c.hbins - 1)]++;
     if (c.mode == ORC_CANON && (c.dbins || c.distinct)) {
@@ -323,7 +375,8 @@ extern "C" u64 orc_total(int n, int mode) { return range_total(n, mode); }
 
 // Play every position in [lo, hi) (deal ranks, or canon positions). 0 on success.
 // det: 0 = Brent (every output), 1 = the streak test (odd n, rule 0; no cycle lengths, so no
-// hist / dhist / distinct / shape).
+// hist / dhist / distinct / shape), 2 = Brent with alternation checkpoints (odd n, rule 0; every
+// output, its own turn count).
 extern "C" int orc_run_det(int n, int mode, int rule, u64 lo, u64 hi, int threads,
                            int distinct, int shape_blk, int det, orc_res* res) {
     if (!res) { g_err = "null result"; return 1; }
@@ -335,7 +388,10 @@ extern "C" int orc_run_det(int n, int mode, int rule, u64 lo, u64 hi, int thread
     if (mode == ORC_HALF && (n & 1)) { g_err = "half mode needs even n"; return 6; }
     if ((distinct || res->dbins) && (mode != ORC_CANON || n > 15)) { g_err = "distinct needs canon mode and n <= 15"; return 7; }
     if (shape_blk && mode != ORC_CANON) { g_err = "shape needs canon mode"; return 8; }
-    if (det != 0 && det != 1) { g_err = "det must be 0 (Brent) or 1 (streak)"; return 9; }
+    if (det < 0 || det > 2) { g_err = "det must be 0 (Brent), 1 (streak) or 2 (alternation checkpoints)"; return 9; }
+    if (det == 2 && ((n % 2) == 0 || rule != 0)) {
+        g_err = "alternation checkpoints need odd n and rule 0"; return 10;
+    }
     if (det == 1 && ((n % 2) == 0 || rule != 0 || distinct || shape_blk || (res->hist && res->hbins) ||
                      (res->dhist && res->dbins))) {
         g_err = "the streak test needs odd n, rule 0 and no cycle-length outputs"; return 10;
@@ -557,7 +613,7 @@ extern "C" void orc_unrank(u64 r, int m, int* p) { unrank(r, m, p); }
 static void usage() {
     fprintf(stderr, "usage: war_oracle n [--mode full|half|canon] [--rule 0|1] [--lo L] [--hi H]\n"
                     "                  [--hist B] [--lbins B] [--dbins B] [--distinct] [--shape] [--threads T]\n"
-                    "                  [--det brent|streak]\n");
+                    "                  [--det brent|streak|alt]\n");
 }
 int main(int argc, char** argv) {
     if (argc < 2) { usage(); return 2; }
@@ -577,7 +633,7 @@ int main(int argc, char** argv) {
         else if (a == "--distinct") distinct = 1;
         else if (a == "--shape") shape = 1;
         else if (a == "--threads") threads = atoi(nxt().c_str());
-        else if (a == "--det") det = nxt() == "streak" ? 1 : 0;
+        else if (a == "--det") { std::string d = nxt(); det = d == "streak" ? 1 : d == "alt" ? 2 : 0; }
         else { usage(); return 2; }
     }
     u64 total = range_total(n, mode);

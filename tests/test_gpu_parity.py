@@ -209,3 +209,69 @@ def test_layout_refusals(eng):
         eng.run_range(17, lo=0, hi=100, tuning={"pile_layout": 4})
     with pytest.raises(WarError, match="4-bit"):
         eng.run_range(11, "canon", lo=0, hi=100, distinct=True, tuning={"pile_layout": 5})
+
+
+# ---- the alternation detector (odd n, rule 0): every output equal to the oracle's, turns its own
+ALT_FIELDS = ("cyc", "done", "maxfin", "turns", "hist", "lenhist", "undetermined")
+
+
+def check_alt(eng, orc, n, mode, lo, hi, kernel="persistent", tuning=None, **kw):
+    want = orc.run(n, mode, lo, hi, hbins=256, lbins=512, detector="alt", **kw)
+    got = eng.run_range(n, mode, lo, hi, kernel=kernel, hbins=256, lbins=512, detector="alt", tuning=tuning, **kw)
+    walk_fields = ("oncycle", "distinct_minkey", "dhist", "shape_nonstd", "shape_unchk") if kw else ()
+    for k in ALT_FIELDS + walk_fields:
+        assert got[k] == want[k], (n, mode, lo, hi, kernel, tuning, k, got[k], want[k])
+    return got, want
+
+
+@pytest.mark.parametrize("kernel", ["plain", "persistent"])
+@pytest.mark.parametrize("n,mode", [(3, "full"), (5, "full"), (7, "full"), (9, "full"), (11, "full"),
+                                    (5, "canon"), (7, "canon"), (9, "canon"), (11, "canon")])
+def test_alt_whole_range(eng, orc, n, mode, kernel):
+    print("case alt", n, mode, kernel, flush=True)
+    got, _ = check_alt(eng, orc, n, mode, 0, orc.total(n, mode), kernel=kernel)
+    brent = eng.run_range(n, mode, hbins=256, lbins=512)
+    assert (got["cyc"], got["maxfin"], got["hist"], got["lenhist"]) == \
+           (brent["cyc"], brent["maxfin"], brent["hist"], brent["lenhist"])
+
+
+def test_alt_sampled_and_layouts(eng, orc):
+    rng = random.Random(2)
+    for n, mode in [(13, "full"), (15, "full"), (13, "canon"), (15, "canon"), (17, "full"), (17, "canon"),
+                    (19, "full")]:
+        total = orc.total(n, mode)
+        for ln in (1, 65, 200_003):
+            lo = rng.randrange(0, total - ln)
+            for kernel in ("plain", "persistent"):
+                print("case alt", n, mode, lo, ln, kernel, flush=True)
+                check_alt(eng, orc, n, mode, lo, lo + ln, kernel=kernel)
+    for lay in (5, 6):
+        check_alt(eng, orc, 17, "full", 10**14, 10**14 + 300_000, tuning={"pile_layout": lay})
+
+
+@pytest.mark.parametrize("n", [9, 11])
+def test_alt_walks(eng, orc, n):
+    print("case alt walks", n, flush=True)
+    check_alt(eng, orc, n, "canon", 0, orc.total(n, "canon"), dbins=4096, distinct=True, shape_blk=n & -n)
+
+
+@pytest.mark.parametrize("tuning", [{"brent_cap": 3}, {"brent_cap": 7, "ranks_per_refill": 64},
+                                    {"brent_cap": 63, "waves_per_cu": 4}, {"brent_cap": 255, "ranks_per_refill": 8192}])
+def test_alt_schedules_are_exact(eng, orc, tuning):
+    print("case alt schedule", tuning, flush=True)
+    check_alt(eng, orc, 11, "full", 0, orc.total(11, "full"), tuning=tuning)
+    check_alt(eng, orc, 13, "canon", 3 * 10**9, 3 * 10**9 + 400_000, tuning=tuning, dbins=64, distinct=True,
+              shape_blk=1)
+
+
+def test_alt_n13_goldens_and_refusals(eng, goldens):
+    g = goldens["deals"]["13"]
+    r = eng.run_range(13, "full", hbins=256, lbins=512, detector="alt")
+    assert r["done"] == g["deals"] and r["cyc"] == g["never_end"] and r["maxfin"] == g["longest"]
+    assert {str(i): v for i, v in enumerate(r["hist"]) if v} == g["hist"]
+    print("n13 full alt device_ms", r["device_ms"], flush=True)
+    for kw in ({"n": 14}, {"n": 13, "rule": 1}, {"n": 13, "tuning": {"brent_cap": 1}}):
+        kw = dict(kw)
+        n = kw.pop("n")
+        with pytest.raises(WarError, match="alternation"):
+            eng.run_range(n, lo=0, hi=100, detector="alt", **kw)
