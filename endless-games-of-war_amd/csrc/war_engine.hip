@@ -242,8 +242,9 @@ int layout_of(const war_request &r, const war_tuning &t, int *out) {
     if (L == 6 && !held_ok(r))
         return fail(11, "pile_layout 6 needs n = 17, the persistent kernel, Brent and no dbins / distinct / shape");
     if (L == 5 && (r.distinct || r.dbins)) return fail(11, "distinct / dbins need 4-bit piles (n <= 15)");
-    if (r.detector == WAR_DET_ALT && t.brent_cap < 3)
-        return fail(11, "the alternation detector needs brent_cap >= 3 (stage windows of brent_cap + 1 turns)");
+    if (r.detector == WAR_DET_ALT && (u32)t.brent_cap + 1u < ALT_K)
+        return fail(11, "the alternation detector needs brent_cap >= %u (stage windows of brent_cap + 1 turns, a "
+                        "multiple of its %u-turn checkpoints)", ALT_K - 1u, ALT_K);
     *out = L;
     return 0;
 }

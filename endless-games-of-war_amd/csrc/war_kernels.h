@@ -323,7 +323,7 @@ enum { F_MULTI = 1, F_WALK = 2, F_ALT = 4 };
 // ...). Before the first turn, player 1 counts as the last winner. Repeats are only declared on
 // exact state matches, and the first match after a jump is the cycle length, so every output is
 // exact; games are seen to repeat in about 30 % fewer turns.
-constexpr u32 ALT_K = 4, ALT_L0 = 128;
+constexpr u32 ALT_K = 8, ALT_L0 = 128;
 
 template <int W, int RULE, bool FRESH, int FLAGS>
 __device__ __forceinline__ u64 play(const StageArgs<W> &A, WaveAcc &acc, St<W> &s, u32 C, u32 n, u64 alive,
@@ -461,11 +461,8 @@ __device__ __forceinline__ u64 play_alt(const StageArgs<W> &A, WaveAcc &acc, St<
     }
     for (;;) {
         // one block of ALT_K turns up to the next checkpoint (ages a0, a1 are multiples of ALT_K)
-        static_assert(ALT_K == 4, "the unrolled block plays 4 turns");
-        WAR_ALT_TURN();
-        WAR_ALT_TURN();
-        WAR_ALT_TURN();
-        WAR_ALT_TURN();
+#pragma unroll
+        for (u32 q = 0; q < ALT_K; q++) WAR_ALT_TURN();
         if (!alive) break;
         // checkpoint: timed jumps (the wait doubles), then jumps after a repeated winner
         const u64 ex = ballot(age >= due) & alive;

@@ -48,7 +48,7 @@ extern "C" {
                               no cycle lengths (hbins, dbins, distinct, shape must be 0)        */
 #define WAR_DET_ALT    2   /* Brent with alternation checkpoints (this engine's; odd n, rule 0,
                               either kernel): the tortoise also jumps, at ages that are multiples
-                              of 4, after a repeated winner (odd-n cycles alternate, Spivey
+                              of 8, after a repeated winner (odd-n cycles alternate, Spivey
                               Thm 6), and after waiting 128, 256, ... turns. Every output is
                               exact and equals Brent's; only `turns` (its own) is smaller.     */
 

@@ -164,7 +164,7 @@ template <int W> static inline int streak(St<W> s, int n, u64& steps) {
 // turns (then twice as long, and so on) jumps too, as in Brent. A repeat is only ever declared
 // on an exact state match, and the first match after a jump is the cycle length, so every output
 // is exact; the turns to the repeat are fewer than Brent's (about 30 % at n = 13).
-constexpr u64 ALT_K = 4, ALT_L0 = 128;
+constexpr u64 ALT_K = 8, ALT_L0 = 128;
 template <int W> static inline bool alt_brent(St<W> s, u64& rounds, u64& lam_out, St<W>* meet) {
     if (s.la == 0 || s.lb == 0) { rounds = 0; return false; }
     St<W> t = s;

@@ -255,7 +255,7 @@ def test_alt_walks(eng, orc, n):
     check_alt(eng, orc, n, "canon", 0, orc.total(n, "canon"), dbins=4096, distinct=True, shape_blk=n & -n)
 
 
-@pytest.mark.parametrize("tuning", [{"brent_cap": 3}, {"brent_cap": 7, "ranks_per_refill": 64},
+@pytest.mark.parametrize("tuning", [{"brent_cap": 7}, {"brent_cap": 15, "ranks_per_refill": 64},
                                     {"brent_cap": 63, "waves_per_cu": 4}, {"brent_cap": 255, "ranks_per_refill": 8192}])
 def test_alt_schedules_are_exact(eng, orc, tuning):
     print("case alt schedule", tuning, flush=True)

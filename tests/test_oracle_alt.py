@@ -17,7 +17,7 @@ import pytest
 
 from test_oracle_model import _canon, _deal, model
 
-K, L0 = 4, 128
+K, L0 = 8, 128
 
 
 def alt_model(p1, p2):
