@@ -108,6 +108,8 @@ struct QueueBufs {
 };
 
 constexpr u64 MIN_TILE = 64, MAX_TILE = 8192;
+constexpr int FLOW_MIN_FREE = 1;        // flow kernel: refill at a checkpoint once this many lanes are free
+constexpr u64 FLOW_BATCH = 1ull << 32;  // flow kernel: positions per batch
 constexpr u32 PLAIN_CYC_TILE = 1024;    // home-tile size of the plain kernel's cycle records
 
 struct GraphKey {
@@ -195,7 +197,10 @@ bool held_ok(const war_request &r) {
 war_tuning default_tuning(int n, int mode, int det) {
     war_tuning t{28, 1024, 15, 32, 2, 0};
     if (mode != WAR_MODE_CANON && n >= 12) t.brent_cap = 63;
-    if (det == WAR_DET_ALT) t.brent_cap = mode == WAR_MODE_CANON ? 31 : 63;   // stage windows of 32 / 64 turns
+    if (det == WAR_DET_ALT) {
+        t.brent_cap = mode == WAR_MODE_CANON ? 31 : 63;   // stage windows of 32 / 64 turns (when staged)
+        t.stage_kind = 3;                                  // the flow kernel where it applies
+    }
     if (mode == WAR_MODE_CANON) t.ranks_per_refill = 2048;
     return t;
 }
@@ -226,7 +231,7 @@ int check_tuning(const war_tuning &t) {
     const int c = t.brent_cap;
     if (c < 1 || c > 65535 || ((c + 1) & c)) return fail(11, "brent_cap must be 2^k - 1 in [1, 65535]");
     if (t.plain_per < 1 || t.plain_per > 4096) return fail(11, "plain_per must be in [1, 4096]");
-    if (t.stage_kind != 1 && t.stage_kind != 2) return fail(11, "stage_kind must be 1 (refill) or 2 (lockstep)");
+    if (t.stage_kind < 1 || t.stage_kind > 3) return fail(11, "stage_kind must be 1 (refill), 2 (lockstep) or 3 (flow)");
     if (t.pile_layout != 0 && t.pile_layout != 4 && t.pile_layout != 5 && t.pile_layout != 6)
         return fail(11, "pile_layout must be 0, 4, 5 or 6");
     return 0;
@@ -453,10 +458,21 @@ int build_graph(Engine &E, const GraphKey &key, hipGraphExec_t *out) {
     QueueBufs &Q = qbufs<W>(E);
     const int blocks = std::max(1, E.cus * key.wpc / 4);
     const size_t lds = (size_t)4 * (key.hb + key.lb) * sizeof(u32);
-    const std::vector<u32> caps = key.alt ? stage_caps_alt(key.cap0) : stage_caps(key.cap0);
+    const std::vector<u32> caps = key.kind == 3 ? std::vector<u32>{}
+                                  : key.alt ? stage_caps_alt(key.cap0) : stage_caps(key.cap0);
     HIPCHK(hipStreamBeginCapture(E.st, hipStreamCaptureModeThreadLocal));
     hipLaunchKernelGGL(k_setup, dim3(256), dim3(256), 0, E.st, E.pipe, (const JobHdr *)E.hdr,
                        key.walk ? Q.ccnt : (u32 *)nullptr);
+    if constexpr (RULE == 0) {
+        if (key.kind == 3) {            // the flow kernel: one launch plays the whole batch
+            StageArgs<W> A{};
+            A.pipe = E.pipe; A.hdr = E.hdr; A.jobs = E.jobs; A.reqs = E.reqs;
+            A.per = (u32)key.per; A.tile_pos = 64u * (u32)key.per;
+            A.hb = (u32)key.hb; A.lb = (u32)key.lb;
+            A.flow_min = (u32)key.cap0;
+            hipLaunchKernelGGL((k_flow<W>), dim3(blocks), dim3(256), lds, E.st, A);
+        }
+    }
     u32 a0 = 0;
     for (size_t s = 0; s < caps.size(); s++) {
         StageArgs<W> A{};
@@ -504,14 +520,21 @@ int run_persistent_batch(Engine &E, std::vector<Job> &jobs, const war_tuning &t,
                          bool multi, float *ms, bool alt) {
     if (jobs.empty()) return 0;
     if ((int)jobs.size() > MAXJOBS) return fail(31, "too many jobs in one batch");
-    int rc = ensure_queues<W>(E, walk, alt);
+    // the flow kernel (stage_kind 3) plays single-request batches of the alternation detector
+    // without cycle walks; elsewhere stage_kind 3 runs lockstep stages
+    const bool flow = RULE == 0 && alt && t.stage_kind == 3 && !walk && !multi;
+    int rc = flow ? 0 : ensure_queues<W>(E, walk, alt);
     if (rc) return rc;
     const u64 tile_pos = (u64)t.ranks_per_refill;
     u64 tiles = 0, npos = 0;
     for (Job &j : jobs) { j.tile_base = tiles; tiles += (j.hi - j.lo + tile_pos - 1) / tile_pos; npos += j.hi - j.lo; }
-    GraphKey key{W, RULE, 1, t.waves_per_cu, t.ranks_per_refill / 64, t.brent_cap, walk ? 1 : 0, multi ? 1 : 0,
-                 round_pow2(hb), round_pow2(lb), t.stage_kind, alt ? 1 : 0};
-    if (tiles > qbufs<W>(E).tmax || tiles * tile_pos > qbufs<W>(E).cap) return fail(32, "batch exceeds the queue capacity");
+    const int kind = flow ? 3 : (t.stage_kind == 3 ? 2 : t.stage_kind);
+    // flow: cap0 holds the refill threshold (free lanes of 64), brent_cap's stage windows unused
+    const int cap0 = flow ? FLOW_MIN_FREE : t.brent_cap;
+    GraphKey key{W, RULE, 1, t.waves_per_cu, t.ranks_per_refill / 64, cap0, walk ? 1 : 0, multi ? 1 : 0,
+                 round_pow2(hb), round_pow2(lb), kind, alt ? 1 : 0};
+    if (!flow && (tiles > qbufs<W>(E).tmax || tiles * tile_pos > qbufs<W>(E).cap))
+        return fail(32, "batch exceeds the queue capacity");
     JobHdr h{};
     h.njobs = (u32)jobs.size(); h.req0 = jobs[0].req; h.ntiles = tiles;
     // copy the job list in (the staging buffer is free: the previous batch was synchronised)
@@ -626,8 +649,11 @@ int run_request_locked(Engine &E, ReqHost &h) {
             dev += ms;
         }
     } else {
-        for (u64 lo = r.lo; lo < r.hi && !rc; lo += E.batch) {
-            const u64 hi = std::min<u64>(r.hi, lo + E.batch);
+        // the flow kernel needs no queues, so its batches are larger (fewer batch ends)
+        const bool flow = r.detector == WAR_DET_ALT && t.stage_kind == 3 && r.rule == 0 && !h.dev.walk;
+        const u64 bstep = flow ? FLOW_BATCH : E.batch;
+        for (u64 lo = r.lo; lo < r.hi && !rc; lo += bstep) {
+            const u64 hi = std::min<u64>(r.hi, lo + bstep);
             std::vector<Job> jobs(1);
             jobs[0].lo = lo; jobs[0].hi = hi; jobs[0].req = (u32)h.slot;
             float ms = 0;

@@ -136,6 +136,7 @@ template <int W> struct StageArgs {
     u64 max_regions;       // regions of tile_pos slots each queue buffer holds
     u32 in_refill;         // the input regions were written by a refill stage (count: nreg[stage-1])
     u32 zero;              // always 0 (see Zh)
+    u32 flow_min;          // flow kernel: refill once this many lanes of 64 are free
 };
 
 // ------------------------------------------------------------------------------------------
@@ -144,6 +145,8 @@ __device__ __forceinline__ u64 ballot(bool p) { return __builtin_amdgcn_ballot_w
 __device__ __forceinline__ u32 rank_in(u64 mask) {   // set bits of mask below this lane
     return __builtin_amdgcn_mbcnt_hi((u32)(mask >> 32), __builtin_amdgcn_mbcnt_lo((u32)mask, 0u));
 }
+// this lane is in mask m (the mask itself is the condition: no vector work to test a lane's bit)
+__device__ __forceinline__ bool in_mask(u64 m) { return __builtin_amdgcn_inverse_ballot_w64(m); }
 __device__ __forceinline__ u32 uni(u32 x) { return __builtin_amdgcn_readfirstlane(x); }
 __device__ __forceinline__ u64 uni64(u64 x) {
     return ((u64)__builtin_amdgcn_readfirstlane((u32)(x >> 32)) << 32) | (u64)__builtin_amdgcn_readfirstlane((u32)x);
@@ -391,6 +394,14 @@ __device__ __forceinline__ u64 play(const StageArgs<W> &A, WaveAcc &acc, St<W> &
         alive = 0;
     }
     return alive;
+}
+
+// The lanes whose player 2 pile and length (and layout 6's depth of card n-1) equal y's: the rest
+// of the repeat test, as one mask (rest_same per lane, without turning the result into a value).
+template <int W> __device__ __forceinline__ u64 rest_mask(const St<W> &x, const St<W> &y) {
+    u64 m = ballot(x.b == y.b) & ballot(x.sa == y.sa);
+    if constexpr (W == 6) m &= ballot(x.pb == y.pb);
+    return m;
 }
 
 // Per-lane state of the alternation detector beside the game state.
@@ -931,6 +942,210 @@ __global__ __launch_bounds__(256) void k_refill(StageArgs<W> A) {
     reduce_lanes();
     acc.flush(reqs);
     if (acc.pushed && lane == 0) atomicAdd((unsigned long long *)&A.pipe->ctotal, (unsigned long long)acc.pushed);
+}
+
+// ------------------------------------------------------------------------------------------
+// Flow kernel: the alternation detector without stages (one request, no cycle walks; modes full
+// and canon: --half needs even n, the detector odd n). Every lane
+// plays its own game, and a lane whose game is over takes the next position at the next
+// checkpoint, so a wave never waits for the slowest game of a group and no survivor records are
+// written. Positions enter only at checkpoints (the clock a multiple of ALT_K), so every lane's
+// age is a multiple of ALT_K at every checkpoint, as the detector needs; the age is clock - birth
+// and the cycle length clock - R, per lane. Positions come from tiles of the job list as in the
+// first stage: lane l of a tile owns positions l*per .. l*per+per-1 (stepped with next_perm); the
+// 64 lanes' current positions, dealt, are the pool that refills hand out in lane order (cross-lane
+// reads), and the pool steps every lane to its next position once all 64 are handed out.
+template <typename P>
+__device__ __forceinline__ P shfl_word(P v, u32 src) {
+    if constexpr (sizeof(P) == 8) {
+        return ((u64)(u32)__shfl((int)(u32)(v >> 32), (int)src) << 32) | (u64)(u32)__shfl((int)(u32)v, (int)src);
+    } else {
+        P r = 0;
+        for (int q = 3; q >= 0; q--) r = (r << 32) | (P)(u32)__shfl((int)(u32)(v >> (32 * q)), (int)src);
+        return r;
+    }
+}
+
+template <int W>
+__device__ __forceinline__ St<W> shfl_state(const St<W> &x, u32 src) {
+    St<W> r;
+    r.a = shfl_word(x.a, src);
+    r.b = shfl_word(x.b, src);
+    r.sa = (u32)__shfl((int)x.sa, (int)src);
+    if constexpr (W == 6) r.pb = (u32)__shfl((int)x.pb, (int)src);
+    return r;
+}
+
+template <int W>
+__global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
+    typedef typename Lay<W>::P P;
+    constexpr int PW = Lay<W>::PW;
+    extern __shared__ u32 lds[];
+    const u32 wid = threadIdx.x >> 6;
+    const u32 lane = lane_id();
+    WaveAcc acc;
+    acc.init(lds + wid * (A.hb + A.lb), A.hb, A.lb);
+    const ReqDev *reqs = A.reqs;
+    const u32 req = A.hdr->req0;
+    acc.select(req, reqs);
+    const u32 n = reqs[req].n, mode = reqs[req].mode;
+    const u32 C = c_of<W>(n);
+    const u32 per = A.per, TP = A.tile_pos;
+    const u64 ntiles = A.hdr->ntiles;
+    const u32 min_dead = A.flow_min;       // refill once this many lanes are free (or none is alive)
+    u64 *grab = &A.pipe->grab[0];
+    u64 nxt = 0;
+    if (lane == 0) nxt = atomicAdd((unsigned long long *)grab, 1ull);
+
+    // ---- the pool: lane l holds the current position of its run, already dealt (the deal of a
+    // whole generation of 64 runs as one wave step)
+    P q = 0;
+    u32 k = 0, left = 0;       // this lane's run: current order / split; positions left, it included
+    St<W> ps = {};             // the current position, dealt
+    u32 ptr = 64;              // next pool entry to hand out (64: all handed out)
+    bool more = true;          // positions remain (in the pool or in tiles not yet taken)
+    auto step = [&]() {
+        if (mode == 2) { if (++k == n - 1u) { k = 0; q = next_perm<PW>(q, (int)n - 1); } }
+        else q = next_perm<PW>(q, (int)n);
+    };
+    // every lane to the next position of its run; a new tile once every run is done
+    auto advance = [&]() -> bool {
+        if (left > 1u) step();
+        left = left ? left - 1u : 0u;
+        while (ballot(left != 0u) == 0ull) {
+            const u64 g0 = uni64(nxt);
+            if (g0 >= ntiles) return false;
+            if (lane == 0) nxt = atomicAdd((unsigned long long *)grab, 1ull);
+            u32 lo = 0, hi = A.hdr->njobs;
+            while (hi - lo > 1) { u32 mid = (lo + hi) >> 1; if (A.jobs[mid].tile_base <= g0) lo = mid; else hi = mid; }
+            const Job J = A.jobs[lo];
+            const u64 first = J.lo + (g0 - J.tile_base) * (u64)TP + (u64)lane * per;
+            left = first < J.hi ? (u32)((J.hi - first) < per ? (J.hi - first) : per) : 0u;
+            if (left) {
+                if (mode == 2) {
+                    const u32 m = n - 1u;
+                    const u64 r = first / m;
+                    k = (u32)(first - r * m);
+                    q = unrank<PW>(r, (int)m);
+                } else {
+                    k = 0;
+                    q = unrank<PW>(first, (int)n);
+                }
+            }
+        }
+        if (left) ps = (mode == 2) ? canon<W>(q, (int)n, (int)k) : deal<W>(q, (int)n);
+        ptr = 0;
+        return true;
+    };
+
+    // ---- the games
+    St<W> s = {};
+    AltLane<W> T_;
+    T_.t = s; T_.R = 0; T_.L = ALT_L0;
+    u32 birth = 0, clk = 0;
+    u64 alive = 0, prevm = 0, streakm = 0, endm = 0, cycm = 0;
+    constexpr int FB = __builtin_ctz(ALT_K);
+    u64 fb[FB];                // the turn of the block each finished game ended in, bit by bit
+    for (int b = 0; b < FB; b++) fb[b] = 0;
+    u64 lturns = 0;            // per lane: sum of its finished games' lengths / detection ages
+    u32 lmax = 0;              // per lane: longest ending game
+    Zh<W> z = make_zh<W>(A.zero);
+
+    // free lanes take pool entries in lane order until every lane plays or no position is left
+    // (every lane calls this)
+    auto refill = [&]() {
+        for (;;) {
+            u64 dead = ~alive;
+            while (dead) {
+                if (ptr == 64u && !advance()) { more = false; break; }
+                const u32 i = rank_in(dead);
+                const bool take = in_mask(dead) && i < 64u - ptr;
+                const u32 src = (ptr + i) & 63u;
+                const St<W> ns = shfl_state<W>(ps, src);
+                const u32 lf = (u32)__shfl((int)left, (int)src);
+                const u64 tm = ballot(take);
+                ptr += (u32)__popcll(tm);
+                dead &= ~tm;
+                const u64 gm = ballot(take && lf != 0u);
+                if (in_mask(gm)) {
+                    s = ns;
+                    open_sa<W>(s, A.zero);
+                    T_.t = s; T_.R = clk; T_.L = ALT_L0; birth = clk;
+                }
+                // a position that is over before it starts: game length 0 (gpu/war_gpu.py:80)
+                const u64 e0 = ballot(n < 2u || ended<W>(s, C)) & gm;
+                if (e0) acc.ended((u32)__popcll(e0), 0);
+                alive |= gm & ~e0;
+                prevm |= gm;   // before the first turn, player 1 counts as the last winner
+            }
+            if (alive || !more) return;
+        }
+    };
+    // account the games that finished in the block that began at clock clk0 (every lane calls this)
+    auto settle = [&](u32 clk0) {
+        u32 fin = clk0 + 1u;
+        for (int b = 0; b < FB; b++) { if (in_mask(fb[b])) fin += 1u << b; fb[b] = 0; }
+        const u32 age = fin - birth, lam = fin - T_.R;
+        if (in_mask(endm | cycm)) lturns += age;
+        if (in_mask(endm)) {
+            lmax = age > lmax ? age : lmax;
+            if (acc.lb) atomicAdd(&acc.ll[age < acc.lb - 1 ? age : acc.lb - 1], 1u);
+        }
+        if (acc.hb && in_mask(cycm)) atomicAdd(&acc.lh[lam < acc.hb - 1 ? lam : acc.hb - 1], 1u);
+        acc.done += __popcll(endm | cycm); acc.cyc += __popcll(cycm); acc.dirty = true;
+        endm = cycm = 0;
+    };
+
+    refill();
+    while (alive) {
+        const u32 clk0 = clk;
+        // one block of ALT_K turns up to the next checkpoint
+#pragma unroll
+        for (u32 u = 0; u < ALT_K; u++) {
+            const u64 pm = ballot(turn<W, 0>(s, C, z));
+            const u64 em = ballot(ended<W>(s, C)) & alive;
+            u64 dm = ballot(s.a == T_.t.a) & alive;
+            streakm |= ~(pm ^ prevm);
+            prevm = pm;
+            if (em | dm) {
+                dm &= ~em;
+                if (dm) dm &= rest_mask<W>(s, T_.t);
+                const u64 ev = em | dm;
+                for (int b = 0; b < FB; b++) if ((u >> b) & 1u) fb[b] |= ev;
+                endm |= em; cycm |= dm; alive &= ~ev;
+            }
+        }
+        clk += ALT_K;
+        if (endm | cycm) settle(clk0);
+        // checkpoint: timed jumps (the wait doubles), then jumps after a repeated winner
+        const u64 ex = ballot(clk - T_.R >= T_.L) & alive;
+        const u64 js = (ex | streakm) & alive;
+        streakm = 0;
+        if (js) {
+            if (in_mask(js)) {
+                T_.t = s; T_.R = clk;
+                if (in_mask(ex)) T_.L <<= 1;
+            }
+        }
+        // the safety stop: a game still running at AGE_CAP turns is undetermined (never for a valid game)
+        const u64 um = ballot(clk - birth >= AGE_CAP) & alive;
+        if (um) {
+            acc.undet += __popcll(um); acc.done += __popcll(um); acc.dirty = true;
+            alive &= ~um;
+        }
+        if (more && (u32)__popcll(~alive) >= min_dead) refill();
+    }
+    u64 tsum = lturns;
+    u32 tmax = lmax;
+    for (int off = 32; off > 0; off >>= 1) {
+        tsum += __shfl_xor(tsum, off);
+        const u32 o = __shfl_xor(tmax, off);
+        tmax = o > tmax ? o : tmax;
+    }
+    acc.turns += tsum;
+    acc.maxfin = tmax > acc.maxfin ? tmax : acc.maxfin;
+    acc.dirty = true;
+    acc.flush(reqs);
 }
 
 // ------------------------------------------------------------------------------------------

@@ -26,7 +26,7 @@ def total_positions(n, mode):
     return math.factorial(n)
 
 
-STAGE_KINDS = {"refill": 1, "lockstep": 2}
+STAGE_KINDS = {"refill": 1, "lockstep": 2, "flow": 3}
 
 
 def _tuning(t):
@@ -42,7 +42,7 @@ def _tuning(t):
 
 def _tuning_dict(t):
     return {"waves_per_cu": t.waves_per_cu, "ranks_per_refill": t.ranks_per_refill, "brent_cap": t.brent_cap,
-            "plain_per": t.plain_per, "stage_kind": {1: "refill", 2: "lockstep"}.get(t.stage_kind, t.stage_kind),
+            "plain_per": t.plain_per, "stage_kind": {1: "refill", 2: "lockstep", 3: "flow"}.get(t.stage_kind, t.stage_kind),
             "pile_layout": t.pile_layout}
 
 

@@ -58,7 +58,12 @@ typedef struct war_tuning {
     int32_t brent_cap;         /* first stage boundary 2^k-1 (1..65535); later stages double    */
     int32_t plain_per;         /* plain kernel: consecutive positions per lane (1..4096)        */
     int32_t stage_kind;        /* persistent stages after the first: 1 = refill (a lane that
-                                  finishes takes the next record at once), 2 = lockstep; 0 = default */
+                                  finishes takes the next record at once), 2 = lockstep;
+                                  3 = flow: no stages, one kernel per batch in which a lane whose
+                                  game is over takes the next position at the next checkpoint
+                                  (WAR_DET_ALT requests of one request per batch without cycle
+                                  walks; elsewhere 3 runs as 2); 0 = default (3 for WAR_DET_ALT,
+                                  else 2) */
     int32_t pile_layout;       /* 0 = default; 4 = 4-bit piles (n <= 16); 5 = 5-bit piles (any n);
                                   6 = n = 17 with card 16 held apart and 4-bit piles (the layout
                                   of gpu/war17.py; persistent kernel, Brent, no cycle walks).

@@ -255,8 +255,13 @@ def test_alt_walks(eng, orc, n):
     check_alt(eng, orc, n, "canon", 0, orc.total(n, "canon"), dbins=4096, distinct=True, shape_blk=n & -n)
 
 
-@pytest.mark.parametrize("tuning", [{"brent_cap": 7}, {"brent_cap": 15, "ranks_per_refill": 64},
-                                    {"brent_cap": 63, "waves_per_cu": 4}, {"brent_cap": 255, "ranks_per_refill": 8192}])
+@pytest.mark.parametrize("tuning", [{"brent_cap": 7, "stage_kind": "lockstep"},
+                                    {"brent_cap": 15, "ranks_per_refill": 64, "stage_kind": "lockstep"},
+                                    {"brent_cap": 63, "waves_per_cu": 4, "stage_kind": "lockstep"},
+                                    {"brent_cap": 255, "ranks_per_refill": 8192, "stage_kind": "lockstep"},
+                                    {"stage_kind": "flow", "ranks_per_refill": 64},
+                                    {"stage_kind": "flow", "ranks_per_refill": 8192, "waves_per_cu": 4},
+                                    {"stage_kind": "flow", "ranks_per_refill": 128, "waves_per_cu": 32}])
 def test_alt_schedules_are_exact(eng, orc, tuning):
     print("case alt schedule", tuning, flush=True)
     check_alt(eng, orc, 11, "full", 0, orc.total(11, "full"), tuning=tuning)
