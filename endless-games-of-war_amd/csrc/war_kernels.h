@@ -327,6 +327,8 @@ enum { F_MULTI = 1, F_WALK = 2, F_ALT = 4 };
 // exact state matches, and the first match after a jump is the cycle length, so every output is
 // exact; games are seen to repeat in about 30 % fewer turns.
 constexpr u32 ALT_K = 8, ALT_L0 = 128;
+// flow kernel: checkpoints per span (finished games are accounted and replaced once per span)
+constexpr u32 FLOW_BLOCKS = 2;
 
 template <int W, int RULE, bool FRESH, int FLAGS>
 __device__ __forceinline__ u64 play(const StageArgs<W> &A, WaveAcc &acc, St<W> &s, u32 C, u32 n, u64 alive,
@@ -1044,8 +1046,11 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
     T_.t = s; T_.R = 0; T_.L = ALT_L0;
     u32 birth = 0, clk = 0;
     u64 alive = 0, prevm = 0, streakm = 0, endm = 0, cycm = 0;
-    constexpr int FB = __builtin_ctz(ALT_K);
-    u64 fb[FB];                // the turn of the block each finished game ended in, bit by bit
+    // games end, are accounted and are replaced once per FLOW_SPAN turns (FLOW_BLOCKS checkpoints)
+    constexpr u32 SPAN = ALT_K * FLOW_BLOCKS;
+    constexpr int FB = __builtin_ctz(SPAN);
+    static_assert((SPAN & (SPAN - 1u)) == 0u && (AGE_CAP % SPAN) == 0u, "span: a power of 2 dividing AGE_CAP");
+    u64 fb[FB];                // the turn of the span each finished game ended in, bit by bit
     for (int b = 0; b < FB; b++) fb[b] = 0;
     u64 lturns = 0;            // per lane: sum of its finished games' lengths / detection ages
     u32 lmax = 0;              // per lane: longest ending game
@@ -1059,29 +1064,28 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
             while (dead) {
                 if (ptr == 64u && !advance()) { more = false; break; }
                 const u32 i = rank_in(dead);
-                const bool take = in_mask(dead) && i < 64u - ptr;
+                const u64 tm = ballot(i < 64u - ptr) & dead;       // the first free lanes, one per entry
                 const u32 src = (ptr + i) & 63u;
                 const St<W> ns = shfl_state<W>(ps, src);
                 const u32 lf = (u32)__shfl((int)left, (int)src);
-                const u64 tm = ballot(take);
                 ptr += (u32)__popcll(tm);
                 dead &= ~tm;
-                const u64 gm = ballot(take && lf != 0u);
+                const u64 gm = ballot(lf != 0u) & tm;               // entries that hold a position
                 if (in_mask(gm)) {
                     s = ns;
                     open_sa<W>(s, A.zero);
                     T_.t = s; T_.R = clk; T_.L = ALT_L0; birth = clk;
                 }
-                // a position that is over before it starts: game length 0 (gpu/war_gpu.py:80)
-                const u64 e0 = ballot(n < 2u || ended<W>(s, C)) & gm;
-                if (e0) acc.ended((u32)__popcll(e0), 0);
+                u64 e0 = 0;
+                // a position that is over before it starts (only with fewer than 2 cards): length 0
+                if (n < 2u) { e0 = gm; acc.ended((u32)__popcll(e0), 0); }
                 alive |= gm & ~e0;
                 prevm |= gm;   // before the first turn, player 1 counts as the last winner
             }
             if (alive || !more) return;
         }
     };
-    // account the games that finished in the block that began at clock clk0 (every lane calls this)
+    // account the games that finished in the span that began at clock clk0 (every lane calls this)
     auto settle = [&](u32 clk0) {
         u32 fin = clk0 + 1u;
         for (int b = 0; b < FB; b++) { if (in_mask(fb[b])) fin += 1u << b; fb[b] = 0; }
@@ -1099,34 +1103,38 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
     refill();
     while (alive) {
         const u32 clk0 = clk;
-        // one block of ALT_K turns up to the next checkpoint
 #pragma unroll
-        for (u32 u = 0; u < ALT_K; u++) {
-            const u64 pm = ballot(turn<W, 0>(s, C, z));
-            const u64 em = ballot(ended<W>(s, C)) & alive;
-            u64 dm = ballot(s.a == T_.t.a) & alive;
-            streakm |= ~(pm ^ prevm);
-            prevm = pm;
-            if (em | dm) {
-                dm &= ~em;
-                if (dm) dm &= rest_mask<W>(s, T_.t);
-                const u64 ev = em | dm;
-                for (int b = 0; b < FB; b++) if ((u >> b) & 1u) fb[b] |= ev;
-                endm |= em; cycm |= dm; alive &= ~ev;
+        for (u32 blk = 0; blk < FLOW_BLOCKS; blk++) {
+            // one block of ALT_K turns up to the next checkpoint
+#pragma unroll
+            for (u32 u = 0; u < ALT_K; u++) {
+                const u32 tix = blk * ALT_K + u;       // turn of the span
+                const u64 pm = ballot(turn<W, 0>(s, C, z));
+                const u64 em = ballot(ended<W>(s, C)) & alive;
+                u64 dm = ballot(s.a == T_.t.a) & alive;
+                streakm |= ~(pm ^ prevm);
+                prevm = pm;
+                if (em | dm) {
+                    dm &= ~em;
+                    if (dm) dm &= rest_mask<W>(s, T_.t);
+                    const u64 ev = em | dm;
+                    for (int b = 0; b < FB; b++) if ((tix >> b) & 1u) fb[b] |= ev;
+                    endm |= em; cycm |= dm; alive &= ~ev;
+                }
+            }
+            clk += ALT_K;
+            // checkpoint: timed jumps (the wait doubles), then jumps after a repeated winner
+            const u64 ex = ballot(clk - T_.R >= T_.L) & alive;
+            const u64 js = (ex | streakm) & alive;
+            streakm = 0;
+            if (js) {
+                if (in_mask(js)) {
+                    T_.t = s; T_.R = clk;
+                    if (in_mask(ex)) T_.L <<= 1;
+                }
             }
         }
-        clk += ALT_K;
         if (endm | cycm) settle(clk0);
-        // checkpoint: timed jumps (the wait doubles), then jumps after a repeated winner
-        const u64 ex = ballot(clk - T_.R >= T_.L) & alive;
-        const u64 js = (ex | streakm) & alive;
-        streakm = 0;
-        if (js) {
-            if (in_mask(js)) {
-                T_.t = s; T_.R = clk;
-                if (in_mask(ex)) T_.L <<= 1;
-            }
-        }
         // the safety stop: a game still running at AGE_CAP turns is undetermined (never for a valid game)
         const u64 um = ballot(clk - birth >= AGE_CAP) & alive;
         if (um) {
