@@ -1,22 +1,25 @@
 """Benchmark of the MI355X War engine; prints ONE JSON line.
 
 Headline workload (value): the n = 13 full enumeration, every one of the 13! = 6,227,020,800
-deals, played to its end or to its cycle by the persistent batched kernel, counts checked
-bit-exactly against the published a(13) = 4,880,546,113, longest 57 and cycle-length histogram.
-One step = one whole enumeration. Beside it, on the same GPU in the same run:
+deals, played to its end or to its cycle, as the drop-in war_gpu.py runs it by default: the flow
+kernel with the alternation-checkpoint detector (exact; odd n, rule 0). Counts are checked
+bit-exactly against the published a(13) = 4,880,546,113, longest 57 and cycle-length histogram
+every step. One step = one whole enumeration. Beside it, on the same GPU in the same run:
+  brent_flow     the same enumeration with Brent's detector on the flow kernel (its turn count is
+                 the reference's Sum of rounds), and with Brent on the staged pipeline;
   plain_kernel   the one-deal-per-lane kernel on the same n = 13 enumeration;
-  canon14        the n = 14 canonical scan (13! * 13 positions), persistent kernel, against the
-                 published counts, and both kernels on the same sampled chunks (the batched/plain
-                 ratio);
-  n17_range      the n = 17 rank range [0, 8992587776) the reference logged, in the held-card
-                 layout (4-bit piles, card 16 apart) and in 5-bit piles;
+  canon14        the n = 14 canonical scan (13! * 13 positions, Brent; flow kernel and staged
+                 pipeline) against the published counts, and the plain kernel on sampled chunks;
+  n17_range      the n = 17 rank range [0, 8992587776) the reference logged: held-card layout
+                 (4-bit piles, card 16 apart) and 5-bit piles, Brent and the alternation detector;
   serving        a continuous-batching run of mixed queries with p50/p99 latency, beside the same
-                 queries one by one through the plain kernel;
-  roofline       integer VALU: useful lane-ops/s (Brent turns x VALU ops per turn of the
-                 age-synchronous loop) against CUs x 64 lanes x clock;
+                 queries one by one through the plain and the persistent kernels;
+  roofline       integer VALU: useful lane-ops/s (turns x VALU ops per turn of the flow kernel's
+                 loop) against CUs x 64 lanes x clock;
   cpu_baseline   the CPU oracle (OpenMP) on a bounded sample of the same n = 13 workload.
 
-  python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--only main|canon14|plain13|copy] [--no-extras]
+  python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--only main|brent13|staged13|canon14|plain13|copy]
+                  [--no-extras]
 """
 import argparse
 import json
@@ -30,16 +33,18 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "endless-games-of-war_amd"))
 
 METRIC = "deals/sec (n=13 full, n=14 canon), bit-exact; batched vs plain kernel; p99 latency"
+DET13 = "auto"     # set in main(): the cycle test war_gpu.py picks for n = 13 (war_gpu.detector_for)
 PUBLISHED_N13_DEALS_PER_S = 6227020800 / 15.3      # RTX 5080, war_gpu.py 13 (reference data/checks.txt:23)
 A13 = {"cyc": 4880546113, "maxfin": 57, "hist": {42: 1475465743, 56: 2394985450, 70: 1010094920}}
 C14 = {"cyc": 2872129173, "maxfin": 465, "hist": {96: 43669233, 112: 2828459940}}
-# VALU instructions per War turn in the age-synchronous loop (18 for the turn, one end test, one
-# compare of player 1's pile with the tortoise's), counted in the gfx950 assembly of k_stage; the
-# algorithmic unit is one Brent turn.
+# VALU instructions per War turn in the flow kernel's loop (18 for the turn, one end test, one
+# compare of player 1's pile with the tortoise's), counted in the gfx950 assembly of k_flow (the
+# same 20 as k_stage's age-synchronous loop); the algorithmic unit is one turn of the detector run.
 VALU_OPS_PER_TURN = 20
-# HBM bytes per k_stage launch on the n = 13 workload (survivor records; 209 GB over 144 launches),
-# from the rocprofv3 byte counters calibrated on a copy of known size.
-K_STAGE_HBM_BYTES_PER_LAUNCH = 1.45e9
+# HBM bytes per k_flow launch on the n = 13 workload (2 launches; no records are written: 0.31 MB
+# read, 98.9 MB of tile-grab atomics and counters), from the rocprofv3 byte counters calibrated on a
+# copy of known size.
+K_FLOW_HBM_BYTES_PER_LAUNCH = 9.92e7
 
 
 def log(*a):
@@ -83,7 +88,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--only", choices=["main", "canon14", "plain13", "copy"], default=None,
+    ap.add_argument("--only", choices=["main", "brent13", "staged13", "canon14", "plain13", "copy"], default=None,
                     help="run one workload only (for profiling); copy: a 1 GiB device-to-device copy that "
                          "calibrates the HBM byte counters")
     ap.add_argument("--no-extras", action="store_true")
@@ -107,6 +112,9 @@ def main():
         return
 
     from war_amd import Engine
+    from war_gpu import detector_for
+    global DET13
+    DET13 = detector_for(13, 0, "persistent")
     eng = Engine(0)
     dev = eng.device_info()
     log("device", dev)
@@ -117,9 +125,19 @@ def main():
     tun = eng.tuning_for(13, "full")
 
     if a.only == "main":
+        r = eng.run_range(13, "full", hbins=256, detector=DET13)
+        check_n13(r)
+        log("n13 flow (%s) device_ms %.1f" % (DET13, r["device_ms"]))
+        return
+    if a.only == "brent13":
         r = eng.run_range(13, "full", hbins=256)
         check_n13(r)
-        log("n13 persistent device_ms %.1f" % r["device_ms"])
+        log("n13 flow (brent) device_ms %.1f" % r["device_ms"])
+        return
+    if a.only == "staged13":
+        r = eng.run_range(13, "full", hbins=256, tuning={"stage_kind": "lockstep"})
+        check_n13(r)
+        log("n13 staged (brent) device_ms %.1f" % r["device_ms"])
         return
     if a.only == "canon14":
         r = eng.run_range(14, "canon", hbins=256)
@@ -132,14 +150,14 @@ def main():
         log("n13 plain device_ms %.1f" % r["device_ms"])
         return
 
-    # ---- headline: n = 13 full enumeration, persistent kernel
+    # ---- headline: n = 13 full enumeration as the drop-in runs it (flow kernel, alternation detector)
     for _ in range(a.warmup):
-        check_n13(eng.run_range(13, "full", hbins=256))
+        check_n13(eng.run_range(13, "full", hbins=256, detector=DET13))
     t0 = time.perf_counter()
     dev_ms = 0.0
     turns = 0
     for _ in range(a.steps):
-        r = eng.run_range(13, "full", hbins=256, lbins=512)
+        r = eng.run_range(13, "full", hbins=256, lbins=512, detector=DET13)
         check_n13(r)
         dev_ms += r["device_ms"]
         turns += r["turns"]
@@ -152,22 +170,41 @@ def main():
         "ms_per_step": 1e3 * wall / a.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": value / PUBLISHED_N13_DEALS_PER_S, "dtype": "u64",
         "data": "synthetic: exhaustive enumeration of all 13! deals (no input data)",
-        "config": {"workload": "n13_full: all 13! = 6227020800 deals, persistent batched kernel",
-                   "n": 13, "mode": "full", "rule": 0, "positions": n13, "kernel": "persistent", "tuning": tun,
+        "config": {"workload": "n13_full: all 13! = 6227020800 deals, flow kernel (persistent), as war_gpu.py 13 "
+                               "runs it", "n": 13, "mode": "full", "rule": 0, "positions": n13,
+                   "kernel": "persistent", "detector": DET13, "tuning": tun,
                    "parity": "a(13), longest 57, cycle-length histogram equal the published values every step"},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": peak_lane_ops / 1e12,
-                     "unit": "Tlane-op/s", "frac": achieved / peak_lane_ops, "traffic": K_STAGE_HBM_BYTES_PER_LAUNCH,
-                     "kernel": "k_stage: the stage kernels of each batch graph (FRESH + queue stages)",
-                     "traffic_note": "HBM bytes per k_stage launch at this workload from rocprofv3 FETCH_SIZE and "
+                     "unit": "Tlane-op/s", "frac": achieved / peak_lane_ops, "traffic": K_FLOW_HBM_BYTES_PER_LAUNCH,
+                     "kernel": "k_flow<4, 0, alt>: one launch per batch of 2^32 deals",
+                     "traffic_note": "HBM bytes per k_flow launch at this workload from rocprofv3 FETCH_SIZE and "
                                      "WRITE_SIZE (scaled by a 1 GiB copy: x2.0 and x1.0), profiles/"
-                                     "k_stage_n13_full_pmc.md; not measured inside this run",
+                                     "k_flow_n13_full_pmc.md; not measured inside this run",
                      "turns_per_step": turns // a.steps, "valu_ops_per_turn": VALU_OPS_PER_TURN,
                      "device_ms_per_step": dev_ms / a.steps,
-                     "note": "achieved = Brent turns x VALU ops per turn / device time (HIP events)"},
+                     "note": "achieved = the detector's turns x VALU ops per turn / device time (HIP events on "
+                             "the engine's stream around each batch)"},
     }
-    log("n13 persistent: %.3e deals/s, %.1f ms/step" % (value, 1e3 * wall / a.steps))
+    log("n13 flow (%s): %.3e deals/s, %.1f ms/step" % (DET13, value, 1e3 * wall / a.steps))
 
     if not a.no_extras:
+        # ---- Brent on the same workload: flow kernel, and the staged pipeline
+        bf = {}
+        for name, t in (("flow", None), ("staged", {"stage_kind": "lockstep"})):
+            r = eng.run_range(13, "full", hbins=256, tuning=t)
+            check_n13(r)
+            bf[name] = r
+        bt = bf["flow"]["turns"]
+        out["brent_flow"] = {"workload": "n13_full", "value": n13 / (bf["flow"]["device_ms"] / 1e3), "unit": "deals/s",
+                             "device_ms": bf["flow"]["device_ms"], "turns": bt,
+                             "frac": bt * VALU_OPS_PER_TURN / (bf["flow"]["device_ms"] / 1e3) / peak_lane_ops,
+                             "staged_device_ms": bf["staged"]["device_ms"],
+                             "staged_frac": bt * VALU_OPS_PER_TURN / (bf["staged"]["device_ms"] / 1e3) / peak_lane_ops,
+                             "note": "Brent's turns are the reference's Sum of rounds (the same on both paths); "
+                                     "frac as for the roofline, on these turns"}
+        assert bf["staged"]["turns"] == bt
+        log("n13 brent: flow %.1f ms, staged %.1f ms" % (bf["flow"]["device_ms"], bf["staged"]["device_ms"]))
+
         # ---- plain kernel on the same workload
         t0 = time.perf_counter()
         r = eng.run_range(13, "full", kernel="plain", hbins=256)
@@ -177,14 +214,18 @@ def main():
                                "device_ms": r["device_ms"], "persistent_over_plain": value / (n13 / pw)}
         log("n13 plain: %.3e deals/s" % (n13 / pw))
 
-        # ---- n = 14 canonical scan, persistent, whole range
+        # ---- n = 14 canonical scan, persistent, whole range: flow kernel, then the staged pipeline
         t0 = time.perf_counter()
         r = eng.run_range(14, "canon", hbins=256)
         cw = time.perf_counter() - t0
         assert r["cyc"] == C14["cyc"] and r["maxfin"] == C14["maxfin"] and hist_of(r) == C14["hist"]
         tot14 = math.factorial(13) * 13
+        rs = eng.run_range(14, "canon", hbins=256, tuning={"stage_kind": "lockstep"})
+        assert (rs["cyc"], rs["maxfin"], rs["turns"]) == (r["cyc"], r["maxfin"], r["turns"])
         canon = {"workload": "n14_canon: all 13!*13 = 80951270400 positions", "persistent_value": tot14 / cw,
                  "unit": "positions/s", "persistent_device_ms": r["device_ms"],
+                 "frac": r["turns"] * VALU_OPS_PER_TURN / (r["device_ms"] / 1e3) / peak_lane_ops,
+                 "staged_device_ms": rs["device_ms"], "turns": r["turns"],
                  "parity": "cycling 2872129173, longest 465, hist {96,112} equal the published values"}
         # both kernels on the same sampled chunks (seed 14, whole q-ranks): batched / plain ratio
         ch = [(lo - lo % 13, lo - lo % 13 + 13 * (1 << 22)) for lo, _ in chunks_spread(tot14, 8, 13 * (1 << 22), 14)]
@@ -201,16 +242,6 @@ def main():
         out["canon14"] = canon
         log("canon14: %.3e pos/s persistent (whole), ratio %.2f on samples" % (tot14 / cw, tp / tq))
 
-        # ---- the alternation-checkpoint detector (odd n; exact, fewer turns) on the same n = 13 run
-        ra = eng.run_range(13, "full", hbins=256, detector="alt")
-        check_n13(ra)
-        rb = eng.run_range(13, "full", hbins=256)
-        out["alt_detector"] = {"workload": "n13_full", "value": n13 / (ra["device_ms"] / 1e3), "unit": "deals/s",
-                               "device_ms": ra["device_ms"], "brent_device_ms": rb["device_ms"],
-                               "turns_per_deal": ra["turns"] / n13, "brent_turns_per_deal": rb["turns"] / n13,
-                               "note": "same exact counts; fewer turns, more per-lane work per turn"}
-        log("n13 alt detector: %.1f ms (Brent %.1f ms)" % (ra["device_ms"], rb["device_ms"]))
-
         # ---- n = 17, the published rank range [0, 8992587776): held-card layout against 5-bit piles
         lo17, hi17 = 0, 8992587776
         n17 = {"workload": "n17_range: deal ranks [0, 8992587776) of 17!", "unit": "deals/s",
@@ -225,6 +256,7 @@ def main():
         r = eng.run_range(17, "full", lo17, hi17, detector="alt")
         assert (r["cyc"], r["maxfin"], r["done"]) == (8016845372, 94, hi17 - lo17), r
         n17["layout6_alt_value"] = (hi17 - lo17) / (r["device_ms"] / 1e3)
+        n17["layout6_alt_device_ms"] = r["device_ms"]
         out["n17_range"] = n17
         log("n17 range: held layout %.3e deals/s, 5-bit %.3e" % (n17["layout6_value"], n17["layout5_value"]))
 

@@ -195,12 +195,9 @@ bool held_ok(const war_request &r) {
 // Defaults measured on MI355X: deals (long games, ~110 turns at n = 13) want a late first stage
 // boundary; canonical positions (most end within ~30 turns) an early one and larger tiles.
 war_tuning default_tuning(int n, int mode, int det) {
-    war_tuning t{28, 1024, 15, 32, 2, 0};
+    war_tuning t{28, 1024, 15, 32, 3, 0};     // stage_kind 3: the flow kernel where it applies
     if (mode != WAR_MODE_CANON && n >= 12) t.brent_cap = 63;
-    if (det == WAR_DET_ALT) {
-        t.brent_cap = mode == WAR_MODE_CANON ? 31 : 63;   // stage windows of 32 / 64 turns (when staged)
-        t.stage_kind = 3;                                  // the flow kernel where it applies
-    }
+    if (det == WAR_DET_ALT) t.brent_cap = mode == WAR_MODE_CANON ? 31 : 63;   // stage windows of 32 / 64 turns (when staged)
     if (mode == WAR_MODE_CANON) t.ranks_per_refill = 2048;
     return t;
 }
@@ -463,15 +460,14 @@ int build_graph(Engine &E, const GraphKey &key, hipGraphExec_t *out) {
     HIPCHK(hipStreamBeginCapture(E.st, hipStreamCaptureModeThreadLocal));
     hipLaunchKernelGGL(k_setup, dim3(256), dim3(256), 0, E.st, E.pipe, (const JobHdr *)E.hdr,
                        key.walk ? Q.ccnt : (u32 *)nullptr);
-    if constexpr (RULE == 0) {
-        if (key.kind == 3) {            // the flow kernel: one launch plays the whole batch
-            StageArgs<W> A{};
-            A.pipe = E.pipe; A.hdr = E.hdr; A.jobs = E.jobs; A.reqs = E.reqs;
-            A.per = (u32)key.per; A.tile_pos = 64u * (u32)key.per;
-            A.hb = (u32)key.hb; A.lb = (u32)key.lb;
-            A.flow_min = (u32)key.cap0;
-            hipLaunchKernelGGL((k_flow<W>), dim3(blocks), dim3(256), lds, E.st, A);
-        }
+    if (key.kind == 3) {                // the flow kernel: one launch plays the whole batch
+        StageArgs<W> A{};
+        A.pipe = E.pipe; A.hdr = E.hdr; A.jobs = E.jobs; A.reqs = E.reqs;
+        A.per = (u32)key.per; A.tile_pos = 64u * (u32)key.per;
+        A.hb = (u32)key.hb; A.lb = (u32)key.lb;
+        A.flow_min = (u32)key.cap0;
+        if (!key.alt) hipLaunchKernelGGL((k_flow<W, RULE, FLOW_BRENT>), dim3(blocks), dim3(256), lds, E.st, A);
+        else if constexpr (RULE == 0) hipLaunchKernelGGL((k_flow<W, 0, FLOW_ALT>), dim3(blocks), dim3(256), lds, E.st, A);
     }
     u32 a0 = 0;
     for (size_t s = 0; s < caps.size(); s++) {
@@ -520,9 +516,9 @@ int run_persistent_batch(Engine &E, std::vector<Job> &jobs, const war_tuning &t,
                          bool multi, float *ms, bool alt) {
     if (jobs.empty()) return 0;
     if ((int)jobs.size() > MAXJOBS) return fail(31, "too many jobs in one batch");
-    // the flow kernel (stage_kind 3) plays single-request batches of the alternation detector
-    // without cycle walks; elsewhere stage_kind 3 runs lockstep stages
-    const bool flow = RULE == 0 && alt && t.stage_kind == 3 && !walk && !multi;
+    // the flow kernel (stage_kind 3) plays single-request batches without cycle walks; elsewhere
+    // stage_kind 3 runs lockstep stages
+    const bool flow = t.stage_kind == 3 && !walk && !multi && (RULE == 0 || !alt);
     int rc = flow ? 0 : ensure_queues<W>(E, walk, alt);
     if (rc) return rc;
     const u64 tile_pos = (u64)t.ranks_per_refill;
@@ -650,7 +646,7 @@ int run_request_locked(Engine &E, ReqHost &h) {
         }
     } else {
         // the flow kernel needs no queues, so its batches are larger (fewer batch ends)
-        const bool flow = r.detector == WAR_DET_ALT && t.stage_kind == 3 && r.rule == 0 && !h.dev.walk;
+        const bool flow = t.stage_kind == 3 && !h.dev.walk;
         const u64 bstep = flow ? FLOW_BATCH : E.batch;
         for (u64 lo = r.lo; lo < r.hi && !rc; lo += bstep) {
             const u64 hi = std::min<u64>(r.hi, lo + bstep);
@@ -1081,12 +1077,12 @@ int war_autotune(int32_t n, int32_t mode, int32_t rule, double budget_ms, war_tu
     const int caps[] = {7, 15, 31, 63, 127};
     const int rprs[] = {256, 1024, 2048, 4096};
     const int wpcs[] = {8, 16, 24, 28, 32};
-    const int kinds[] = {1, 2};
+    const int kinds[] = {1, 2, 3};
     const int lays[] = {5, 6};
     const int ndim = held_ok(r) ? 5 : 4;
     for (int pass = 0; pass < 2; pass++) {
         for (int dim = 0; dim < ndim; dim++) {
-            const int nc = dim == 0 ? 5 : dim == 1 ? 4 : dim == 2 ? 5 : 2;
+            const int nc = dim == 0 ? 5 : dim == 1 ? 4 : dim == 2 ? 5 : dim == 3 ? 3 : 2;
             war_tuning bestt = cur;
             for (int i = 0; i < nc; i++) {
                 if (now_ms() - t_start > budget_ms) break;

@@ -947,16 +947,21 @@ __global__ __launch_bounds__(256) void k_refill(StageArgs<W> A) {
 }
 
 // ------------------------------------------------------------------------------------------
-// Flow kernel: the alternation detector without stages (one request, no cycle walks; modes full
-// and canon: --half needs even n, the detector odd n). Every lane
-// plays its own game, and a lane whose game is over takes the next position at the next
-// checkpoint, so a wave never waits for the slowest game of a group and no survivor records are
-// written. Positions enter only at checkpoints (the clock a multiple of ALT_K), so every lane's
-// age is a multiple of ALT_K at every checkpoint, as the detector needs; the age is clock - birth
-// and the cycle length clock - R, per lane. Positions come from tiles of the job list as in the
-// first stage: lane l of a tile owns positions l*per .. l*per+per-1 (stepped with next_perm); the
-// 64 lanes' current positions, dealt, are the pool that refills hand out in lane order (cross-lane
-// reads), and the pool steps every lane to its next position once all 64 are handed out.
+// Flow kernel: no stages (one request per batch, no cycle walks). Every lane plays its own game,
+// and a lane whose game is over takes the next position at the start of the next span of
+// FLOW_SPAN turns, so a wave never waits for the slowest game of a group and no survivor records
+// are written. Births fall on span starts (the clock a multiple of FLOW_SPAN); the age is clock -
+// birth and the cycle length clock - R, per lane. Because every age is a multiple of FLOW_SPAN at a
+// span start, each detector's tortoise moves fall on fixed turns of the span:
+// * Brent (DET 0, any n and rule): the tortoise takes the state at ages 2^k - 1 (after that turn's
+//   repeat test), exactly as the reference's war_sim (gpu/war_gpu.py:77-96), so `turns` is the
+//   reference's count. Age 2^k - 1 is turn 2^k - 2 of the lane's first span, and turn FLOW_SPAN - 2
+//   of a later span (when age + 1 is a power of 2).
+// * the alternation detector (DET 2, odd n, rule 0): checkpoints every ALT_K turns.
+// Positions come from tiles of the job list as in the first stage: lane l of a tile owns positions
+// l*per .. l*per+per-1 (stepped with next_perm); the 64 lanes' current positions, dealt, are the
+// pool that refills hand out in lane order (cross-lane reads), and the pool steps every lane to its
+// next position once all 64 are handed out.
 template <typename P>
 __device__ __forceinline__ P shfl_word(P v, u32 src) {
     if constexpr (sizeof(P) == 8) {
@@ -978,8 +983,12 @@ __device__ __forceinline__ St<W> shfl_state(const St<W> &x, u32 src) {
     return r;
 }
 
-template <int W>
+constexpr int FLOW_BRENT = 0, FLOW_ALT = 2;
+constexpr u32 FLOW_SPAN = ALT_K * FLOW_BLOCKS;
+
+template <int W, int RULE, int DET>
 __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
+    static_assert(DET == FLOW_BRENT || (DET == FLOW_ALT && RULE == 0), "the alternation detector is for rule 0");
     typedef typename Lay<W>::P P;
     constexpr int PW = Lay<W>::PW;
     extern __shared__ u32 lds[];
@@ -1010,10 +1019,15 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
         if (mode == 2) { if (++k == n - 1u) { k = 0; q = next_perm<PW>(q, (int)n - 1); } }
         else q = next_perm<PW>(q, (int)n);
     };
+    // --half (gpu/war_gpu.py:150-154) keeps the deals whose card n-1 goes to player 1
+    auto skip_filtered = [&]() {
+        if (mode == 1) while (left && !top_card_to_p1<PW>(q, (int)n)) { if (--left) step(); }
+    };
     // every lane to the next position of its run; a new tile once every run is done
     auto advance = [&]() -> bool {
         if (left > 1u) step();
         left = left ? left - 1u : 0u;
+        skip_filtered();
         while (ballot(left != 0u) == 0ull) {
             const u64 g0 = uni64(nxt);
             if (g0 >= ntiles) return false;
@@ -1034,6 +1048,7 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
                     q = unrank<PW>(first, (int)n);
                 }
             }
+            skip_filtered();
         }
         if (left) ps = (mode == 2) ? canon<W>(q, (int)n, (int)k) : deal<W>(q, (int)n);
         ptr = 0;
@@ -1042,14 +1057,13 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
 
     // ---- the games
     St<W> s = {};
-    AltLane<W> T_;
+    AltLane<W> T_;             // tortoise, its clock R; L: the alternation detector's timed wait
     T_.t = s; T_.R = 0; T_.L = ALT_L0;
     u32 birth = 0, clk = 0;
-    u64 alive = 0, prevm = 0, streakm = 0, endm = 0, cycm = 0;
-    // games end, are accounted and are replaced once per FLOW_SPAN turns (FLOW_BLOCKS checkpoints)
-    constexpr u32 SPAN = ALT_K * FLOW_BLOCKS;
-    constexpr int FB = __builtin_ctz(SPAN);
-    static_assert((SPAN & (SPAN - 1u)) == 0u && (AGE_CAP % SPAN) == 0u, "span: a power of 2 dividing AGE_CAP");
+    u64 alive = 0, newm = 0, prevm = 0, streakm = 0, endm = 0, cycm = 0;
+    constexpr int FB = __builtin_ctz(FLOW_SPAN);
+    static_assert((FLOW_SPAN & (FLOW_SPAN - 1u)) == 0u && (AGE_CAP % FLOW_SPAN) == 0u,
+                  "span: a power of 2 dividing AGE_CAP");
     u64 fb[FB];                // the turn of the span each finished game ended in, bit by bit
     for (int b = 0; b < FB; b++) fb[b] = 0;
     u64 lturns = 0;            // per lane: sum of its finished games' lengths / detection ages
@@ -1080,6 +1094,7 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
                 // a position that is over before it starts (only with fewer than 2 cards): length 0
                 if (n < 2u) { e0 = gm; acc.ended((u32)__popcll(e0), 0); }
                 alive |= gm & ~e0;
+                newm |= gm & ~e0;
                 prevm |= gm;   // before the first turn, player 1 counts as the last winner
             }
             if (alive || !more) return;
@@ -1104,36 +1119,55 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
     while (alive) {
         const u32 clk0 = clk;
 #pragma unroll
-        for (u32 blk = 0; blk < FLOW_BLOCKS; blk++) {
-            // one block of ALT_K turns up to the next checkpoint
-#pragma unroll
-            for (u32 u = 0; u < ALT_K; u++) {
-                const u32 tix = blk * ALT_K + u;       // turn of the span
-                const u64 pm = ballot(turn<W, 0>(s, C, z));
-                const u64 em = ballot(ended<W>(s, C)) & alive;
-                u64 dm = ballot(s.a == T_.t.a) & alive;
+        for (u32 tix = 0; tix < FLOW_SPAN; tix++) {       // turn tix of the span
+            const bool p1 = turn<W, RULE>(s, C, z);
+            const u64 em = ballot(ended<W>(s, C)) & alive;
+            u64 dm = ballot(s.a == T_.t.a) & alive;
+            if constexpr (DET == FLOW_ALT) {
+                const u64 pm = ballot(p1);
                 streakm |= ~(pm ^ prevm);
                 prevm = pm;
-                if (em | dm) {
-                    dm &= ~em;
-                    if (dm) dm &= rest_mask<W>(s, T_.t);
-                    const u64 ev = em | dm;
-                    for (int b = 0; b < FB; b++) if ((tix >> b) & 1u) fb[b] |= ev;
-                    endm |= em; cycm |= dm; alive &= ~ev;
-                }
+            } else {
+                (void)p1;
             }
-            clk += ALT_K;
-            // checkpoint: timed jumps (the wait doubles), then jumps after a repeated winner
-            const u64 ex = ballot(clk - T_.R >= T_.L) & alive;
-            const u64 js = (ex | streakm) & alive;
-            streakm = 0;
-            if (js) {
-                if (in_mask(js)) {
-                    T_.t = s; T_.R = clk;
-                    if (in_mask(ex)) T_.L <<= 1;
+            if (em | dm) {
+                dm &= ~em;
+                if (dm) dm &= rest_mask<W>(s, T_.t);
+                const u64 ev = em | dm;
+                for (int b = 0; b < FB; b++) if ((tix >> b) & 1u) fb[b] |= ev;
+                endm |= em; cycm |= dm; alive &= ~ev;
+            }
+            if constexpr (DET == FLOW_BRENT) {
+                // Brent's tortoise moves at ages 2^k - 1: turn 2^k - 2 of a lane's first span; turn
+                // FLOW_SPAN - 2 of a later span when age + 1 is a power of 2
+                if (((tix + 2u) & (tix + 1u)) == 0u) {           // folded: the loop is unrolled
+                    u64 jm = newm & alive;
+                    if (tix == FLOW_SPAN - 2u) {
+                        const u32 age1 = clk0 + tix + 2u - birth;    // age + 1
+                        jm |= ballot((age1 & (age1 - 1u)) == 0u) & alive;
+                    }
+                    if (jm) {
+                        if (in_mask(jm)) { T_.t = s; T_.R = clk0 + tix + 1u; }
+                    }
+                }
+            } else {
+                if ((tix + 1u) % ALT_K == 0u) {
+                    // checkpoint: timed jumps (the wait doubles), then jumps after a repeated winner
+                    const u32 now = clk0 + tix + 1u;
+                    const u64 ex = ballot(now - T_.R >= T_.L) & alive;
+                    const u64 js = (ex | streakm) & alive;
+                    streakm = 0;
+                    if (js) {
+                        if (in_mask(js)) {
+                            T_.t = s; T_.R = now;
+                            if (in_mask(ex)) T_.L <<= 1;
+                        }
+                    }
                 }
             }
         }
+        clk += FLOW_SPAN;
+        newm = 0;
         if (endm | cycm) settle(clk0);
         // the safety stop: a game still running at AGE_CAP turns is undetermined (never for a valid game)
         const u64 um = ballot(clk - birth >= AGE_CAP) & alive;

@@ -4,7 +4,8 @@
     python -m war_amd.profsum pmc    <results.db>      counters per kernel (summed over dispatches)
 
 A batch of the persistent pipeline starts with k_setup, so the trace view also lists, for the
-first and last batches, the duration of each stage kernel in launch order.
+first and last batches, the duration of each of its kernels (the flow kernel, or the stage
+kernels) in launch order.
 """
 import collections
 import sqlite3
@@ -31,10 +32,10 @@ def trace(db):
     for n, s, e in rows:
         if "k_setup" in n:
             batches.append([])
-        elif batches and ("k_stage" in n or "k_walk" in n or "k_plain" in n):
+        elif batches and ("k_stage" in n or "k_refill" in n or "k_flow" in n or "k_walk" in n or "k_plain" in n):
             batches[-1].append((e - s) / 1e6)
     if batches:
-        out += ["", "Per batch (ms, stage kernels in launch order):", ""]
+        out += ["", "Per batch (ms, the batch's kernels in launch order):", ""]
         show = list(enumerate(batches))
         if len(show) > 6:
             show = show[:3] + show[-3:]

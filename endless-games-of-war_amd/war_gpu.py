@@ -14,7 +14,8 @@ then the losing card under its own pile (--rule 1 reverses the order).
 
 Reference behaviour followed (gpu/war_gpu.py): run() :272-352 (progress keys :331-338, resume
 asserts :297,306, --half doubling :346-352), CLI :355-398 (asserts :380-390, output :384-398).
-Extra flags: --kernel plain|persistent (default persistent), --device D, --lbins B.
+Extra flags: --kernel plain|persistent (default persistent), --device D, --lbins B,
+--det auto|brent|alt (the cycle test; every printed number is the same either way).
 """
 import argparse
 import json
@@ -40,13 +41,23 @@ def engine(device=0):
     return _engine
 
 
+def detector_for(n, rule, kernel, det="auto"):
+    """The cycle test a run uses: "auto" takes the alternation-checkpoint detector where it applies
+    (odd n, rule 0, persistent kernel) and Brent elsewhere. Both are exact: every output of run()
+    is the same; only the engine's own turn count differs."""
+    if det != "auto":
+        return det
+    return "alt" if (n % 2 == 1 and rule == 0 and kernel == "persistent") else "brent"
+
+
 def run(n, half=False, per=256, threads=256, batch_threads=1 << 18, out=None, resume=False, quiet=False, hbins=0,
-        canon=False, shape=False, rule=0, dbins=0, kernel="persistent", lbins=0, device=0):
+        canon=False, shape=False, rule=0, dbins=0, kernel="persistent", lbins=0, device=0, det="auto"):
     """Same signature and return tuple as the reference's run() (gpu/war_gpu.py:272-352):
     (cycling, played, longest_finite, elapsed_s, hist{len: count}, shape (nonstd, unchecked, blk) or None,
     dhist{weight: count} or None). threads / batch_threads only shape the reference's launches and
-    are accepted for compatibility."""
+    are accepted for compatibility. det: the cycle test (detector_for)."""
     eng = engine(device)
+    detector = detector_for(n, rule, kernel, det)
     # canon: iterate over the (n-1)! orders of the other cards; each yields n-1 positions
     total = math.factorial(n - 1) if canon else math.factorial(n)
     unit = max(1, n - 1) if canon else 1               # positions per unit of `start`
@@ -89,7 +100,7 @@ def run(n, half=False, per=256, threads=256, batch_threads=1 << 18, out=None, re
             start = stop
             continue
         r = eng.run_range(n, mode, start * unit, stop * unit, rule=rule, kernel=kernel, hbins=hbins, lbins=lbins,
-                          dbins=dbins, shape_blk=blk, tuning=tuning)
+                          dbins=dbins, shape_blk=blk, tuning=tuning, detector=detector)
         cyc += r["cyc"]; done += r["done"]; maxfin = max(maxfin, r["maxfin"])
         for i, v in enumerate(r["hist"]):
             hist[i] += v
@@ -146,6 +157,8 @@ def main(argv=None):
     ap.add_argument("--kernel", choices=["plain", "persistent"], default="persistent")
     ap.add_argument("--device", type=int, default=0)
     ap.add_argument("--lbins", type=int, default=0, help="histogram of ending-game lengths (not printed)")
+    ap.add_argument("--det", choices=["auto", "brent", "alt"], default="auto",
+                    help="cycle test (both exact): auto = alt for odd n, rule 0, persistent kernel; else brent")
     a = ap.parse_args(argv)
     lo = a.nmin if a.nmin is not None else a.nmax
     for n in range(lo, a.nmax + 1):
@@ -153,7 +166,7 @@ def main(argv=None):
         c, d, mx, el, h, sh, dd = run(n, half=half, per=a.per, out=a.out, resume=a.resume,
                                       quiet=a.out is None, hbins=a.hist, canon=a.canon, shape=a.shape,
                                       rule=a.rule, dbins=4096 if a.distinct else 0, kernel=a.kernel,
-                                      lbins=a.lbins, device=a.device)
+                                      lbins=a.lbins, device=a.device, det=a.det)
         expect = math.factorial(n - 1) * (n - 1) if a.canon else math.factorial(n)
         assert d == expect, (d, expect)
         if h:

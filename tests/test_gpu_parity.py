@@ -79,6 +79,9 @@ def test_distinct_partial_ranges(eng, orc, n):
     {"brent_cap": 3, "ranks_per_refill": 8192, "waves_per_cu": 1},
     {"brent_cap": 15, "ranks_per_refill": 1024, "waves_per_cu": 32, "stage_kind": "lockstep"},
     {"brent_cap": 1, "ranks_per_refill": 256, "waves_per_cu": 8, "stage_kind": "lockstep"},
+    {"stage_kind": "flow", "ranks_per_refill": 64, "waves_per_cu": 4},
+    {"stage_kind": "flow", "ranks_per_refill": 8192, "waves_per_cu": 32},
+    {"stage_kind": "flow", "ranks_per_refill": 128, "waves_per_cu": 1},
 ])
 def test_schedules_are_exact(eng, orc, tuning):
     check(eng, orc, 13, "canon", 123_456_789, 123_456_789 + 2_000_000, tuning=tuning)
@@ -86,6 +89,29 @@ def test_schedules_are_exact(eng, orc, tuning):
     check(eng, orc, 8, "full", 0, None, rule=1, tuning=tuning)          # short games: sparse later stages
     check(eng, orc, 12, "canon", 7_000_000, 7_000_000 + 500_000, tuning=tuning, dbins=64, distinct=True,
           shape_blk=4, fields=FIELDS + ("oncycle", "distinct_minkey", "dhist", "shape_nonstd"))
+
+
+FLOW = {"stage_kind": "flow"}
+
+
+@pytest.mark.parametrize("n,mode,rule", [(n, m, r) for n in (2, 3, 4, 5, 6, 7, 8, 9, 10) for m in ("full", "canon", "half")
+                                         for r in (0, 1) if m != "half" or n % 2 == 0])
+def test_flow_brent_whole_ranges(eng, orc, n, mode, rule):
+    print("case flow brent", n, mode, rule, flush=True)
+    check(eng, orc, n, mode, 0, None, rule=rule, tuning=FLOW)
+
+
+def test_flow_brent_sampled(eng, orc):
+    rng = random.Random(11)
+    for n, mode, rule, lay in [(12, "half", 0, 0), (13, "full", 1, 0), (14, "canon", 0, 0), (14, "full", 0, 0),
+                               (15, "canon", 1, 0), (16, "full", 0, 0), (17, "full", 0, 6), (17, "full", 0, 5),
+                               (17, "canon", 1, 0), (18, "full", 0, 0), (20, "full", 1, 0)]:
+        total = orc.total(n, mode)
+        for ln in (1, 63, 150_001):
+            lo = rng.randrange(0, total - ln)
+            tuning = dict(FLOW, pile_layout=lay) if lay else FLOW
+            print("case flow brent", n, mode, rule, lay, lo, ln, flush=True)
+            check(eng, orc, n, mode, lo, lo + ln, rule=rule, tuning=tuning)
 
 
 def test_5bit_ranges(eng, orc):
