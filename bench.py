@@ -18,7 +18,8 @@ every step. One step = one whole enumeration. Beside it, on the same GPU in the 
                  loop) against CUs x 64 lanes x clock;
   cpu_baseline   the CPU oracle (OpenMP) on a bounded sample of the same n = 13 workload.
 
-  python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--only main|brent13|staged13|canon14|plain13|copy]
+  python bench.py [--gpus 1] [--steps 3] [--warmup 1]
+                  [--only main|brent13|staged13|canon14|canon14walk|plain13|copy]
                   [--no-extras]
 """
 import argparse
@@ -88,7 +89,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--only", choices=["main", "brent13", "staged13", "canon14", "plain13", "copy"], default=None,
+    ap.add_argument("--only", choices=["main", "brent13", "staged13", "canon14", "canon14walk", "plain13", "copy"], default=None,
                     help="run one workload only (for profiling); copy: a 1 GiB device-to-device copy that "
                          "calibrates the HBM byte counters")
     ap.add_argument("--no-extras", action="store_true")
@@ -143,6 +144,12 @@ def main():
         r = eng.run_range(14, "canon", hbins=256)
         assert r["cyc"] == C14["cyc"] and r["maxfin"] == C14["maxfin"] and hist_of(r) == C14["hist"]
         log("canon14 persistent device_ms %.1f" % r["device_ms"])
+        return
+    if a.only == "canon14walk":
+        r = eng.run_range(14, "canon", hbins=256, dbins=4096, distinct=True, shape_blk=2)
+        assert r["cyc"] == C14["cyc"] and r["maxfin"] == C14["maxfin"] and hist_of(r) == C14["hist"]
+        assert (r["distinct_minkey"], r["shape_nonstd"], r["oncycle"]) == (26640, 43669233, 496800)
+        log("canon14 with distinct + shape device_ms %.1f" % r["device_ms"])
         return
     if a.only == "plain13":
         r = eng.run_range(13, "full", kernel="plain", hbins=256)

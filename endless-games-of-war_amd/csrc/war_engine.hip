@@ -466,8 +466,17 @@ int build_graph(Engine &E, const GraphKey &key, hipGraphExec_t *out) {
         A.per = (u32)key.per; A.tile_pos = 64u * (u32)key.per;
         A.hb = (u32)key.hb; A.lb = (u32)key.lb;
         A.flow_min = (u32)key.cap0;
-        if (!key.alt) hipLaunchKernelGGL((k_flow<W, RULE, FLOW_BRENT>), dim3(blocks), dim3(256), lds, E.st, A);
-        else if constexpr (RULE == 0) hipLaunchKernelGGL((k_flow<W, 0, FLOW_ALT>), dim3(blocks), dim3(256), lds, E.st, A);
+        constexpr bool CW = W != 6;     // cycle walks exist for layouts 4 and 5
+        if (key.walk) {
+            A.cq = cyc_queue<W>(E);
+            if constexpr (CW) {
+                if (!key.alt) hipLaunchKernelGGL((k_flow<W, RULE, FLOW_BRENT, CW>), dim3(blocks), dim3(256), lds, E.st, A);
+                else if constexpr (RULE == 0) hipLaunchKernelGGL((k_flow<W, 0, FLOW_ALT, CW>), dim3(blocks), dim3(256), lds, E.st, A);
+            }
+        } else {
+            if (!key.alt) hipLaunchKernelGGL((k_flow<W, RULE, FLOW_BRENT, false>), dim3(blocks), dim3(256), lds, E.st, A);
+            else if constexpr (RULE == 0) hipLaunchKernelGGL((k_flow<W, 0, FLOW_ALT, false>), dim3(blocks), dim3(256), lds, E.st, A);
+        }
     }
     u32 a0 = 0;
     for (size_t s = 0; s < caps.size(); s++) {
@@ -516,10 +525,10 @@ int run_persistent_batch(Engine &E, std::vector<Job> &jobs, const war_tuning &t,
                          bool multi, float *ms, bool alt) {
     if (jobs.empty()) return 0;
     if ((int)jobs.size() > MAXJOBS) return fail(31, "too many jobs in one batch");
-    // the flow kernel (stage_kind 3) plays single-request batches without cycle walks; elsewhere
-    // stage_kind 3 runs lockstep stages
-    const bool flow = t.stage_kind == 3 && !walk && !multi && (RULE == 0 || !alt);
-    int rc = flow ? 0 : ensure_queues<W>(E, walk, alt);
+    // the flow kernel (stage_kind 3) plays single-request batches; elsewhere stage_kind 3 runs
+    // lockstep stages. Without cycle walks it needs no queue buffers.
+    const bool flow = t.stage_kind == 3 && !multi && (RULE == 0 || !alt);
+    int rc = (flow && !walk) ? 0 : ensure_queues<W>(E, walk, alt);
     if (rc) return rc;
     const u64 tile_pos = (u64)t.ranks_per_refill;
     u64 tiles = 0, npos = 0;
@@ -529,7 +538,7 @@ int run_persistent_batch(Engine &E, std::vector<Job> &jobs, const war_tuning &t,
     const int cap0 = flow ? FLOW_MIN_FREE : t.brent_cap;
     GraphKey key{W, RULE, 1, t.waves_per_cu, t.ranks_per_refill / 64, cap0, walk ? 1 : 0, multi ? 1 : 0,
                  round_pow2(hb), round_pow2(lb), kind, alt ? 1 : 0};
-    if (!flow && (tiles > qbufs<W>(E).tmax || tiles * tile_pos > qbufs<W>(E).cap))
+    if ((!flow || walk) && (tiles > qbufs<W>(E).tmax || tiles * tile_pos > qbufs<W>(E).cap))
         return fail(32, "batch exceeds the queue capacity");
     JobHdr h{};
     h.njobs = (u32)jobs.size(); h.req0 = jobs[0].req; h.ntiles = tiles;

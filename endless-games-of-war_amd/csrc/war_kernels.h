@@ -947,7 +947,7 @@ __global__ __launch_bounds__(256) void k_refill(StageArgs<W> A) {
 }
 
 // ------------------------------------------------------------------------------------------
-// Flow kernel: no stages (one request per batch, no cycle walks). Every lane plays its own game,
+// Flow kernel: no stages (one request per batch). Every lane plays its own game,
 // and a lane whose game is over takes the next position at the start of the next span of
 // FLOW_SPAN turns, so a wave never waits for the slowest game of a group and no survivor records
 // are written. Births fall on span starts (the clock a multiple of FLOW_SPAN); the age is clock -
@@ -961,7 +961,10 @@ __global__ __launch_bounds__(256) void k_refill(StageArgs<W> A) {
 // Positions come from tiles of the job list as in the first stage: lane l of a tile owns positions
 // l*per .. l*per+per-1 (stepped with next_perm); the 64 lanes' current positions, dealt, are the
 // pool that refills hand out in lane order (cross-lane reads), and the pool steps every lane to its
-// next position once all 64 are handed out.
+// next position once all 64 are handed out. With cycle walks (WALK), a repeat's meeting state (the
+// tortoise, which stays put once the game is over), cycle length and position index go to the
+// cycle records of the position's tile when the span is accounted, as in the staged pipeline, for
+// k_walk.
 template <typename P>
 __device__ __forceinline__ P shfl_word(P v, u32 src) {
     if constexpr (sizeof(P) == 8) {
@@ -986,9 +989,10 @@ __device__ __forceinline__ St<W> shfl_state(const St<W> &x, u32 src) {
 constexpr int FLOW_BRENT = 0, FLOW_ALT = 2;
 constexpr u32 FLOW_SPAN = ALT_K * FLOW_BLOCKS;
 
-template <int W, int RULE, int DET>
+template <int W, int RULE, int DET, bool WALK>
 __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
     static_assert(DET == FLOW_BRENT || (DET == FLOW_ALT && RULE == 0), "the alternation detector is for rule 0");
+    static_assert(!WALK || W != 6, "no cycle walks in layout 6");
     typedef typename Lay<W>::P P;
     constexpr int PW = Lay<W>::PW;
     extern __shared__ u32 lds[];
@@ -1000,6 +1004,7 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
     const u32 req = A.hdr->req0;
     acc.select(req, reqs);
     const u32 n = reqs[req].n, mode = reqs[req].mode;
+    const bool walk = WALK && reqs[req].walk != 0;   // cycles go to the cycle records for k_walk
     const u32 C = c_of<W>(n);
     const u32 per = A.per, TP = A.tile_pos;
     const u64 ntiles = A.hdr->ntiles;
@@ -1013,11 +1018,14 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
     P q = 0;
     u32 k = 0, left = 0;       // this lane's run: current order / split; positions left, it included
     St<W> ps = {};             // the current position, dealt
+    u64 pp = 0;                // its index (cycle walks rebuild the start from it)
+    u32 tile = 0;              // the pool's tile: the home of its positions' cycle records
     u32 ptr = 64;              // next pool entry to hand out (64: all handed out)
     bool more = true;          // positions remain (in the pool or in tiles not yet taken)
     auto step = [&]() {
         if (mode == 2) { if (++k == n - 1u) { k = 0; q = next_perm<PW>(q, (int)n - 1); } }
         else q = next_perm<PW>(q, (int)n);
+        if constexpr (WALK) pp++;
     };
     // --half (gpu/war_gpu.py:150-154) keeps the deals whose card n-1 goes to player 1
     auto skip_filtered = [&]() {
@@ -1048,6 +1056,7 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
                     q = unrank<PW>(first, (int)n);
                 }
             }
+            if constexpr (WALK) { pp = first; tile = (u32)g0; }
             skip_filtered();
         }
         if (left) ps = (mode == 2) ? canon<W>(q, (int)n, (int)k) : deal<W>(q, (int)n);
@@ -1060,6 +1069,8 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
     AltLane<W> T_;             // tortoise, its clock R; L: the alternation detector's timed wait
     T_.t = s; T_.R = 0; T_.L = ALT_L0;
     u32 birth = 0, clk = 0;
+    u64 gpos = 0;              // cycle walks: the game's position index and home tile
+    u32 ghome = 0;
     u64 alive = 0, newm = 0, prevm = 0, streakm = 0, endm = 0, cycm = 0;
     constexpr int FB = __builtin_ctz(FLOW_SPAN);
     static_assert((FLOW_SPAN & (FLOW_SPAN - 1u)) == 0u && (AGE_CAP % FLOW_SPAN) == 0u,
@@ -1082,6 +1093,8 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
                 const u32 src = (ptr + i) & 63u;
                 const St<W> ns = shfl_state<W>(ps, src);
                 const u32 lf = (u32)__shfl((int)left, (int)src);
+                u64 npos = 0;
+                if constexpr (WALK) npos = shfl_word(pp, src);
                 ptr += (u32)__popcll(tm);
                 dead &= ~tm;
                 const u64 gm = ballot(lf != 0u) & tm;               // entries that hold a position
@@ -1089,6 +1102,7 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
                     s = ns;
                     open_sa<W>(s, A.zero);
                     T_.t = s; T_.R = clk; T_.L = ALT_L0; birth = clk;
+                    if constexpr (WALK) { gpos = npos; ghome = tile; }
                 }
                 u64 e0 = 0;
                 // a position that is over before it starts (only with fewer than 2 cards): length 0
@@ -1111,6 +1125,10 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
             if (acc.lb) atomicAdd(&acc.ll[age < acc.lb - 1 ? age : acc.lb - 1], 1u);
         }
         if (acc.hb && in_mask(cycm)) atomicAdd(&acc.lh[lam < acc.hb - 1 ? lam : acc.hb - 1], 1u);
+        // cycle walks: the meeting state is the tortoise (a finished lane's tortoise stays put)
+        if constexpr (WALK) {
+            if (walk && cycm) acc.pushed += push_cycles<W>(A.cq, TP, cycm, T_.t, n, req, lam, gpos, ghome);
+        }
         acc.done += __popcll(endm | cycm); acc.cyc += __popcll(cycm); acc.dirty = true;
         endm = cycm = 0;
     };
@@ -1188,6 +1206,7 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
     acc.maxfin = tmax > acc.maxfin ? tmax : acc.maxfin;
     acc.dirty = true;
     acc.flush(reqs);
+    if (WALK && acc.pushed && lane == 0) atomicAdd((unsigned long long *)&A.pipe->ctotal, (unsigned long long)acc.pushed);
 }
 
 // ------------------------------------------------------------------------------------------
