@@ -628,6 +628,21 @@ int alloc_slot(Engine &E) {
     return s;
 }
 
+// One persistent batch [lo, hi) of a single request.
+int run_range_batch(Engine &E, ReqHost &h, int W, u64 lo, u64 hi, const war_tuning &t, double *dev) {
+    const war_request &r = h.rq;
+    std::vector<Job> jobs(1);
+    jobs[0].lo = lo; jobs[0].hi = hi; jobs[0].req = (u32)h.slot;
+    float ms = 0;
+    const bool alt = r.detector == WAR_DET_ALT;
+    int rc;
+    if (W == 4)      rc = run_batch_dispatch<4>(E, r.rule, jobs, t, r.hbins, r.lbins, h.dev.walk != 0, false, &ms, alt);
+    else if (W == 6) rc = run_batch_dispatch<6>(E, r.rule, jobs, t, r.hbins, r.lbins, false, false, &ms, alt);
+    else             rc = run_batch_dispatch<5>(E, r.rule, jobs, t, r.hbins, r.lbins, h.dev.walk != 0, false, &ms, alt);
+    *dev += ms;
+    return rc;
+}
+
 // Whole range of one request, synchronously (engine mutex held by the caller).
 int run_request_locked(Engine &E, ReqHost &h) {
     const war_request &r = h.rq;
@@ -654,20 +669,31 @@ int run_request_locked(Engine &E, ReqHost &h) {
             dev += ms;
         }
     } else {
-        // the flow kernel needs no queues, so its batches are larger (fewer batch ends)
-        const bool flow = t.stage_kind == 3 && !h.dev.walk;
-        const u64 bstep = flow ? FLOW_BATCH : E.batch;
-        for (u64 lo = r.lo; lo < r.hi && !rc; lo += bstep) {
-            const u64 hi = std::min<u64>(r.hi, lo + bstep);
-            std::vector<Job> jobs(1);
-            jobs[0].lo = lo; jobs[0].hi = hi; jobs[0].req = (u32)h.slot;
-            float ms = 0;
-            const bool alt = r.detector == WAR_DET_ALT;
-            if (W == 4)      rc = run_batch_dispatch<4>(E, r.rule, jobs, t, r.hbins, r.lbins, h.dev.walk != 0, false, &ms, alt);
-            else if (W == 6) rc = run_batch_dispatch<6>(E, r.rule, jobs, t, r.hbins, r.lbins, false, false, &ms, alt);
-            else             rc = run_batch_dispatch<5>(E, r.rule, jobs, t, r.hbins, r.lbins, h.dev.walk != 0, false, &ms, alt);
-            dev += ms;
+        const bool flow = t.stage_kind == 3;
+        war_tuning bt = t;
+        u64 bstep = E.batch;
+        u64 lo = r.lo;
+        if (flow) {
+            // the flow kernel needs no queues, so its batches are larger (fewer batch ends); with
+            // cycle walks the cycle records bound them as in the staged pipeline
+            if (!h.dev.walk) bstep = FLOW_BATCH;
+            if (r.mode == WAR_MODE_CANON && r.n >= 3) {
+                // canonical positions: lane runs of whole orders (a multiple of n-1 positions) that
+                // start at split 0, so every lane of a wave steps to its next order (next_perm) in
+                // the same pool step instead of a few lanes in almost every step
+                const u64 m = (u64)r.n - 1u;
+                const u64 per = std::max<u64>(1, ((u64)bt.ranks_per_refill / 64 + m / 2) / m) * m;
+                if (64 * per <= MAX_TILE) bt.ranks_per_refill = (int)(64 * per);
+                const u64 tp = (u64)bt.ranks_per_refill;
+                bstep = bstep / tp * tp;
+                const u64 a = (lo + m - 1) / m * m;          // the first order boundary
+                if (a > lo && a < r.hi) {                    // the part of an order before it
+                    rc = run_range_batch(E, h, W, lo, a, bt, &dev);
+                    lo = a;
+                }
+            }
         }
+        for (; lo < r.hi && !rc; lo += bstep) rc = run_range_batch(E, h, W, lo, std::min<u64>(r.hi, lo + bstep), bt, &dev);
     }
     h.dev_ms = dev;
     if (!rc) rc = req_collect(E, h);
