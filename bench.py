@@ -38,10 +38,11 @@ DET13 = "auto"     # set in main(): the cycle test war_gpu.py picks for n = 13 (
 PUBLISHED_N13_DEALS_PER_S = 6227020800 / 15.3      # RTX 5080, war_gpu.py 13 (reference data/checks.txt:23)
 A13 = {"cyc": 4880546113, "maxfin": 57, "hist": {42: 1475465743, 56: 2394985450, 70: 1010094920}}
 C14 = {"cyc": 2872129173, "maxfin": 465, "hist": {96: 43669233, 112: 2828459940}}
-# VALU instructions per War turn in the flow kernel's loop (18 for the turn, one end test, one
-# compare of player 1's pile with the tortoise's), counted in the gfx950 assembly of k_flow (the
-# same 20 as k_stage's age-synchronous loop); the algorithmic unit is one turn of the detector run.
-VALU_OPS_PER_TURN = 20
+# VALU instructions per War turn in the flow kernel's loop (18 for the turn, whose last subtraction
+# also yields the end test as its borrow, and one compare of player 1's pile with the tortoise's),
+# counted in the gfx950 assembly of k_flow (the same 19 as k_stage's age-synchronous loop); the
+# algorithmic unit is one turn of the detector run.
+VALU_OPS_PER_TURN = 19
 # HBM bytes per k_flow launch on the n = 13 workload (2 launches; no records are written: 0.31 MB
 # read, 98.9 MB of tile-grab atomics and counters), from the rocprofv3 byte counters calibrated on a
 # copy of known size.

@@ -124,8 +124,9 @@ __device__ __forceinline__ bool turn(St<W> &s, u32 C, Zh<W> &z) {
 // Otherwise both play their words' top cards and the holder's card n-1 moves up one. The holder
 // never runs out, so the game is over iff the other player is out: sa > C. Returns true if the
 // holder won.
+// sbo = C - s.sa, the other player's append offset.
 template <int RULE>
-__device__ __forceinline__ bool turn_held(St<6> &s, u32 C, Zh<6> &z) {
+__device__ __forceinline__ bool turn_held_at(St<6> &s, u32 sbo, Zh<6> &z) {
     const u32 ch = (u32)s.a & 15u, co = (u32)s.b & 15u;
     const bool top = s.pb == 0u;
     const bool hw = top | (ch > co);
@@ -140,15 +141,52 @@ __device__ __forceinline__ bool turn_held(St<6> &s, u32 C, Zh<6> &z) {
     z.h1 = (z.h1 & 0xFFFFFFFF00000000ull) | xh;
     z.h2 = (z.h2 & 0xFFFFFFFF00000000ull) | xo;
     s.a |= z.h1 << (hoff & 63u);
-    s.b |= z.h2 << ((C - s.sa) & 63u);
+    s.b |= z.h2 << (sbo & 63u);
     // card n-1 went under: rule 0 puts it before the captured card (depth = cards in the word
     // before the capture), rule 1 after it
     s.pb = top ? (RULE ? s.sa + 4u : s.sa) : s.pb - 4u;
     s.sa += hw ? 4u : (u32)-4;
     return hw;
 }
+template <int RULE>
+__device__ __forceinline__ bool turn_held(St<6> &s, u32 C, Zh<6> &z) { return turn_held_at<RULE>(s, C - s.sa, z); }
 
 template <int W> __device__ __forceinline__ bool ended(const St<W> &s, u32 C) { return s.sa > C; }
+
+// turn() for loops that carry sb = C - sa from turn to turn (player 2's append offset): the
+// subtraction that makes the next turn's sb borrows exactly when sa > C, so its borrow is the end
+// test and the turn needs no compare of its own for it (19 VALU instead of 20 for 4-bit piles).
+// `over` receives the wave's lane mask of games that ended (bits of inactive lanes are 0). The
+// subtraction is written as one instruction so that its borrow lands in a scalar register pair as
+// the mask itself (the compiler would otherwise copy the borrow through a vector register).
+template <int W, int RULE>
+__device__ __forceinline__ bool turn_sb(St<W> &s, u32 &sb, u32 C, Zh<W> &z, u64 &over) {
+    if constexpr (W == 4) {
+        const u32 c1 = (u32)s.a & 15u, c2 = (u32)s.b & 15u;
+        s.a >>= 4;
+        s.b >>= 4;
+        const bool p1 = c1 > c2;
+        const u32 v1 = RULE ? (c2 | (c1 << 4)) : (c1 | (c2 << 4));
+        const u32 v2 = RULE ? (c1 | (c2 << 4)) : (c2 | (c1 << 4));
+        const u32 x1 = p1 ? v1 : 0u, x2 = p1 ? 0u : v2;
+        z.h1 = (z.h1 & 0xFFFFFFFF00000000ull) | x1;
+        z.h2 = (z.h2 & 0xFFFFFFFF00000000ull) | x2;
+        s.a |= z.h1 << (s.sa & 63u);
+        s.b |= z.h2 << (sb & 63u);
+        s.sa += p1 ? 4u : (u32)-4;
+        asm("v_sub_co_u32 %0, %1, %2, %3" : "=v"(sb), "=s"(over) : "s"(C), "v"(s.sa));
+        return p1;
+    } else if constexpr (W == 6) {
+        const bool hw = turn_held_at<RULE>(s, sb, z);
+        asm("v_sub_co_u32 %0, %1, %2, %3" : "=v"(sb), "=s"(over) : "s"(C), "v"(s.sa));
+        return hw;
+    } else {
+        const bool p1 = turn<W, RULE>(s, C, z);
+        over = __builtin_amdgcn_ballot_w64(ended<W>(s, C));
+        sb = C - s.sa;
+        return p1;
+    }
+}
 
 // The repeat test compares (w1, w2, l1): card 0 and an empty slot look alike (gpu/war_gpu.py:91).
 // Layout 6 adds the depth of card n-1.

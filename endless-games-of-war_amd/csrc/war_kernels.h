@@ -336,6 +336,7 @@ __device__ __forceinline__ u64 play(const StageArgs<W> &A, WaveAcc &acc, St<W> &
     const ReqDev *reqs = A.reqs;
     const bool last = (A.a1 == UNBOUNDED);
     open_sa<W>(s, A.zero);
+    u32 sb = C - s.sa;          // player 2's append offset, carried from turn to turn
     St<W> t = s;
     u32 R = A.a0;
     u32 age = A.a0;
@@ -350,8 +351,9 @@ __device__ __forceinline__ u64 play(const StageArgs<W> &A, WaveAcc &acc, St<W> &
             u64 em, dm;
             do {
                 ++age;
-                turn<W, RULE>(s, C, z);
-                em = ballot(ended<W>(s, C)) & alive;
+                u64 over;
+                turn_sb<W, RULE>(s, sb, C, z, over);
+                em = over & alive;
                 dm = ballot(s.a == t.a) & alive;
             } while (!(em | dm) && age < wend);
             dm &= ~em;
@@ -424,6 +426,7 @@ __device__ __forceinline__ u64 play_alt(const StageArgs<W> &A, WaveAcc &acc, St<
     const bool last = (A.a1 == UNBOUNDED);
     open_sa<W>(s, A.zero);
     open_sa<W>(T_.t, A.zero);
+    u32 sb = C - s.sa;                     // player 2's append offset, carried from turn to turn
     u32 age = A.a0;
     const u32 a1 = last ? AGE_CAP : A.a1;
     Zh<W> z = make_zh<W>(A.zero);
@@ -454,8 +457,9 @@ __device__ __forceinline__ u64 play_alt(const StageArgs<W> &A, WaveAcc &acc, St<
 #define WAR_ALT_TURN()                                                                     \
     {                                                                                      \
         ++age;                                                                             \
-        const u64 pm = ballot(turn<W, RULE>(s, C, z));                                     \
-        const u64 em = ballot(ended<W>(s, C)) & alive;                                     \
+        u64 over;                                                                          \
+        const u64 pm = ballot(turn_sb<W, RULE>(s, sb, C, z, over));                        \
+        const u64 em = over & alive;                                                       \
         u64 dm = ballot(s.a == T_.t.a) & alive;                                            \
         streakm |= ~(pm ^ prevm);                                                          \
         prevm = pm;                                                                        \
@@ -1066,6 +1070,7 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
 
     // ---- the games
     St<W> s = {};
+    u32 sb = C;                // C - s.sa: player 2's append offset, carried from turn to turn
     AltLane<W> T_;             // tortoise, its clock R; L: the alternation detector's timed wait
     T_.t = s; T_.R = 0; T_.L = ALT_L0;
     u32 birth = 0, clk = 0;
@@ -1101,6 +1106,7 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
                 if (in_mask(gm)) {
                     s = ns;
                     open_sa<W>(s, A.zero);
+                    sb = C - s.sa;
                     T_.t = s; T_.R = clk; T_.L = ALT_L0; birth = clk;
                     if constexpr (WALK) { gpos = npos; ghome = tile; }
                 }
@@ -1138,8 +1144,9 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
         const u32 clk0 = clk;
 #pragma unroll
         for (u32 tix = 0; tix < FLOW_SPAN; tix++) {       // turn tix of the span
-            const bool p1 = turn<W, RULE>(s, C, z);
-            const u64 em = ballot(ended<W>(s, C)) & alive;
+            u64 over;
+            const bool p1 = turn_sb<W, RULE>(s, sb, C, z, over);
+            const u64 em = over & alive;
             u64 dm = ballot(s.a == T_.t.a) & alive;
             if constexpr (DET == FLOW_ALT) {
                 const u64 pm = ballot(p1);
