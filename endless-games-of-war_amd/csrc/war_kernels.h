@@ -1063,7 +1063,10 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
             if constexpr (WALK) { pp = first; tile = (u32)g0; }
             skip_filtered();
         }
-        if (left) ps = (mode == 2) ? canon<W>(q, (int)n, (int)k) : deal<W>(q, (int)n);
+        if (left) {
+            ps = (mode == 2) ? canon<W>(q, (int)n, (int)k) : deal<W>(q, (int)n);
+            if constexpr (W == 4) ps = flip_piles(ps);     // 4-bit games play in turn_top's form
+        }
         ptr = 0;
         return true;
     };
@@ -1085,6 +1088,7 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
     u64 lturns = 0;            // per lane: sum of its finished games' lengths / detection ages
     u32 lmax = 0;              // per lane: longest ending game
     Zh<W> z = make_zh<W>(A.zero);
+    Zt zt = make_zt(A.zero);
 
     // free lanes take pool entries in lane order until every lane plays or no position is left
     // (every lane calls this)
@@ -1131,9 +1135,14 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
             if (acc.lb) atomicAdd(&acc.ll[age < acc.lb - 1 ? age : acc.lb - 1], 1u);
         }
         if (acc.hb && in_mask(cycm)) atomicAdd(&acc.lh[lam < acc.hb - 1 ? lam : acc.hb - 1], 1u);
-        // cycle walks: the meeting state is the tortoise (a finished lane's tortoise stays put)
+        // cycle walks: the meeting state is the tortoise (a finished lane's tortoise stays put),
+        // recorded in the usual pile form
         if constexpr (WALK) {
-            if (walk && cycm) acc.pushed += push_cycles<W>(A.cq, TP, cycm, T_.t, n, req, lam, gpos, ghome);
+            if (walk && cycm) {
+                St<W> m = T_.t;
+                if constexpr (W == 4) m = flip_piles(m);
+                acc.pushed += push_cycles<W>(A.cq, TP, cycm, m, n, req, lam, gpos, ghome);
+            }
         }
         acc.done += __popcll(endm | cycm); acc.cyc += __popcll(cycm); acc.dirty = true;
         endm = cycm = 0;
@@ -1145,7 +1154,9 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
 #pragma unroll
         for (u32 tix = 0; tix < FLOW_SPAN; tix++) {       // turn tix of the span
             u64 over;
-            const bool p1 = turn_sb<W, RULE>(s, sb, C, z, over);
+            bool p1;
+            if constexpr (W == 4) p1 = turn_top<RULE>(s, sb, C, zt, over);
+            else p1 = turn_sb<W, RULE>(s, sb, C, z, over);
             const u64 em = over & alive;
             u64 dm = ballot(s.a == T_.t.a) & alive;
             if constexpr (DET == FLOW_ALT) {
