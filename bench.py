@@ -14,8 +14,10 @@ every step. One step = one whole enumeration. Beside it, on the same GPU in the 
                  (4-bit piles, card 16 apart) and 5-bit piles, Brent and the alternation detector;
   serving        a continuous-batching run of mixed queries with p50/p99 latency, beside the same
                  queries one by one through the plain and the persistent kernels;
-  roofline       integer VALU: useful lane-ops/s (turns x VALU ops per turn of the flow kernel's
-                 loop) against CUs x 64 lanes x clock;
+  roofline       integer VALU: useful lane-ops/s (turns x VALU instructions per turn of the flow
+                 kernel's loop x 64 lanes) against the VALU's dual-issue peak, CUs x 128 lane-ops
+                 per clock, with the fraction of the single-issue rate and of the turn's own
+                 instruction-mix ceiling beside it;
   cpu_baseline   the CPU oracle (OpenMP) on a bounded sample of the same n = 13 workload.
 
   python bench.py [--gpus 1] [--steps 3] [--warmup 1]
@@ -38,11 +40,21 @@ DET13 = "auto"     # set in main(): the cycle test war_gpu.py picks for n = 13 (
 PUBLISHED_N13_DEALS_PER_S = 6227020800 / 15.3      # RTX 5080, war_gpu.py 13 (reference data/checks.txt:23)
 A13 = {"cyc": 4880546113, "maxfin": 57, "hist": {42: 1475465743, 56: 2394985450, 70: 1010094920}}
 C14 = {"cyc": 2872129173, "maxfin": 465, "hist": {96: 43669233, 112: 2828459940}}
-# VALU instructions per War turn in the flow kernel's loop (18 for the turn, whose last subtraction
-# also yields the end test as its borrow, and one compare of player 1's pile with the tortoise's),
-# counted in the gfx950 assembly of k_flow (the same 19 as k_stage's age-synchronous loop); the
+# VALU instructions per War turn in the flow kernel's loop: 16 for the turn (its last subtraction
+# also yields the end test as its borrow) and one compare of player 1's pile with the tortoise's,
+# counted in the gfx950 assembly of k_flow (k_stage's age-synchronous loop also has 17); the
 # algorithmic unit is one turn of the detector run.
-VALU_OPS_PER_TURN = 19
+VALU_OPS_PER_TURN = 17
+# VALU issue on gfx950, measured with an issue-rate probe at 7-8 waves per SIMD: VOP2 v_add_u32,
+# v_and_b32 and v_or_b32 issue two wave64 instructions per SIMD per 4 clocks (dual issue); 64-bit
+# shifts, adds and compares, VOP3 forms, selects and compares issue one per 4 clocks. The peak is
+# therefore 2 x 64 lanes per SIMD per 4 clocks = CUs x 128 lane-ops per clock (78.6 T/s at 256 CUs
+# and 2.4 GHz, the rate the published 157.3 TFLOP/s FP32 vector figure implies). Of the turn's 17
+# instructions 5 can dual-issue (2 ANDs, 2 ORs, the length add) and 12 cannot, so the turn itself
+# needs at least 12 x 4 + 5 x 2 = 58 clocks per wave: its instruction mix caps VALU issue at
+# 17 x 4 / 58 = 1.17 instructions per SIMD per 4 clocks, 0.586 of the dual-issue peak.
+VALU_LANE_OPS_PER_CU_CLOCK = 128
+TURN_MIX_CEILING = (VALU_OPS_PER_TURN * 4.0 / (12 * 4 + 5 * 2)) / 2.0
 # HBM bytes per k_flow launch on the n = 13 workload (2 launches; no records are written: 0.31 MB
 # read, 98.9 MB of tile-grab atomics and counters), from the rocprofv3 byte counters calibrated on a
 # copy of known size.
@@ -171,7 +183,7 @@ def main():
         turns += r["turns"]
     wall = time.perf_counter() - t0
     value = n13 * a.steps / wall
-    peak_lane_ops = dev["cus"] * 64 * dev["max_clock_mhz"] * 1e6
+    peak_lane_ops = dev["cus"] * VALU_LANE_OPS_PER_CU_CLOCK * dev["max_clock_mhz"] * 1e6
     achieved = turns * VALU_OPS_PER_TURN / (dev_ms / 1e3)
     out = {
         "metric": METRIC, "value": value, "unit": "deals/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -184,6 +196,11 @@ def main():
                    "parity": "a(13), longest 57, cycle-length histogram equal the published values every step"},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": peak_lane_ops / 1e12,
                      "unit": "Tlane-op/s", "frac": achieved / peak_lane_ops, "traffic": K_FLOW_HBM_BYTES_PER_LAUNCH,
+                     "peak_note": "VALU dual-issue peak: 2 wave64 instructions per SIMD per 4 clocks (measured for "
+                                  "v_add/v_and/v_or; every other form in the turn issues one per 4 clocks)",
+                     "frac_single_issue": 2.0 * achieved / peak_lane_ops,
+                     "turn_mix_ceiling": TURN_MIX_CEILING,
+                     "frac_of_turn_mix_ceiling": achieved / peak_lane_ops / TURN_MIX_CEILING,
                      "kernel": "k_flow<4, 0, alt>: one launch per batch of 2^32 deals",
                      "traffic_note": "HBM bytes per k_flow launch at this workload from rocprofv3 FETCH_SIZE and "
                                      "WRITE_SIZE (scaled by a 1 GiB copy: x2.0 and x1.0), profiles/"
