@@ -1111,7 +1111,11 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
                     s = ns;
                     open_sa<W>(s, A.zero);
                     sb = C - s.sa;
-                    T_.t = s; T_.R = clk; T_.L = ALT_L0; birth = clk;
+                    birth = clk;
+                    // Brent's first tortoise (the start) needs no copy: no state repeats after one
+                    // turn (player 1's card count changes every turn), so the repeat test of a new
+                    // game's first turn is masked off, and the tortoise takes the state after it
+                    if constexpr (DET == FLOW_ALT) { T_.t = s; T_.R = clk; T_.L = ALT_L0; }
                     if constexpr (WALK) { gpos = npos; ghome = tile; }
                 }
                 u64 e0 = 0;
@@ -1157,8 +1161,12 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
             bool p1;
             if constexpr (W == 4) p1 = turn_top<RULE>(s, sb, C, zt, over);
             else p1 = turn_sb<W, RULE>(s, sb, C, z, over);
-            const u64 em = over & alive;
-            u64 dm = ballot(s.a == T_.t.a) & alive;
+            // the common case (no lane ended or matched the tortoise's player-1 pile) costs two
+            // scalar operations and the branch; the scalar unit is busy about 70 % of the time here
+            u64 cand = ballot(s.a == T_.t.a);
+            if constexpr (DET == FLOW_BRENT) {
+                if (tix == 0u) cand &= ~newm;               // a new game's first turn: no repeat
+            }
             if constexpr (DET == FLOW_ALT) {
                 const u64 pm = ballot(p1);
                 streakm |= ~(pm ^ prevm);
@@ -1166,8 +1174,9 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
             } else {
                 (void)p1;
             }
-            if (em | dm) {
-                dm &= ~em;
+            if ((over | cand) & alive) {
+                const u64 em = over & alive;
+                u64 dm = cand & alive & ~em;
                 if (dm) dm &= rest_mask<W>(s, T_.t);
                 const u64 ev = em | dm;
                 for (int b = 0; b < FB; b++) if ((tix >> b) & 1u) fb[b] |= ev;
@@ -1205,11 +1214,14 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
         clk += FLOW_SPAN;
         newm = 0;
         if (endm | cycm) settle(clk0);
-        // the safety stop: a game still running at AGE_CAP turns is undetermined (never for a valid game)
-        const u64 um = ballot(clk - birth >= AGE_CAP) & alive;
-        if (um) {
-            acc.undet += __popcll(um); acc.done += __popcll(um); acc.dirty = true;
-            alive &= ~um;
+        // the safety stop, tested every 1024 turns: a game still running after AGE_CAP turns is
+        // undetermined (never for a valid game; its turns are not counted)
+        if ((clk & 1023u) == 0u) {
+            const u64 um = ballot(clk - birth >= AGE_CAP) & alive;
+            if (um) {
+                acc.undet += __popcll(um); acc.done += __popcll(um); acc.dirty = true;
+                alive &= ~um;
+            }
         }
         if (more && (u32)__popcll(~alive) >= min_dead) refill();
     }
