@@ -113,10 +113,10 @@ constexpr u64 FLOW_BATCH = 1ull << 32;  // flow kernel: positions per batch
 constexpr u32 PLAIN_CYC_TILE = 1024;    // home-tile size of the plain kernel's cycle records
 
 struct GraphKey {
-    int W, rule, kernel, wpc, per, cap0, walk, multi, hb, lb, kind, alt;
+    int W, rule, kernel, wpc, per, cap0, walk, multi, hb, lb, kind, det;   // det: WAR_DET_* of the persistent path
     bool operator<(const GraphKey &o) const {
-        return std::tie(W, rule, kernel, wpc, per, cap0, walk, multi, hb, lb, kind, alt) <
-               std::tie(o.W, o.rule, o.kernel, o.wpc, o.per, o.cap0, o.walk, o.multi, o.hb, o.lb, o.kind, o.alt);
+        return std::tie(W, rule, kernel, wpc, per, cap0, walk, multi, hb, lb, kind, det) <
+               std::tie(o.W, o.rule, o.kernel, o.wpc, o.per, o.cap0, o.walk, o.multi, o.hb, o.lb, o.kind, o.det);
     }
 };
 
@@ -171,8 +171,14 @@ int check_request(const war_request *r) {
         return fail(9, "distinct / dbins need canon mode and n <= 15");
     if (r->shape_blk && r->mode != WAR_MODE_CANON) return fail(10, "shape needs canon mode");
     if (r->shape_blk < 0 || r->shape_blk > 512) return fail(10, "shape_blk must be in [0, 512]");
-    if (r->detector != WAR_DET_BRENT && r->detector != WAR_DET_STREAK && r->detector != WAR_DET_ALT)
+    if (r->detector != WAR_DET_BRENT && r->detector != WAR_DET_STREAK && r->detector != WAR_DET_ALT &&
+        r->detector != WAR_DET_PAIR)
         return fail(12, "unknown detector");
+    if (r->detector == WAR_DET_PAIR) {
+        if (r->n % 4 != 2 || r->rule != 0)
+            return fail(12, "the pair-run detector needs n = 2 mod 4 and rule 0 (runs of two: the loop-shape test)");
+        if (r->kernel != WAR_KERNEL_PERSISTENT) return fail(12, "the pair-run detector runs in the persistent kernel");
+    }
     if (r->detector == WAR_DET_ALT && ((r->n & 1) == 0 || r->rule != 0))
         return fail(12, "the alternation detector needs odd n and rule 0 (odd-n cycles alternate: Spivey Thm 6)");
     if (r->detector == WAR_DET_STREAK) {
@@ -456,7 +462,7 @@ int build_graph(Engine &E, const GraphKey &key, hipGraphExec_t *out) {
     const int blocks = std::max(1, E.cus * key.wpc / 4);
     const size_t lds = (size_t)4 * (key.hb + key.lb) * sizeof(u32);
     const std::vector<u32> caps = key.kind == 3 ? std::vector<u32>{}
-                                  : key.alt ? stage_caps_alt(key.cap0) : stage_caps(key.cap0);
+                                  : key.det == WAR_DET_ALT ? stage_caps_alt(key.cap0) : stage_caps(key.cap0);
     HIPCHK(hipStreamBeginCapture(E.st, hipStreamCaptureModeThreadLocal));
     hipLaunchKernelGGL(k_setup, dim3(256), dim3(256), 0, E.st, E.pipe, (const JobHdr *)E.hdr,
                        key.walk ? Q.ccnt : (u32 *)nullptr);
@@ -470,12 +476,18 @@ int build_graph(Engine &E, const GraphKey &key, hipGraphExec_t *out) {
         if (key.walk) {
             A.cq = cyc_queue<W>(E);
             if constexpr (CW) {
-                if (!key.alt) hipLaunchKernelGGL((k_flow<W, RULE, FLOW_BRENT, CW>), dim3(blocks), dim3(256), lds, E.st, A);
-                else if constexpr (RULE == 0) hipLaunchKernelGGL((k_flow<W, 0, FLOW_ALT, CW>), dim3(blocks), dim3(256), lds, E.st, A);
+                if (key.det == WAR_DET_BRENT) hipLaunchKernelGGL((k_flow<W, RULE, FLOW_BRENT, CW>), dim3(blocks), dim3(256), lds, E.st, A);
+                else if constexpr (RULE == 0) {
+                    if (key.det == WAR_DET_ALT) hipLaunchKernelGGL((k_flow<W, 0, FLOW_ALT, CW>), dim3(blocks), dim3(256), lds, E.st, A);
+                    else hipLaunchKernelGGL((k_flow<W, 0, FLOW_PAIR, CW>), dim3(blocks), dim3(256), lds, E.st, A);
+                }
             }
         } else {
-            if (!key.alt) hipLaunchKernelGGL((k_flow<W, RULE, FLOW_BRENT, false>), dim3(blocks), dim3(256), lds, E.st, A);
-            else if constexpr (RULE == 0) hipLaunchKernelGGL((k_flow<W, 0, FLOW_ALT, false>), dim3(blocks), dim3(256), lds, E.st, A);
+            if (key.det == WAR_DET_BRENT) hipLaunchKernelGGL((k_flow<W, RULE, FLOW_BRENT, false>), dim3(blocks), dim3(256), lds, E.st, A);
+            else if constexpr (RULE == 0) {
+                if (key.det == WAR_DET_ALT) hipLaunchKernelGGL((k_flow<W, 0, FLOW_ALT, false>), dim3(blocks), dim3(256), lds, E.st, A);
+                else if constexpr (W != 6) hipLaunchKernelGGL((k_flow<W, 0, FLOW_PAIR, false>), dim3(blocks), dim3(256), lds, E.st, A);
+            }
         }
     }
     u32 a0 = 0;
@@ -494,11 +506,11 @@ int build_graph(Engine &E, const GraphKey &key, hipGraphExec_t *out) {
         A.a0 = a0; A.a1 = caps[s]; A.stage = (int)s;
         A.hb = (u32)key.hb; A.lb = (u32)key.lb;
         A.max_regions = Q.cap / A.tile_pos;
-        const int flags = (key.multi ? F_MULTI : 0) | (key.walk ? F_WALK : 0) | (key.alt ? F_ALT : 0);
+        const int flags = (key.multi ? F_MULTI : 0) | (key.walk ? F_WALK : 0) | (key.det == WAR_DET_ALT ? F_ALT : 0);
         // refill stages need a bounded window (no tortoise move inside); the unbounded last stage
         // and the first (enumerating) stage are lockstep
-        const bool refill = W != 6 && !key.alt && key.kind == 1 && s > 0 && caps[s] != UNBOUNDED;
-        A.in_refill = (W != 6 && !key.alt && key.kind == 1 && s > 1) ? 1u : 0u;
+        const bool refill = W != 6 && key.det != WAR_DET_ALT && key.kind == 1 && s > 0 && caps[s] != UNBOUNDED;
+        A.in_refill = (W != 6 && key.det != WAR_DET_ALT && key.kind == 1 && s > 1) ? 1u : 0u;
         if (refill) launch_refill<W, RULE>(flags, blocks, lds, E.st, A);
         else launch_stage<W, RULE>(s == 0, flags, blocks, lds, E.st, A);
         a0 = caps[s];
@@ -522,12 +534,15 @@ int build_graph(Engine &E, const GraphKey &key, hipGraphExec_t *out) {
 // One batch of jobs (all of layout W and rule RULE) through the persistent pipeline.
 template <int W, int RULE>
 int run_persistent_batch(Engine &E, std::vector<Job> &jobs, const war_tuning &t, int hb, int lb, bool walk,
-                         bool multi, float *ms, bool alt) {
+                         bool multi, float *ms, int det) {
     if (jobs.empty()) return 0;
     if ((int)jobs.size() > MAXJOBS) return fail(31, "too many jobs in one batch");
+    const bool alt = det == WAR_DET_ALT;
     // the flow kernel (stage_kind 3) plays single-request batches; elsewhere stage_kind 3 runs
     // lockstep stages. Without cycle walks it needs no queue buffers.
-    const bool flow = t.stage_kind == 3 && !multi && (RULE == 0 || !alt);
+    const bool flow = t.stage_kind == 3 && !multi && (RULE == 0 || det == WAR_DET_BRENT);
+    if (det == WAR_DET_PAIR && !flow)
+        return fail(12, "the pair-run detector runs on the flow kernel (stage_kind 3, single requests)");
     int rc = (flow && !walk) ? 0 : ensure_queues<W>(E, walk, alt);
     if (rc) return rc;
     const u64 tile_pos = (u64)t.ranks_per_refill;
@@ -537,7 +552,7 @@ int run_persistent_batch(Engine &E, std::vector<Job> &jobs, const war_tuning &t,
     // flow: cap0 holds the refill threshold (free lanes of 64), brent_cap's stage windows unused
     const int cap0 = flow ? FLOW_MIN_FREE : t.brent_cap;
     GraphKey key{W, RULE, 1, t.waves_per_cu, t.ranks_per_refill / 64, cap0, walk ? 1 : 0, multi ? 1 : 0,
-                 round_pow2(hb), round_pow2(lb), kind, alt ? 1 : 0};
+                 round_pow2(hb), round_pow2(lb), kind, det};
     if ((!flow || walk) && (tiles > qbufs<W>(E).tmax || tiles * tile_pos > qbufs<W>(E).cap))
         return fail(32, "batch exceeds the queue capacity");
     JobHdr h{};
@@ -616,9 +631,9 @@ int run_plain_batch(Engine &E, ReqHost &h, u64 lo, u64 hi, const war_tuning &t, 
 
 template <int W>
 int run_batch_dispatch(Engine &E, int rule, std::vector<Job> &jobs, const war_tuning &t, int hb, int lb, bool walk,
-                       bool multi, float *ms, bool alt = false) {
-    return rule ? run_persistent_batch<W, 1>(E, jobs, t, hb, lb, walk, multi, ms, false)
-                : run_persistent_batch<W, 0>(E, jobs, t, hb, lb, walk, multi, ms, alt);
+                       bool multi, float *ms, int det = WAR_DET_BRENT) {
+    return rule ? run_persistent_batch<W, 1>(E, jobs, t, hb, lb, walk, multi, ms, WAR_DET_BRENT)
+                : run_persistent_batch<W, 0>(E, jobs, t, hb, lb, walk, multi, ms, det);
 }
 
 int alloc_slot(Engine &E) {
@@ -634,11 +649,11 @@ int run_range_batch(Engine &E, ReqHost &h, int W, u64 lo, u64 hi, const war_tuni
     std::vector<Job> jobs(1);
     jobs[0].lo = lo; jobs[0].hi = hi; jobs[0].req = (u32)h.slot;
     float ms = 0;
-    const bool alt = r.detector == WAR_DET_ALT;
+    const int det = r.detector;
     int rc;
-    if (W == 4)      rc = run_batch_dispatch<4>(E, r.rule, jobs, t, r.hbins, r.lbins, h.dev.walk != 0, false, &ms, alt);
-    else if (W == 6) rc = run_batch_dispatch<6>(E, r.rule, jobs, t, r.hbins, r.lbins, false, false, &ms, alt);
-    else             rc = run_batch_dispatch<5>(E, r.rule, jobs, t, r.hbins, r.lbins, h.dev.walk != 0, false, &ms, alt);
+    if (W == 4)      rc = run_batch_dispatch<4>(E, r.rule, jobs, t, r.hbins, r.lbins, h.dev.walk != 0, false, &ms, det);
+    else if (W == 6) rc = run_batch_dispatch<6>(E, r.rule, jobs, t, r.hbins, r.lbins, false, false, &ms, det);
+    else             rc = run_batch_dispatch<5>(E, r.rule, jobs, t, r.hbins, r.lbins, h.dev.walk != 0, false, &ms, det);
     *dev += ms;
     return rc;
 }
@@ -784,9 +799,10 @@ void serve_loop(Engine *Ep) {
                     walk = walk || h->dev.walk;
                 }
                 float ms = 0;
-                rc = (W == 4) ? run_batch_dispatch<4>(E, rule, jobs, gt, hb, lb, walk, true, &ms, alt != 0)
-                   : (W == 6) ? run_batch_dispatch<6>(E, rule, jobs, gt, hb, lb, false, true, &ms, alt != 0)
-                              : run_batch_dispatch<5>(E, rule, jobs, gt, hb, lb, walk, true, &ms, alt != 0);
+                const int det = alt ? WAR_DET_ALT : WAR_DET_BRENT;
+                rc = (W == 4) ? run_batch_dispatch<4>(E, rule, jobs, gt, hb, lb, walk, true, &ms, det)
+                   : (W == 6) ? run_batch_dispatch<6>(E, rule, jobs, gt, hb, lb, false, true, &ms, det)
+                              : run_batch_dispatch<5>(E, rule, jobs, gt, hb, lb, walk, true, &ms, det);
                 for (auto &h : grp) h->dev_ms += ms;   // device time of the steps the request took part in
                 if (rc) {
                     for (auto &h : grp) { h->status = rc; h->err = t_err; h->next = h->rq.hi; }
@@ -1182,6 +1198,8 @@ int war_submit(const war_request *req, war_result *res, uint64_t *id) {
     int rc = check_request(req);
     if (rc) return rc;
     if (req->kernel != WAR_KERNEL_PERSISTENT) return fail(82, "the serving loop runs the persistent kernel only");
+    if (req->detector == WAR_DET_PAIR)
+        return fail(82, "the pair-run detector runs on single requests (war_run_range), not in the serving loop");
     if (!res || !id) return fail(1, "null result or id");
     auto h = std::make_shared<ReqHost>();
     h->rq = *req;

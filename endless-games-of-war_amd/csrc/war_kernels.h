@@ -990,12 +990,22 @@ __device__ __forceinline__ St<W> shfl_state(const St<W> &x, u32 src) {
     return r;
 }
 
-constexpr int FLOW_BRENT = 0, FLOW_ALT = 2;
+constexpr int FLOW_BRENT = 0, FLOW_ALT = 2, FLOW_PAIR = 3;
+// The pair-run detector (n = 2 mod 4, rule 0): the reference's loop-shape test
+// (gpu/war_gpu.py:51-73, Theorem 3) finds the cycles of such n made of winner runs of exactly two
+// turns, all but a few (n = 14 canonical positions: all but the cycles of length 96). So a game whose
+// last ALT_K turns held a run of one turn or of three or more is, for all but those, not on its
+// cycle yet, and its tortoise jumps at the checkpoint, as after a repeated winner in the alternation
+// detector; such pattern jumps stop once the game is PAIR_P turns old, after which only the timed
+// jumps (waits of ALT_L0, then twice as long, ...) move the tortoise, so a cycle of any shape is
+// seen. Every output is exact and equals Brent's; only `turns` (its own) differs.
+constexpr u32 PAIR_P = 128;
 constexpr u32 FLOW_SPAN = ALT_K * FLOW_BLOCKS;
 
 template <int W, int RULE, int DET, bool WALK>
 __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
-    static_assert(DET == FLOW_BRENT || (DET == FLOW_ALT && RULE == 0), "the alternation detector is for rule 0");
+    static_assert(DET == FLOW_BRENT || ((DET == FLOW_ALT || DET == FLOW_PAIR) && RULE == 0),
+                  "the alternation and pair-run detectors are for rule 0");
     static_assert(!WALK || W != 6, "no cycle walks in layout 6");
     typedef typename Lay<W>::P P;
     constexpr int PW = Lay<W>::PW;
@@ -1080,7 +1090,9 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
     u64 gpos = 0;              // cycle walks: the game's position index and home tile
     u32 ghome = 0;
     u64 alive = 0, newm = 0, prevm = 0, streakm = 0, endm = 0, cycm = 0;
+    u64 preveqm = 0, youngm = 0;   // pair-run detector: last turn's same-winner mask; games <= PAIR_P old
     constexpr int FB = __builtin_ctz(FLOW_SPAN);
+    static_assert(FLOW_SPAN == 2u * ALT_K && PAIR_P % FLOW_SPAN == 0u, "checkpoints at turns 7 and 15 of a span");
     static_assert((FLOW_SPAN & (FLOW_SPAN - 1u)) == 0u && (AGE_CAP % FLOW_SPAN) == 0u,
                   "span: a power of 2 dividing AGE_CAP");
     u64 fb[FB];                // the turn of the span each finished game ended in, bit by bit
@@ -1115,7 +1127,7 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
                     // Brent's first tortoise (the start) needs no copy: no state repeats after one
                     // turn (player 1's card count changes every turn), so the repeat test of a new
                     // game's first turn is masked off, and the tortoise takes the state after it
-                    if constexpr (DET == FLOW_ALT) { T_.t = s; T_.R = clk; T_.L = ALT_L0; }
+                    if constexpr (DET != FLOW_BRENT) { T_.t = s; T_.R = clk; T_.L = ALT_L0; }
                     if constexpr (WALK) { gpos = npos; ghome = tile; }
                 }
                 u64 e0 = 0;
@@ -1124,6 +1136,7 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
                 alive |= gm & ~e0;
                 newm |= gm & ~e0;
                 prevm |= gm;   // before the first turn, player 1 counts as the last winner
+                preveqm &= ~gm;   // ... and as having changed hands (pair-run detector)
             }
             if (alive || !more) return;
         }
@@ -1155,6 +1168,11 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
     refill();
     while (alive) {
         const u32 clk0 = clk;
+        if constexpr (DET == FLOW_PAIR) {
+            // the span's checkpoints are at ages clk0 - birth + 8 and + 16 (births fall on span
+            // starts, and PAIR_P is a multiple of the span): both are <= PAIR_P iff this holds
+            youngm = ballot(clk0 - birth <= PAIR_P - FLOW_SPAN);
+        }
 #pragma unroll
         for (u32 tix = 0; tix < FLOW_SPAN; tix++) {       // turn tix of the span
             u64 over;
@@ -1171,6 +1189,14 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
                 const u64 pm = ballot(p1);
                 streakm |= ~(pm ^ prevm);
                 prevm = pm;
+            } else if constexpr (DET == FLOW_PAIR) {
+                // a run violation: the winner repeated twice in a row (a run of three or more) or
+                // changed twice in a row (a run of one)
+                const u64 pm = ballot(p1);
+                const u64 eqm = ~(pm ^ prevm);
+                streakm |= ~(eqm ^ preveqm);
+                prevm = pm;
+                preveqm = eqm;
             } else {
                 (void)p1;
             }
@@ -1198,9 +1224,11 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
             } else {
                 if ((tix + 1u) % ALT_K == 0u) {
                     // checkpoint: timed jumps (the wait doubles), then jumps after a repeated winner
+                    // (alternation) or a run violation while the game is young (pair-run)
                     const u32 now = clk0 + tix + 1u;
                     const u64 ex = ballot(now - T_.R >= T_.L) & alive;
-                    const u64 js = (ex | streakm) & alive;
+                    const u64 pat = DET == FLOW_PAIR ? (streakm & youngm) : streakm;
+                    const u64 js = (ex | pat) & alive;
                     streakm = 0;
                     if (js) {
                         if (in_mask(js)) {

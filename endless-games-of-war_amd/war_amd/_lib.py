@@ -14,8 +14,8 @@ HEADER = os.path.join(os.path.dirname(PKG_DIR), "include", "war_engine.h")
 MODE_FULL, MODE_HALF, MODE_CANON = 0, 1, 2
 KERNEL_PLAIN, KERNEL_PERSISTENT = 0, 1
 KERNELS = {"plain": KERNEL_PLAIN, "persistent": KERNEL_PERSISTENT}
-DET_BRENT, DET_STREAK, DET_ALT = 0, 1, 2
-DETECTORS = {"brent": DET_BRENT, "streak": DET_STREAK, "alt": DET_ALT}
+DET_BRENT, DET_STREAK, DET_ALT, DET_PAIR = 0, 1, 2, 3
+DETECTORS = {"brent": DET_BRENT, "streak": DET_STREAK, "alt": DET_ALT, "pair": DET_PAIR}
 
 
 class WarTuning(ctypes.Structure):

@@ -51,6 +51,13 @@ extern "C" {
                               of 8, after a repeated winner (odd-n cycles alternate, Spivey
                               Thm 6), and after waiting 128, 256, ... turns. Every output is
                               exact and equals Brent's; only `turns` (its own) is smaller.     */
+#define WAR_DET_PAIR   3   /* Brent with pair-run checkpoints (this engine's; n = 2 mod 4, rule 0,
+                              the persistent kernel on single requests): as WAR_DET_ALT, but the
+                              pattern jump follows a winner run of one turn or of three or more
+                              (the cycles of such n are, all but a few, runs of exactly two: the
+                              loop-shape test of gpu/war_gpu.py:51-73), and only while the game is
+                              at most 128 turns old. Every output is exact and equals Brent's;
+                              only `turns` (its own) is smaller.                               */
 
 typedef struct war_tuning {
     int32_t waves_per_cu;      /* persistent grid = CUs x waves_per_cu wave64s (1..32)          */

@@ -66,8 +66,9 @@ def total(n, mode="full"):
 def run(n, mode="full", lo=0, hi=None, rule=0, hbins=0, lbins=0, dbins=0, distinct=False, shape_blk=0, threads=0,
         detector="brent"):
     """Play every position of [lo, hi) on the CPU; returns a dict shaped like the engine's result.
-    detector: "brent" (every output), "streak" (odd n, rule 0, no cycle-length outputs) or "alt"
-    (Brent with alternation checkpoints: odd n, rule 0, every output)."""
+    detector: "brent" (every output), "streak" (odd n, rule 0, no cycle-length outputs), "alt"
+    (Brent with alternation checkpoints: odd n, rule 0, every output) or "pair" (Brent with pair-run
+    checkpoints: n = 2 mod 4, rule 0, every output)."""
     L = lib()
     m = _MODES.get(mode, mode)
     if hi is None:
@@ -79,7 +80,7 @@ def run(n, mode="full", lo=0, hi=None, rule=0, hbins=0, lbins=0, dbins=0, distin
     r.hist = ctypes.cast(H, _U64P); r.hbins = hbins
     r.lenhist = ctypes.cast(Ln, _U64P); r.lbins = lbins
     r.dhist = ctypes.cast(D, _U64P); r.dbins = dbins
-    det = {"brent": 0, "streak": 1, "alt": 2}[detector]
+    det = {"brent": 0, "streak": 1, "alt": 2, "pair": 3}[detector]
     rc = L.orc_run_det(int(n), m, int(rule), int(lo), int(hi), int(threads), int(bool(distinct)), int(shape_blk),
                        det, ctypes.byref(r))
     if rc:

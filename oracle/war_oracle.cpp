@@ -19,7 +19,8 @@
 //   streak test (odd n) ......... gpu/war24.py:98-125
 //   cycle length / pre-period ... cpu/cycle_length.c:16-30,114-126
 //   reach by a deal ............. cpu/reach_cycles.c:32-69,73-107
-// and one detector of this engine's own (alt_brent, odd n), pinned against Brent.
+// and two detectors of this engine's own (alt_brent, odd n; pair_brent, n = 2 mod 4), pinned
+// against Brent.
 //
 // Extensions the reference does not have (pinned by invariants in the tests, not by goldens):
 // arbitrary rank / position ranges, the histogram of ending-game lengths (lenhist), the total
@@ -186,6 +187,41 @@ template <int W> static inline bool alt_brent(St<W> s, u64& rounds, u64& lam_out
     }
 }
 
+// Brent with pair-run checkpoints (this engine's detector for n = 2 mod 4, rule 0; not in the
+// reference). The reference's loop-shape test (gpu/war_gpu.py:51-73, Theorem 3) finds the cycles of
+// such n made of winner runs of exactly n & -n = 2 turns, all but a few (at n = 14 the canonical
+// positions of every cycle but those of length 96, data/long_runs.txt:20-22). Every ALT_K turns, a
+// game whose last ALT_K turns held a run of one turn or of three or more (the winner changed twice
+// in a row, or repeated twice in a row) jumps its tortoise to the present state, as long as the game
+// is at most PAIR_P turns old; a tortoise that has waited ALT_L0 turns (then twice as long, and so
+// on) jumps too, so the pattern jumps stop and the timed ones find a cycle of any shape. Before the
+// first turn, player 1 counts as the last winner and the hands as having changed. A repeat is only
+// declared on an exact state match and the first match after a jump is the cycle length, so every
+// output is exact; the turns to the repeat are fewer than Brent's (about 8 % at n = 14 canonical).
+constexpr u64 PAIR_P = 128;
+template <int W> static inline bool pair_brent(St<W> s, u64& rounds, u64& lam_out, St<W>* meet) {
+    if (s.la == 0 || s.lb == 0) { rounds = 0; return false; }
+    St<W> t = s;
+    u64 R = 0, age = 0, L = ALT_L0;
+    bool viol = false;
+    int prevw = 1, preveq = 0;
+    for (;;) {
+        const int w = step<W>(s, 0);
+        age++;
+        if (s.la == 0 || s.lb == 0) { rounds = age; return false; }
+        if (same<W>(s, t)) { rounds = age; lam_out = age - R; if (meet) *meet = s; return true; }
+        const int eq = w == prevw;
+        if (eq == preveq) viol = true;
+        prevw = w;
+        preveq = eq;
+        if (age % ALT_K == 0) {
+            if (age - R >= L) { t = s; R = age; L *= 2; }
+            else if (viol && age <= PAIR_P) { t = s; R = age; }
+            viol = false;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------------------------
 // Permutations (lexicographic, p[0] most significant), war_gpu.py:135-146 and :109-119.
 static u64 fact_u64(int k) { u64 f = 1; for (int i = 2; i <= k; i++) f *= (u64)i; return f; }
@@ -268,8 +304,8 @@ template <int W> static void play_cycle(const St<W>& s, const Cfg& c, Acc& A, u6
 
 template <int W> static void play(const St<W>& s, const Cfg& c, Acc& A) {
     u64 rounds = 0, lam = 0; St<W> meet;
-    if (c.det == 2) {      // alternation checkpoints: every output, its own turn count
-        const bool cy = alt_brent<W>(s, rounds, lam, &meet);
+    if (c.det == 2 || c.det == 3) {   // alternation / pair-run checkpoints: every output, own turn count
+        const bool cy = c.det == 2 ? alt_brent<W>(s, rounds, lam, &meet) : pair_brent<W>(s, rounds, lam, &meet);
         A.done++;
         A.turns += rounds;
         if (!cy) {
@@ -376,6 +412,7 @@ extern "C" u64 orc_total(int n, int mode) { return range_total(n, mode); }
 // Play every position in [lo, hi) (deal ranks, or canon positions). 0 on success.
 // det: 0 = Brent (every output), 1 = the streak test (odd n, rule 0; no cycle lengths, so no
 // hist / dhist / distinct / shape), 2 = Brent with alternation checkpoints (odd n, rule 0; every
+// output, its own turn count), 3 = Brent with pair-run checkpoints (n = 2 mod 4, rule 0; every
 // output, its own turn count).
 extern "C" int orc_run_det(int n, int mode, int rule, u64 lo, u64 hi, int threads,
                            int distinct, int shape_blk, int det, orc_res* res) {
@@ -388,9 +425,14 @@ extern "C" int orc_run_det(int n, int mode, int rule, u64 lo, u64 hi, int thread
     if (mode == ORC_HALF && (n & 1)) { g_err = "half mode needs even n"; return 6; }
     if ((distinct || res->dbins) && (mode != ORC_CANON || n > 15)) { g_err = "distinct needs canon mode and n <= 15"; return 7; }
     if (shape_blk && mode != ORC_CANON) { g_err = "shape needs canon mode"; return 8; }
-    if (det < 0 || det > 2) { g_err = "det must be 0 (Brent), 1 (streak) or 2 (alternation checkpoints)"; return 9; }
+    if (det < 0 || det > 3) {
+        g_err = "det must be 0 (Brent), 1 (streak), 2 (alternation checkpoints) or 3 (pair-run checkpoints)"; return 9;
+    }
     if (det == 2 && ((n % 2) == 0 || rule != 0)) {
         g_err = "alternation checkpoints need odd n and rule 0"; return 10;
+    }
+    if (det == 3 && ((n % 4) != 2 || rule != 0)) {
+        g_err = "pair-run checkpoints need n = 2 mod 4 and rule 0"; return 10;
     }
     if (det == 1 && ((n % 2) == 0 || rule != 0 || distinct || shape_blk || (res->hist && res->hbins) ||
                      (res->dhist && res->dbins))) {
@@ -613,7 +655,7 @@ extern "C" void orc_unrank(u64 r, int m, int* p) { unrank(r, m, p); }
 static void usage() {
     fprintf(stderr, "usage: war_oracle n [--mode full|half|canon] [--rule 0|1] [--lo L] [--hi H]\n"
                     "                  [--hist B] [--lbins B] [--dbins B] [--distinct] [--shape] [--threads T]\n"
-                    "                  [--det brent|streak|alt]\n");
+                    "                  [--det brent|streak|alt|pair]\n");
 }
 int main(int argc, char** argv) {
     if (argc < 2) { usage(); return 2; }
@@ -633,7 +675,7 @@ int main(int argc, char** argv) {
         else if (a == "--distinct") distinct = 1;
         else if (a == "--shape") shape = 1;
         else if (a == "--threads") threads = atoi(nxt().c_str());
-        else if (a == "--det") { std::string d = nxt(); det = d == "streak" ? 1 : d == "alt" ? 2 : 0; }
+        else if (a == "--det") { std::string d = nxt(); det = d == "streak" ? 1 : d == "alt" ? 2 : d == "pair" ? 3 : 0; }
         else { usage(); return 2; }
     }
     u64 total = range_total(n, mode);

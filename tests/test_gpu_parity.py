@@ -306,3 +306,55 @@ def test_alt_n13_goldens_and_refusals(eng, goldens):
         n = kw.pop("n")
         with pytest.raises(WarError, match="alternation"):
             eng.run_range(n, lo=0, hi=100, detector="alt", **kw)
+
+
+# ---- the pair-run detector (n = 2 mod 4, rule 0; flow kernel): every output equal to the oracle's
+def check_pair(eng, orc, n, mode, lo, hi, tuning=None, **kw):
+    want = orc.run(n, mode, lo, hi, hbins=256, lbins=512, detector="pair", **kw)
+    got = eng.run_range(n, mode, lo, hi, hbins=256, lbins=512, detector="pair", tuning=tuning, **kw)
+    walk_fields = ("oncycle", "distinct_minkey", "dhist", "shape_nonstd", "shape_unchk") if kw else ()
+    for k in ALT_FIELDS + walk_fields:
+        assert got[k] == want[k], (n, mode, lo, hi, tuning, k, got[k], want[k])
+    return got, want
+
+
+@pytest.mark.parametrize("n,mode", [(6, "full"), (6, "canon"), (6, "half"), (10, "full"), (10, "canon"),
+                                    (10, "half")])
+def test_pair_whole_range(eng, orc, n, mode):
+    print("case pair", n, mode, flush=True)
+    got, _ = check_pair(eng, orc, n, mode, 0, orc.total(n, mode))
+    brent = eng.run_range(n, mode, hbins=256, lbins=512)
+    assert (got["cyc"], got["maxfin"], got["hist"], got["lenhist"]) == \
+           (brent["cyc"], brent["maxfin"], brent["hist"], brent["lenhist"])
+
+
+def test_pair_sampled_and_tunings(eng, orc):
+    rng = random.Random(3)
+    for n, mode in [(14, "full"), (14, "canon"), (14, "half"), (18, "full"), (18, "canon")]:
+        total = orc.total(n, mode)
+        for ln in (1, 65, 200_003):
+            lo = rng.randrange(0, total - ln)
+            print("case pair", n, mode, lo, ln, flush=True)
+            check_pair(eng, orc, n, mode, lo, lo + ln)
+    for tuning in ({"ranks_per_refill": 64}, {"ranks_per_refill": 8192, "waves_per_cu": 4},
+                   {"ranks_per_refill": 128, "waves_per_cu": 32}):
+        print("case pair tuning", tuning, flush=True)
+        check_pair(eng, orc, 10, "full", 0, orc.total(10, "full"), tuning=tuning)
+        check_pair(eng, orc, 14, "canon", 4 * 10**10, 4 * 10**10 + 300_000, tuning=tuning)
+
+
+def test_pair_walks(eng, orc):
+    print("case pair walks", flush=True)
+    check_pair(eng, orc, 10, "canon", 0, orc.total(10, "canon"), dbins=4096, distinct=True, shape_blk=2)
+    check_pair(eng, orc, 14, "canon", 2 * 10**10, 2 * 10**10 + 1_000_000, dbins=64, distinct=True, shape_blk=2)
+
+
+def test_pair_n14_canon_goldens_and_refusals(eng, goldens):
+    r = eng.run_range(14, "canon", hbins=256, detector="pair")
+    assert (r["cyc"], r["maxfin"], r["done"]) == (2872129173, 465, 80951270400)
+    assert {i: v for i, v in enumerate(r["hist"]) if v} == {96: 43669233, 112: 2828459940}
+    print("n14 canon pair device_ms", r["device_ms"], flush=True)
+    for n, kw in ((13, {}), (12, {}), (14, {"rule": 1}), (14, {"kernel": "plain"}),
+                  (14, {"tuning": {"stage_kind": "lockstep"}})):
+        with pytest.raises(WarError, match="pair-run"):
+            eng.run_range(n, lo=0, hi=100, detector="pair", **kw)

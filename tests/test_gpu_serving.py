@@ -4,6 +4,8 @@ import random
 
 import pytest
 
+from war_amd import WarError
+
 pytestmark = pytest.mark.gpu
 
 KEYS = ("cyc", "done", "maxfin", "turns", "hist", "lenhist", "oncycle", "distinct_minkey", "distinct_hashset",
@@ -58,3 +60,12 @@ def test_poll_and_late_arrivals(eng, orc):
     whole = orc.run(12, "canon", 0, 30_000_000, hbins=64)
     assert sum(r["cyc"] for r in res) == whole["cyc"]
     assert sum(r["turns"] for r in res) == whole["turns"]
+
+
+def test_serving_refuses_pair_detector(eng):
+    eng.serve_start(max_slots=4, step_positions=1 << 20)
+    try:
+        with pytest.raises(WarError, match="pair-run"):
+            eng.submit(14, "canon", 0, 1000, detector="pair")
+    finally:
+        eng.serve_stop()
