@@ -160,11 +160,12 @@ template <int W> static inline int streak(St<W> s, int n, u64& steps) {
 // Brent with alternation checkpoints (this engine's detector for odd n, rule 0; not in the
 // reference). In a cycle of odd-n War the players alternate winning (Spivey Thm 6, the fact the
 // reference's streak test rests on, gpu/war24.py:9-17). Every ALT_K turns (ages multiple of
-// ALT_K), a game whose last ALT_K turns held a repeated winner cannot have been on its cycle for
-// those turns, so its tortoise jumps to the present state; a tortoise that has waited ALT_L0
-// turns (then twice as long, and so on) jumps too, as in Brent. A repeat is only ever declared
-// on an exact state match, and the first match after a jump is the cycle length, so every output
-// is exact; the turns to the repeat are fewer than Brent's (about 30 % at n = 13).
+// ALT_K), a game one of whose last ALT_K turns of odd age repeated the previous turn's winner
+// cannot have been on its cycle for those turns, so its tortoise jumps to the present state; a
+// tortoise that has waited ALT_L0 turns (then twice as long, and so on) jumps too, as in Brent.
+// A repeat is only ever declared on an exact state match, and the first match after a jump is the
+// cycle length, so every output is exact; the turns to the repeat are fewer than Brent's (about
+// 30 % at n = 13).
 constexpr u64 ALT_K = 8, ALT_L0 = 128;
 template <int W> static inline bool alt_brent(St<W> s, u64& rounds, u64& lam_out, St<W>* meet) {
     if (s.la == 0 || s.lb == 0) { rounds = 0; return false; }
@@ -177,7 +178,7 @@ template <int W> static inline bool alt_brent(St<W> s, u64& rounds, u64& lam_out
         age++;
         if (s.la == 0 || s.lb == 0) { rounds = age; return false; }
         if (same<W>(s, t)) { rounds = age; lam_out = age - R; if (meet) *meet = s; return true; }
-        if (w == prevw) streak = true;
+        if (w == prevw && (age & 1)) streak = true;
         prevw = w;
         if (age % ALT_K == 0) {
             if (age - R >= L) { t = s; R = age; L *= 2; }

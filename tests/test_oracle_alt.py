@@ -1,6 +1,6 @@
 """The oracle's alternation-checkpoint detector (odd n, rule 0): Brent's tortoise also jumps, at
-ages that are multiples of 4, when the last 4 turns held a repeated winner, and after waiting 128
-turns (then 256, ...). Pinned by:
+ages that are multiples of 8, when one of the last 8 turns of odd age repeated the previous turn's
+winner, and after waiting 128 turns (then 256, ...). Pinned by:
 
 * a model on plain Python deques of the same schedule (before the first turn player 1 counts as
   the last winner), for every deal and canonical position of odd n <= 7 (outcome, game length,
@@ -8,7 +8,7 @@ turns (then 256, ...). Pinned by:
 * the set-based (mu, lam) model: a repeat is seen no earlier than mu + lam;
 * Brent, over every deal of odd n <= 11 and every canonical position of odd n <= 9: the same
   endless count, longest game, cycle-length histogram, ending-length histogram, weights, distinct
-  counts and shape counts (only the turn count differs, and it is smaller).
+  counts and shape counts (only the turn count differs, and it is smaller for n >= 9).
 """
 import itertools
 from collections import deque
@@ -38,7 +38,7 @@ def alt_model(p1, p2):
         cur = (tuple(a), tuple(b))
         if cur == t:
             return True, age, age - R
-        if w == prevw:
+        if w == prevw and age % 2 == 1:
             streak = True
         prevw = w
         if age % K == 0:
@@ -80,7 +80,10 @@ def test_alt_equals_brent(orc, n, mode):
     for k in a:
         if k != "turns":
             assert a[k] == b[k], (n, mode, k)
-    assert a["turns"] < b["turns"]
+    # fewer turns where games are long enough to matter; the deals of n = 5 and 7 (games of a few
+    # dozen turns) whose last repeated winner fell on a turn of even age wait for the timed jump
+    if n >= 9:
+        assert a["turns"] < b["turns"]
 
 
 def test_alt_refusals(orc):
