@@ -1170,6 +1170,7 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
     refill();
     while (alive) {
         const u32 clk0 = clk;
+        const u64 alive0 = alive;
         if constexpr (DET == FLOW_PAIR) {
             // the span's checkpoints are at ages clk0 - birth + 8 and + 16 (births fall on span
             // starts, and PAIR_P is a multiple of the span): both are <= PAIR_P iff this holds
@@ -1203,13 +1204,17 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
             } else {
                 (void)p1;
             }
-            if ((over | cand) & alive) {
-                const u64 em = over & alive;
-                u64 dm = cand & alive & ~em;
-                if (dm) dm &= rest_mask<W>(s, T_.t);
-                const u64 ev = em | dm;
+            if (const u64 x = (over | cand) & alive) {
+                // ended games: x & over; the ended set of the span is rebuilt at its end (live at
+                // the start, not now, not repeated), so only the rare repeats are collected here
+                u64 ev = x & over;
+                if (const u64 dc = x & ~over) {
+                    const u64 dm = dc & rest_mask<W>(s, T_.t);
+                    cycm |= dm;
+                    ev |= dm;
+                }
                 for (int b = 0; b < FB; b++) if ((tix >> b) & 1u) fb[b] |= ev;
-                endm |= em; cycm |= dm; alive &= ~ev;
+                alive &= ~ev;
             }
             if constexpr (DET == FLOW_BRENT) {
                 // Brent's tortoise moves at ages 2^k - 1: turn 2^k - 2 of a lane's first span; turn
@@ -1244,6 +1249,7 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
         }
         clk += FLOW_SPAN;
         newm = 0;
+        endm = alive0 & ~alive & ~cycm;
         if (endm | cycm) settle(clk0);
         // the safety stop, tested every 1024 turns: a game still running after AGE_CAP turns is
         // undetermined (never for a valid game; its turns are not counted)
