@@ -153,15 +153,25 @@ __device__ __forceinline__ bool turn_held(St<6> &s, u32 C, Zh<6> &z) { return tu
 
 template <int W> __device__ __forceinline__ bool ended(const St<W> &s, u32 C) { return s.sa > C; }
 
+// d = C - x and the wave's mask of lanes where it borrows (x > C), in one instruction (bits of
+// inactive lanes are 0). UC: C is wave-uniform and sits in a scalar register.
+template <bool UC>
+__device__ __forceinline__ void sub_borrow(u32 C, u32 x, u32 &d, u64 &borrow) {
+    if constexpr (UC) asm("v_sub_co_u32 %0, %1, %2, %3" : "=v"(d), "=s"(borrow) : "s"(C), "v"(x));
+    else asm("v_sub_co_u32 %0, %1, %2, %3" : "=v"(d), "=s"(borrow) : "v"(C), "v"(x));
+}
+
 // turn() for loops that carry sb = C - sa from turn to turn (player 2's append offset): the
 // subtraction that makes the next turn's sb borrows exactly when sa > C, so its borrow is the end
 // test and the turn needs no compare of its own for it. `over` receives the wave's lane mask of
 // games that ended (bits of inactive lanes are 0). The subtraction is written as one instruction so
 // that its borrow lands in a scalar register pair as the mask itself (the compiler would otherwise
-// copy the borrow through a vector register). The two cards go in with a 64-bit add (they land on
+// copy the borrow through a vector register). UC: C is the same in every lane of the wave (one
+// request: a scalar register); without it C is taken in a vector register, as where a wave mixes
+// requests of several n. The two cards go in with a 64-bit add (they land on
 // zero bits, so it equals the OR), one v_lshl_add_u64 instead of two 32-bit ORs: 17 VALU per turn
 // for 4-bit piles.
-template <int W, int RULE>
+template <int W, int RULE, bool UC = false>
 __device__ __forceinline__ bool turn_sb(St<W> &s, u32 &sb, u32 C, Zh<W> &z, u64 &over) {
     if constexpr (W == 4) {
         const u32 c1 = (u32)s.a & 15u, c2 = (u32)s.b & 15u;
@@ -176,11 +186,11 @@ __device__ __forceinline__ bool turn_sb(St<W> &s, u32 &sb, u32 C, Zh<W> &z, u64 
         s.a += z.h1 << (s.sa & 63u);
         s.b += z.h2 << (sb & 63u);
         s.sa += p1 ? 4u : (u32)-4;
-        asm("v_sub_co_u32 %0, %1, %2, %3" : "=v"(sb), "=s"(over) : "s"(C), "v"(s.sa));
+        sub_borrow<UC>(C, s.sa, sb, over);
         return p1;
     } else if constexpr (W == 6) {
         const bool hw = turn_held_at<RULE>(s, sb, z);
-        asm("v_sub_co_u32 %0, %1, %2, %3" : "=v"(sb), "=s"(over) : "s"(C), "v"(s.sa));
+        sub_borrow<UC>(C, s.sa, sb, over);
         return hw;
     } else {
         const bool p1 = turn<W, RULE>(s, C, z);
@@ -198,7 +208,8 @@ __device__ __forceinline__ bool turn_sb(St<W> &s, u32 &sb, u32 C, Zh<W> &z, u64 
 // one v_lshl_add_u64 each, against two 64-bit shifts and an add in turn_sb. The offsets, the
 // lengths and the end test are turn_sb's (sa = 4 * (player 1's cards - 1)). On gfx950 the turn
 // issues in about 70 cycles per wave against about 80 for turn_sb (measured, 7 waves per SIMD).
-// Zt: the two 64-bit words the pairs are built in; their low halves stay an opaque zero.
+// Zt: the two 64-bit words the pairs are built in; their low halves stay an opaque zero. C must be
+// wave-uniform (the flow kernel plays one request per batch).
 struct Zt { u64 h1, h2; };
 __device__ __forceinline__ Zt make_zt(u32 zero) {
     // two vector-register zeros the compiler can neither rebuild from a scalar one inside the loop
@@ -219,7 +230,7 @@ __device__ __forceinline__ bool turn_top(St<4> &s, u32 &sb, u32 C, Zt &z, u64 &o
     s.a = (s.a << 4) + (z.h1 >> (s.sa & 63u));
     s.b = (s.b << 4) + (z.h2 >> (sb & 63u));
     s.sa += p1 ? 4u : (u32)-4;
-    asm("v_sub_co_u32 %0, %1, %2, %3" : "=v"(sb), "=s"(over) : "s"(C), "v"(s.sa));
+    sub_borrow<true>(C, s.sa, sb, over);
     return p1;
 }
 

@@ -1181,7 +1181,7 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
             u64 over;
             bool p1;
             if constexpr (W == 4) p1 = turn_top<RULE>(s, sb, C, zt, over);
-            else p1 = turn_sb<W, RULE>(s, sb, C, z, over);
+            else p1 = turn_sb<W, RULE, true>(s, sb, C, z, over);
             // the common case (no lane ended or matched the tortoise's player-1 pile) costs two
             // scalar operations and the branch; the scalar unit is busy about 70 % of the time here
             u64 cand = ballot(s.a == T_.t.a);
