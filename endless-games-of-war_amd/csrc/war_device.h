@@ -208,8 +208,8 @@ __device__ __forceinline__ bool turn_sb(St<W> &s, u32 &sb, u32 C, Zh<W> &z, u64 
 // one v_lshl_add_u64 each, against two 64-bit shifts and an add in turn_sb. The offsets, the
 // lengths and the end test are turn_sb's (sa = 4 * (player 1's cards - 1)). On gfx950 the turn
 // issues in about 70 cycles per wave against about 80 for turn_sb (measured, 7 waves per SIMD).
-// Zt: the two 64-bit words the pairs are built in; their low halves stay an opaque zero. C must be
-// wave-uniform (the flow kernel plays one request per batch).
+// Zt: the two 64-bit words the pairs are built in; their low halves stay an opaque zero. UC: C is
+// wave-uniform (a batch of one request), else it is taken per lane.
 struct Zt { u64 h1, h2; };
 __device__ __forceinline__ Zt make_zt(u32 zero) {
     // two vector-register zeros the compiler can neither rebuild from a scalar one inside the loop
@@ -219,7 +219,7 @@ __device__ __forceinline__ Zt make_zt(u32 zero) {
     asm volatile("v_mov_b32 %0, %1" : "=v"(z2) : "s"(zero));
     Zt z; z.h1 = z1; z.h2 = z2; return z;
 }
-template <int RULE>
+template <int RULE, bool UC = true>
 __device__ __forceinline__ bool turn_top(St<4> &s, u32 &sb, u32 C, Zt &z, u64 &over) {
     const u32 ah = (u32)(s.a >> 32) & 0xF0000000u, bh = (u32)(s.b >> 32) & 0xF0000000u;   // top cards << 28
     const bool p1 = ah > bh;
@@ -230,7 +230,7 @@ __device__ __forceinline__ bool turn_top(St<4> &s, u32 &sb, u32 C, Zt &z, u64 &o
     s.a = (s.a << 4) + (z.h1 >> (s.sa & 63u));
     s.b = (s.b << 4) + (z.h2 >> (sb & 63u));
     s.sa += p1 ? 4u : (u32)-4;
-    sub_borrow<true>(C, s.sa, sb, over);
+    sub_borrow<UC>(C, s.sa, sb, over);
     return p1;
 }
 

@@ -455,6 +455,20 @@ void launch_refill(int flags, int blocks, size_t lds, hipStream_t st, const Stag
     }
 }
 
+// The flow kernel for a batch's detector (WAR_DET_*): Brent for either rule, the alternation and
+// pair-run detectors for rule 0 (pair-run: single requests, layouts 4 and 5).
+template <int W, int RULE, bool WALK, bool MULTI>
+void launch_flow(int det, int blocks, size_t lds, hipStream_t st, const StageArgs<W> &A) {
+    if (det == WAR_DET_BRENT) {
+        hipLaunchKernelGGL((k_flow<W, RULE, FLOW_BRENT, WALK, MULTI>), dim3(blocks), dim3(256), lds, st, A);
+    } else if constexpr (RULE == 0) {
+        if (det == WAR_DET_ALT)
+            hipLaunchKernelGGL((k_flow<W, 0, FLOW_ALT, WALK, MULTI>), dim3(blocks), dim3(256), lds, st, A);
+        else if constexpr (!MULTI && W != 6)
+            hipLaunchKernelGGL((k_flow<W, 0, FLOW_PAIR, WALK, false>), dim3(blocks), dim3(256), lds, st, A);
+    }
+}
+
 template <int W, int RULE>
 int build_graph(Engine &E, const GraphKey &key, hipGraphExec_t *out) {
     typedef typename Lay<W>::T T;
@@ -476,18 +490,12 @@ int build_graph(Engine &E, const GraphKey &key, hipGraphExec_t *out) {
         if (key.walk) {
             A.cq = cyc_queue<W>(E);
             if constexpr (CW) {
-                if (key.det == WAR_DET_BRENT) hipLaunchKernelGGL((k_flow<W, RULE, FLOW_BRENT, CW>), dim3(blocks), dim3(256), lds, E.st, A);
-                else if constexpr (RULE == 0) {
-                    if (key.det == WAR_DET_ALT) hipLaunchKernelGGL((k_flow<W, 0, FLOW_ALT, CW>), dim3(blocks), dim3(256), lds, E.st, A);
-                    else hipLaunchKernelGGL((k_flow<W, 0, FLOW_PAIR, CW>), dim3(blocks), dim3(256), lds, E.st, A);
-                }
+                if (key.multi) launch_flow<W, RULE, true, true>(key.det, blocks, lds, E.st, A);
+                else launch_flow<W, RULE, true, false>(key.det, blocks, lds, E.st, A);
             }
         } else {
-            if (key.det == WAR_DET_BRENT) hipLaunchKernelGGL((k_flow<W, RULE, FLOW_BRENT, false>), dim3(blocks), dim3(256), lds, E.st, A);
-            else if constexpr (RULE == 0) {
-                if (key.det == WAR_DET_ALT) hipLaunchKernelGGL((k_flow<W, 0, FLOW_ALT, false>), dim3(blocks), dim3(256), lds, E.st, A);
-                else if constexpr (W != 6) hipLaunchKernelGGL((k_flow<W, 0, FLOW_PAIR, false>), dim3(blocks), dim3(256), lds, E.st, A);
-            }
+            if (key.multi) launch_flow<W, RULE, false, true>(key.det, blocks, lds, E.st, A);
+            else launch_flow<W, RULE, false, false>(key.det, blocks, lds, E.st, A);
         }
     }
     u32 a0 = 0;
@@ -538,10 +546,10 @@ int run_persistent_batch(Engine &E, std::vector<Job> &jobs, const war_tuning &t,
     if (jobs.empty()) return 0;
     if ((int)jobs.size() > MAXJOBS) return fail(31, "too many jobs in one batch");
     const bool alt = det == WAR_DET_ALT;
-    // the flow kernel (stage_kind 3) plays single-request batches; elsewhere stage_kind 3 runs
-    // lockstep stages. Without cycle walks it needs no queue buffers.
-    const bool flow = t.stage_kind == 3 && !multi && (RULE == 0 || det == WAR_DET_BRENT);
-    if (det == WAR_DET_PAIR && !flow)
+    // the flow kernel (stage_kind 3, the default) plays every batch, of one request or several;
+    // stage_kind 1 / 2 select the staged pipeline. Without cycle walks it needs no queue buffers.
+    const bool flow = t.stage_kind == 3 && (RULE == 0 || det == WAR_DET_BRENT);
+    if (det == WAR_DET_PAIR && (!flow || multi))
         return fail(12, "the pair-run detector runs on the flow kernel (stage_kind 3, single requests)");
     int rc = (flow && !walk) ? 0 : ensure_queues<W>(E, walk, alt);
     if (rc) return rc;

@@ -1004,11 +1004,15 @@ constexpr int FLOW_BRENT = 0, FLOW_ALT = 2, FLOW_PAIR = 3;
 constexpr u32 PAIR_P = 128;
 constexpr u32 FLOW_SPAN = ALT_K * FLOW_BLOCKS;
 
-template <int W, int RULE, int DET, bool WALK>
+// MULTI: the batch holds several requests (the serving loop's steps). A tile belongs to one job,
+// so the pool's positions share one request (its n, mode and C change at a tile grab); a game
+// keeps its request and C per lane, and accounting groups the finished lanes by request.
+template <int W, int RULE, int DET, bool WALK, bool MULTI>
 __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
     static_assert(DET == FLOW_BRENT || ((DET == FLOW_ALT || DET == FLOW_PAIR) && RULE == 0),
                   "the alternation and pair-run detectors are for rule 0");
     static_assert(!WALK || W != 6, "no cycle walks in layout 6");
+    static_assert(!MULTI || DET != FLOW_PAIR, "the pair-run detector runs on single requests");
     typedef typename Lay<W>::P P;
     constexpr int PW = Lay<W>::PW;
     extern __shared__ u32 lds[];
@@ -1019,9 +1023,10 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
     const ReqDev *reqs = A.reqs;
     const u32 req = A.hdr->req0;
     acc.select(req, reqs);
-    const u32 n = reqs[req].n, mode = reqs[req].mode;
-    const bool walk = WALK && reqs[req].walk != 0;   // cycles go to the cycle records for k_walk
-    const u32 C = c_of<W>(n);
+    // the pool's request (all of a batch with one request; the current tile's with MULTI)
+    u32 preq = req, n = reqs[req].n, mode = reqs[req].mode;
+    bool walk = WALK && reqs[req].walk != 0;   // cycles go to the cycle records for k_walk
+    u32 C = c_of<W>(n);
     const u32 per = A.per, TP = A.tile_pos;
     const u64 ntiles = A.hdr->ntiles;
     const u32 min_dead = A.flow_min;       // refill once this many lanes are free (or none is alive)
@@ -1059,6 +1064,9 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
             u32 lo = 0, hi = A.hdr->njobs;
             while (hi - lo > 1) { u32 mid = (lo + hi) >> 1; if (A.jobs[mid].tile_base <= g0) lo = mid; else hi = mid; }
             const Job J = A.jobs[lo];
+            if constexpr (MULTI) {
+                preq = J.req; n = reqs[preq].n; mode = reqs[preq].mode; C = c_of<W>(n);
+            }
             const u64 first = J.lo + (g0 - J.tile_base) * (u64)TP + (u64)lane * per;
             left = first < J.hi ? (u32)((J.hi - first) < per ? (J.hi - first) : per) : 0u;
             if (left) {
@@ -1086,6 +1094,7 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
     // ---- the games
     St<W> s = {};
     u32 sb = C;                // C - s.sa: player 2's append offset, carried from turn to turn
+    u32 gC = C, greq = req;    // MULTI: the game's C and request
     AltLane<W> T_;             // tortoise, its clock R; L: the alternation detector's timed wait
     T_.t = s; T_.R = 0; T_.L = ALT_L0;
     u32 birth = 0, clk = 0;
@@ -1126,6 +1135,7 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
                     open_sa<W>(s, A.zero);
                     sb = C - s.sa;
                     birth = clk;
+                    if constexpr (MULTI) { gC = C; greq = preq; }
                     // Brent's first tortoise (the start) needs no copy: no state repeats after one
                     // turn (player 1's card count changes every turn), so the repeat test of a new
                     // game's first turn is masked off, and the tortoise takes the state after it
@@ -1134,7 +1144,11 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
                 }
                 u64 e0 = 0;
                 // a position that is over before it starts (only with fewer than 2 cards): length 0
-                if (n < 2u) { e0 = gm; acc.ended((u32)__popcll(e0), 0); }
+                if (n < 2u) {
+                    e0 = gm;
+                    if constexpr (MULTI) acc.select(preq, reqs);
+                    acc.ended((u32)__popcll(e0), 0);
+                }
                 alive |= gm & ~e0;
                 newm |= gm & ~e0;
                 prevm |= gm;   // before the first turn, player 1 counts as the last winner
@@ -1148,6 +1162,25 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
         u32 fin = clk0 + 1u;
         for (int b = 0; b < FB; b++) { if (in_mask(fb[b])) fin += 1u << b; fb[b] = 0; }
         const u32 age = fin - birth, lam = fin - T_.R;
+        if constexpr (MULTI) {
+            // by request: wave sums of the group's ages, one LDS add per lane for the histograms
+            for (u64 rem = endm | cycm; rem;) {
+                const u32 r = lanev(greq, (u32)(__ffsll((long long)rem) - 1));
+                const u64 mr = ballot(greq == r) & rem;
+                rem &= ~mr;
+                acc.select(r, reqs);
+                acc.lanes(endm & mr, cycm & mr, age, lam);
+                if constexpr (WALK) {
+                    if (reqs[r].walk && (cycm & mr)) {
+                        St<W> m = T_.t;
+                        if constexpr (W == 4) m = flip_piles(m);
+                        acc.pushed += push_cycles<W>(A.cq, TP, cycm & mr, m, reqs[r].n, r, lam, gpos, ghome);
+                    }
+                }
+            }
+            endm = cycm = 0;
+            return;
+        }
         if (in_mask(endm | cycm)) lturns += age;
         if (in_mask(endm)) {
             lmax = age > lmax ? age : lmax;
@@ -1180,8 +1213,8 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
         for (u32 tix = 0; tix < FLOW_SPAN; tix++) {       // turn tix of the span
             u64 over;
             bool p1;
-            if constexpr (W == 4) p1 = turn_top<RULE>(s, sb, C, zt, over);
-            else p1 = turn_sb<W, RULE, true>(s, sb, C, z, over);
+            if constexpr (W == 4) p1 = turn_top<RULE, !MULTI>(s, sb, MULTI ? gC : C, zt, over);
+            else p1 = turn_sb<W, RULE, !MULTI>(s, sb, MULTI ? gC : C, z, over);
             // the common case (no lane ended or matched the tortoise's player-1 pile) costs two
             // scalar operations and the branch; the scalar unit is busy about 70 % of the time here
             u64 cand = ballot(s.a == T_.t.a);
@@ -1256,22 +1289,33 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
         if ((clk & 1023u) == 0u) {
             const u64 um = ballot(clk - birth >= AGE_CAP) & alive;
             if (um) {
-                acc.undet += __popcll(um); acc.done += __popcll(um); acc.dirty = true;
+                for (u64 rem = um; rem;) {
+                    u64 mr = rem;
+                    if constexpr (MULTI) {
+                        const u32 r = lanev(greq, (u32)(__ffsll((long long)rem) - 1));
+                        mr = ballot(greq == r) & rem;
+                        acc.select(r, reqs);
+                    }
+                    rem &= ~mr;
+                    acc.undet += __popcll(mr); acc.done += __popcll(mr); acc.dirty = true;
+                }
                 alive &= ~um;
             }
         }
         if (more && (u32)__popcll(~alive) >= min_dead) refill();
     }
-    u64 tsum = lturns;
-    u32 tmax = lmax;
-    for (int off = 32; off > 0; off >>= 1) {
-        tsum += __shfl_xor(tsum, off);
-        const u32 o = __shfl_xor(tmax, off);
-        tmax = o > tmax ? o : tmax;
+    if constexpr (!MULTI) {    // MULTI batches accounted turns and longest games at each span end
+        u64 tsum = lturns;
+        u32 tmax = lmax;
+        for (int off = 32; off > 0; off >>= 1) {
+            tsum += __shfl_xor(tsum, off);
+            const u32 o = __shfl_xor(tmax, off);
+            tmax = o > tmax ? o : tmax;
+        }
+        acc.turns += tsum;
+        acc.maxfin = tmax > acc.maxfin ? tmax : acc.maxfin;
+        acc.dirty = true;
     }
-    acc.turns += tsum;
-    acc.maxfin = tmax > acc.maxfin ? tmax : acc.maxfin;
-    acc.dirty = true;
     acc.flush(reqs);
     if (WALK && acc.pushed && lane == 0) atomicAdd((unsigned long long *)&A.pipe->ctotal, (unsigned long long)acc.pushed);
 }

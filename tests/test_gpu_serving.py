@@ -69,3 +69,28 @@ def test_serving_refuses_pair_detector(eng):
             eng.submit(14, "canon", 0, 1000, detector="pair")
     finally:
         eng.serve_stop()
+
+
+@pytest.mark.parametrize("stage_kind", ["flow", "lockstep"])
+def test_mixed_detectors_on_both_pipelines(eng, stage_kind):
+    """The mixed batches of the serving loop on the flow kernel (the default) and on the staged
+    pipeline, with the alternation detector on the odd-n rule-0 queries: each result equals the same
+    request alone (single-request flow batches)."""
+    qs = []
+    for n, mode, lo, hi, rule, kw in queries(11, 36):
+        kw = dict(kw)
+        if n % 2 == 1 and rule == 0 and len(qs) % 2 == 0:
+            kw["detector"] = "alt"
+        qs.append((n, mode, lo, hi, rule, kw))
+    alone = [eng.run_range(n, mode, lo, hi, rule=rule, **kw) for n, mode, lo, hi, rule, kw in qs]
+    eng.set_tuning(stage_kind=stage_kind)
+    eng.serve_start(max_slots=12, step_positions=1 << 21)
+    try:
+        ids = [eng.submit(n, mode, lo, hi, rule=rule, **kw) for n, mode, lo, hi, rule, kw in qs]
+        mixed = [eng.wait(i) for i in ids]
+    finally:
+        eng.serve_stop()
+        eng.set_tuning()
+    for q, a, m in zip(qs, alone, mixed):
+        for k in KEYS:
+            assert a[k] == m[k], (stage_kind, q, k, a[k], m[k])
