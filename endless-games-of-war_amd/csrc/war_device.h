@@ -406,6 +406,18 @@ __device__ __forceinline__ St<W> canon(typename Lay<W>::P q, int n, int k) {
     return s;
 }
 
+// The canonical position of canon<4> directly in turn_top's form (top card in the high nibble),
+// from the order reversed (qr = rev_fields4(q): q0 in the top nibble): player 1 = [n-1, q0..q(k-1)]
+// from the top, player 2 = [qk..q(n-2)]. k <= n - 2 <= 14, so no shift reaches 64.
+__device__ __forceinline__ St<4> canon_top(u64 qr, int n, int k) {
+    St<4> s;
+    const u32 sk = 4u * (u32)k;
+    s.a = ((qr >> 4) | ((u64)(u32)(n - 1) << 60)) & (~0ull << (60u - sk));
+    s.b = qr << sk;
+    s.sa = sk;
+    return s;
+}
+
 // --half filter (gpu/war_gpu.py:151-154): true if card n-1 sits at an even deck position.
 template <int W>
 __device__ __forceinline__ bool top_card_to_p1(typename Lay<W>::T q, int n) {

@@ -1037,6 +1037,7 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
     // ---- the pool: lane l holds the current position of its run, already dealt (the deal of a
     // whole generation of 64 runs as one wave step)
     P q = 0;
+    u64 qr = 0;                // 4-bit canonical runs: the order reversed (canon_top builds from it)
     u32 k = 0, left = 0;       // this lane's run: current order / split; positions left, it included
     St<W> ps = {};             // the current position, dealt
     u64 pp = 0;                // its index (cycle walks rebuild the start from it)
@@ -1044,8 +1045,15 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
     u32 ptr = 64;              // next pool entry to hand out (64: all handed out)
     bool more = true;          // positions remain (in the pool or in tiles not yet taken)
     auto step = [&]() {
-        if (mode == 2) { if (++k == n - 1u) { k = 0; q = next_perm<PW>(q, (int)n - 1); } }
-        else q = next_perm<PW>(q, (int)n);
+        if (mode == 2) {
+            if (++k == n - 1u) {
+                k = 0;
+                q = next_perm<PW>(q, (int)n - 1);
+                if constexpr (W == 4) qr = rev_fields4(q);
+            }
+        } else {
+            q = next_perm<PW>(q, (int)n);
+        }
         if constexpr (WALK) pp++;
     };
     // --half (gpu/war_gpu.py:150-154) keeps the deals whose card n-1 goes to player 1
@@ -1075,6 +1083,7 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
                     const u64 r = first / m;
                     k = (u32)(first - r * m);
                     q = unrank<PW>(r, (int)m);
+                    if constexpr (W == 4) qr = rev_fields4(q);
                 } else {
                     k = 0;
                     q = unrank<PW>(first, (int)n);
@@ -1084,8 +1093,11 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
             skip_filtered();
         }
         if (left) {
-            ps = (mode == 2) ? canon<W>(q, (int)n, (int)k) : deal<W>(q, (int)n);
-            if constexpr (W == 4) ps = flip_piles(ps);     // 4-bit games play in turn_top's form
+            if constexpr (W == 4) {                          // 4-bit games play in turn_top's form
+                ps = (mode == 2) ? canon_top(qr, (int)n, (int)k) : flip_piles(deal<W>(q, (int)n));
+            } else {
+                ps = (mode == 2) ? canon<W>(q, (int)n, (int)k) : deal<W>(q, (int)n);
+            }
         }
         ptr = 0;
         return true;
