@@ -418,6 +418,41 @@ __device__ __forceinline__ St<4> canon_top(u64 qr, int n, int k) {
     return s;
 }
 
+// The 24 deals whose orders share the first n-4 cards (a block of 24 consecutive ranks that starts
+// at a multiple of 24) differ in the last four deck positions only: Heap's algorithm visits them
+// with one swap of two of those positions per step. Step t (0..22) swaps positions i, j of the
+// four: (0,1) at even t, (t/6, 3) at t = 5 mod 6, (0,2) else. Deck position p sits in player 1's
+// pile (p even) or player 2's (p odd) at depth p/2, which in turn_top's form is bits 28-4(p/2) of
+// the pile's high word (n <= 16: depth <= 7). n >= 5; t and n are wave-uniform.
+__device__ __forceinline__ void heap_swap_top(St<4> &s, u32 t, u32 n) {
+    const u32 i = (t & 1u) == 0u ? 0u : (t % 6u == 5u ? t / 6u : 0u);
+    const u32 j = (t & 1u) == 0u ? 1u : (t % 6u == 5u ? 3u : 2u);
+    const u32 pi = n - 4u + i, pj = n - 4u + j;
+    const u32 si = 28u - 4u * (pi >> 1), sj = 28u - 4u * (pj >> 1);
+    u32 ah = (u32)(s.a >> 32), bh = (u32)(s.b >> 32);
+    if (((pi ^ pj) & 1u) == 0u) {              // both in one pile
+        u32 w = (pi & 1u) ? bh : ah;
+        const u32 d = ((w >> si) ^ (w >> sj)) & 15u;
+        w ^= (d << si) | (d << sj);
+        if (pi & 1u) bh = w; else ah = w;
+    } else {                                   // one in each
+        const u32 s1 = (pi & 1u) ? sj : si, s2 = (pi & 1u) ? si : sj;
+        const u32 d = ((ah >> s1) ^ (bh >> s2)) & 15u;
+        ah ^= d << s1;
+        bh ^= d << s2;
+    }
+    s.a = (s.a & 0xFFFFFFFFull) | ((u64)ah << 32);
+    s.b = (s.b & 0xFFFFFFFFull) | ((u64)bh << 32);
+}
+
+// The first order of the next block of 24 from the first order of this one (its last four cards
+// ascending): the block's last order has them descending, and its successor is next_perm's.
+__device__ __forceinline__ u64 next_block24(u64 q, int n) {
+    const u32 sh = 4u * (u32)(n - 4);
+    const u64 tail = (rev_fields4(q) >> (4u * (16u - (u32)n))) & 0xFFFFull;   // cards n-1..n-4
+    return next_perm<4>((q & ((1ull << sh) - 1ull)) | (tail << sh), n);
+}
+
 // --half filter (gpu/war_gpu.py:151-154): true if card n-1 sits at an even deck position.
 template <int W>
 __device__ __forceinline__ bool top_card_to_p1(typename Lay<W>::T q, int n) {

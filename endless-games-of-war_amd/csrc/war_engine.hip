@@ -114,9 +114,10 @@ constexpr u32 PLAIN_CYC_TILE = 1024;    // home-tile size of the plain kernel's 
 
 struct GraphKey {
     int W, rule, kernel, wpc, per, cap0, walk, multi, hb, lb, kind, det;   // det: WAR_DET_* of the persistent path
+    int blk;                                                                // flow kernel over whole blocks of 24 deals
     bool operator<(const GraphKey &o) const {
-        return std::tie(W, rule, kernel, wpc, per, cap0, walk, multi, hb, lb, kind, det) <
-               std::tie(o.W, o.rule, o.kernel, o.wpc, o.per, o.cap0, o.walk, o.multi, o.hb, o.lb, o.kind, o.det);
+        return std::tie(W, rule, kernel, wpc, per, cap0, walk, multi, hb, lb, kind, det, blk) <
+               std::tie(o.W, o.rule, o.kernel, o.wpc, o.per, o.cap0, o.walk, o.multi, o.hb, o.lb, o.kind, o.det, o.blk);
     }
 };
 
@@ -456,16 +457,17 @@ void launch_refill(int flags, int blocks, size_t lds, hipStream_t st, const Stag
 }
 
 // The flow kernel for a batch's detector (WAR_DET_*): Brent for either rule, the alternation and
-// pair-run detectors for rule 0 (pair-run: single requests, layouts 4 and 5).
-template <int W, int RULE, bool WALK, bool MULTI>
+// pair-run detectors for rule 0 (pair-run: single requests, layouts 4 and 5). BLK: the variant for
+// single 4-bit deal requests whose batch and lane runs are whole blocks of 24 deals.
+template <int W, int RULE, bool WALK, bool MULTI, bool BLK = false>
 void launch_flow(int det, int blocks, size_t lds, hipStream_t st, const StageArgs<W> &A) {
     if (det == WAR_DET_BRENT) {
-        hipLaunchKernelGGL((k_flow<W, RULE, FLOW_BRENT, WALK, MULTI>), dim3(blocks), dim3(256), lds, st, A);
+        hipLaunchKernelGGL((k_flow<W, RULE, FLOW_BRENT, WALK, MULTI, BLK>), dim3(blocks), dim3(256), lds, st, A);
     } else if constexpr (RULE == 0) {
         if (det == WAR_DET_ALT)
-            hipLaunchKernelGGL((k_flow<W, 0, FLOW_ALT, WALK, MULTI>), dim3(blocks), dim3(256), lds, st, A);
+            hipLaunchKernelGGL((k_flow<W, 0, FLOW_ALT, WALK, MULTI, BLK>), dim3(blocks), dim3(256), lds, st, A);
         else if constexpr (!MULTI && W != 6)
-            hipLaunchKernelGGL((k_flow<W, 0, FLOW_PAIR, WALK, false>), dim3(blocks), dim3(256), lds, st, A);
+            hipLaunchKernelGGL((k_flow<W, 0, FLOW_PAIR, WALK, false, BLK>), dim3(blocks), dim3(256), lds, st, A);
     }
 }
 
@@ -493,6 +495,8 @@ int build_graph(Engine &E, const GraphKey &key, hipGraphExec_t *out) {
                 if (key.multi) launch_flow<W, RULE, true, true>(key.det, blocks, lds, E.st, A);
                 else launch_flow<W, RULE, true, false>(key.det, blocks, lds, E.st, A);
             }
+        } else if (key.blk) {
+            if constexpr (W == 4) launch_flow<4, RULE, false, false, true>(key.det, blocks, lds, E.st, A);
         } else {
             if (key.multi) launch_flow<W, RULE, false, true>(key.det, blocks, lds, E.st, A);
             else launch_flow<W, RULE, false, false>(key.det, blocks, lds, E.st, A);
@@ -542,7 +546,7 @@ int build_graph(Engine &E, const GraphKey &key, hipGraphExec_t *out) {
 // One batch of jobs (all of layout W and rule RULE) through the persistent pipeline.
 template <int W, int RULE>
 int run_persistent_batch(Engine &E, std::vector<Job> &jobs, const war_tuning &t, int hb, int lb, bool walk,
-                         bool multi, float *ms, int det) {
+                         bool multi, float *ms, int det, bool blk = false) {
     if (jobs.empty()) return 0;
     if ((int)jobs.size() > MAXJOBS) return fail(31, "too many jobs in one batch");
     const bool alt = det == WAR_DET_ALT;
@@ -560,7 +564,7 @@ int run_persistent_batch(Engine &E, std::vector<Job> &jobs, const war_tuning &t,
     // flow: cap0 holds the refill threshold (free lanes of 64), brent_cap's stage windows unused
     const int cap0 = flow ? FLOW_MIN_FREE : t.brent_cap;
     GraphKey key{W, RULE, 1, t.waves_per_cu, t.ranks_per_refill / 64, cap0, walk ? 1 : 0, multi ? 1 : 0,
-                 round_pow2(hb), round_pow2(lb), kind, det};
+                 round_pow2(hb), round_pow2(lb), kind, det, (blk && flow && W == 4 && !walk && !multi) ? 1 : 0};
     if ((!flow || walk) && (tiles > qbufs<W>(E).tmax || tiles * tile_pos > qbufs<W>(E).cap))
         return fail(32, "batch exceeds the queue capacity");
     JobHdr h{};
@@ -639,9 +643,9 @@ int run_plain_batch(Engine &E, ReqHost &h, u64 lo, u64 hi, const war_tuning &t, 
 
 template <int W>
 int run_batch_dispatch(Engine &E, int rule, std::vector<Job> &jobs, const war_tuning &t, int hb, int lb, bool walk,
-                       bool multi, float *ms, int det = WAR_DET_BRENT) {
-    return rule ? run_persistent_batch<W, 1>(E, jobs, t, hb, lb, walk, multi, ms, WAR_DET_BRENT)
-                : run_persistent_batch<W, 0>(E, jobs, t, hb, lb, walk, multi, ms, det);
+                       bool multi, float *ms, int det = WAR_DET_BRENT, bool blk = false) {
+    return rule ? run_persistent_batch<W, 1>(E, jobs, t, hb, lb, walk, multi, ms, WAR_DET_BRENT, blk)
+                : run_persistent_batch<W, 0>(E, jobs, t, hb, lb, walk, multi, ms, det, blk);
 }
 
 int alloc_slot(Engine &E) {
@@ -658,8 +662,11 @@ int run_range_batch(Engine &E, ReqHost &h, int W, u64 lo, u64 hi, const war_tuni
     jobs[0].lo = lo; jobs[0].hi = hi; jobs[0].req = (u32)h.slot;
     float ms = 0;
     const int det = r.detector;
+    // deals in whole blocks of 24, lane runs too: the flow kernel's block-stepping variant
+    const bool blk = W == 4 && r.mode == WAR_MODE_FULL && r.n >= 5 && lo % 24 == 0 && hi % 24 == 0 &&
+                     (t.ranks_per_refill / 64) % 24 == 0;
     int rc;
-    if (W == 4)      rc = run_batch_dispatch<4>(E, r.rule, jobs, t, r.hbins, r.lbins, h.dev.walk != 0, false, &ms, det);
+    if (W == 4)      rc = run_batch_dispatch<4>(E, r.rule, jobs, t, r.hbins, r.lbins, h.dev.walk != 0, false, &ms, det, blk);
     else if (W == 6) rc = run_batch_dispatch<6>(E, r.rule, jobs, t, r.hbins, r.lbins, false, false, &ms, det);
     else             rc = run_batch_dispatch<5>(E, r.rule, jobs, t, r.hbins, r.lbins, h.dev.walk != 0, false, &ms, det);
     *dev += ms;
@@ -695,7 +702,7 @@ int run_request_locked(Engine &E, ReqHost &h) {
         const bool flow = t.stage_kind == 3;
         war_tuning bt = t;
         u64 bstep = E.batch;
-        u64 lo = r.lo;
+        u64 lo = r.lo, hi = r.hi;    // batches cover [lo, hi); a tail [hi, r.hi) runs after them
         if (flow) {
             // the flow kernel needs no queues, so its batches are larger (fewer batch ends); with
             // cycle walks the cycle records bound them as in the staged pipeline
@@ -714,9 +721,24 @@ int run_request_locked(Engine &E, ReqHost &h) {
                     rc = run_range_batch(E, h, W, lo, a, bt, &dev);
                     lo = a;
                 }
+            } else if (r.mode == WAR_MODE_FULL && W == 4 && r.n >= 5 && !h.dev.walk) {
+                // deals: lane runs of whole blocks of 24 (the orders that share all but the last
+                // four cards), which the kernel steps through with one swap of two dealt cards
+                // each; the parts before the first and after the last block boundary run apart
+                const u64 per = std::max<u64>(1, ((u64)bt.ranks_per_refill / 64 + 12) / 24) * 24;
+                if (64 * per <= MAX_TILE) bt.ranks_per_refill = (int)(64 * per);
+                const u64 tp = (u64)bt.ranks_per_refill;
+                bstep = bstep / tp * tp;
+                const u64 a = (lo + 23) / 24 * 24, b = r.hi / 24 * 24;
+                if (a < b) {
+                    if (a > lo) rc = run_range_batch(E, h, W, lo, a, bt, &dev);
+                    lo = a;
+                    hi = b;
+                }
             }
         }
-        for (; lo < r.hi && !rc; lo += bstep) rc = run_range_batch(E, h, W, lo, std::min<u64>(r.hi, lo + bstep), bt, &dev);
+        for (; lo < hi && !rc; lo += bstep) rc = run_range_batch(E, h, W, lo, std::min<u64>(hi, lo + bstep), bt, &dev);
+        if (hi < r.hi && !rc) rc = run_range_batch(E, h, W, hi, r.hi, bt, &dev);
     }
     h.dev_ms = dev;
     if (!rc) rc = req_collect(E, h);

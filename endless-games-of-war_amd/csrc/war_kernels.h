@@ -1007,8 +1007,9 @@ constexpr u32 FLOW_SPAN = ALT_K * FLOW_BLOCKS;
 // MULTI: the batch holds several requests (the serving loop's steps). A tile belongs to one job,
 // so the pool's positions share one request (its n, mode and C change at a tile grab); a game
 // keeps its request and C per lane, and accounting groups the finished lanes by request.
-template <int W, int RULE, int DET, bool WALK, bool MULTI>
+template <int W, int RULE, int DET, bool WALK, bool MULTI, bool BLK = false>
 __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
+    static_assert(!BLK || (W == 4 && !WALK && !MULTI), "deal blocks: 4-bit single requests without walks");
     static_assert(DET == FLOW_BRENT || ((DET == FLOW_ALT || DET == FLOW_PAIR) && RULE == 0),
                   "the alternation and pair-run detectors are for rule 0");
     static_assert(!WALK || W != 6, "no cycle walks in layout 6");
@@ -1024,7 +1025,7 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
     const u32 req = A.hdr->req0;
     acc.select(req, reqs);
     // the pool's request (all of a batch with one request; the current tile's with MULTI)
-    u32 preq = req, n = reqs[req].n, mode = reqs[req].mode;
+    u32 preq = req, n = reqs[req].n, mode = BLK ? 0u : reqs[req].mode;
     bool walk = WALK && reqs[req].walk != 0;   // cycles go to the cycle records for k_walk
     u32 C = c_of<W>(n);
     const u32 per = A.per, TP = A.tile_pos;
@@ -1060,9 +1061,20 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
     auto skip_filtered = [&]() {
         if (mode == 1) while (left && !top_card_to_p1<PW>(q, (int)n)) { if (--left) step(); }
     };
+    // BLK: every lane run is whole blocks of 24 deals (the orders that share all but the last four
+    // cards; the host aligns jobs and runs to them): a run steps through each block in Heap's order,
+    // swapping two cards of the dealt position in place (heap_swap_top), and deals from the order
+    // only at a block's start (next_block24)
+    u32 hj = 0;                // the step within the block (wave-uniform)
     // every lane to the next position of its run; a new tile once every run is done
     auto advance = [&]() -> bool {
-        if (left > 1u) step();
+        bool fresh = true;     // deal ps from q (else heap_swap_top moved it)
+        if constexpr (BLK) {
+            if (hj < 23u) { heap_swap_top(ps, hj, n); hj++; fresh = false; }
+            else { if (left > 1u) q = next_block24(q, (int)n); hj = 0; }
+        } else {
+            if (left > 1u) step();
+        }
         left = left ? left - 1u : 0u;
         skip_filtered();
         while (ballot(left != 0u) == 0ull) {
@@ -1090,9 +1102,11 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
                 }
             }
             if constexpr (WALK) { pp = first; tile = (u32)g0; }
+            hj = 0;
+            fresh = true;
             skip_filtered();
         }
-        if (left) {
+        if (left && fresh) {
             if constexpr (W == 4) {                          // 4-bit games play in turn_top's form
                 ps = (mode == 2) ? canon_top(qr, (int)n, (int)k) : flip_piles(deal<W>(q, (int)n));
             } else {

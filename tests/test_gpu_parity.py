@@ -114,6 +114,23 @@ def test_flow_brent_sampled(eng, orc):
             check(eng, orc, n, mode, lo, lo + ln, rule=rule, tuning=tuning)
 
 
+@pytest.mark.parametrize("n", [5, 6, 7, 9, 11, 12, 13, 14, 15, 16])
+def test_flow_deal_blocks(eng, orc, n):
+    """The flow kernel steps 4-bit deals through blocks of 24 orders that differ in their last four
+    cards, one swap per deal: ranges on and off a block boundary, ending mid-block, of every lane-run
+    length the tile sizes give, with both detectors."""
+    rng = random.Random(24 * n)
+    total = orc.total(n, "full")
+    for ln in (1, 23, 24, 25, 47, 1535, 1537, 100_003):
+        if ln > total:
+            continue
+        lo = min(rng.randrange(0, total - ln) // 24 * 24 + rng.choice([0, 1, 23]), total - ln)
+        for tuning in (None, {"ranks_per_refill": 64}, {"ranks_per_refill": 8192}):
+            for det in (["brent", "alt"] if n % 2 else ["brent"]):
+                print("case deal blocks", n, lo, ln, tuning, det, flush=True)
+                check(eng, orc, n, "full", lo, lo + ln, tuning=tuning, detector=det)
+
+
 def test_5bit_ranges(eng, orc):
     for n, lo, ln in [(17, 10**14, 200_000), (17, 0, 100_000), (18, 5 * 10**15, 50_000), (20, 10**18, 20_000)]:
         for kernel in ("plain", "persistent"):
