@@ -55,10 +55,10 @@ VALU_OPS_PER_TURN = 17
 # 17 x 4 / 58 = 1.17 instructions per SIMD per 4 clocks, 0.586 of the dual-issue peak.
 VALU_LANE_OPS_PER_CU_CLOCK = 128
 TURN_MIX_CEILING = (VALU_OPS_PER_TURN * 4.0 / (12 * 4 + 5 * 2)) / 2.0
-# HBM bytes per k_flow launch on the n = 13 workload (2 launches; no records are written: 0.31 MB
-# read, 98.9 MB of tile-grab atomics and counters), from the rocprofv3 byte counters calibrated on a
+# HBM bytes per k_flow launch on the n = 13 workload (2 launches; no records are written: 0.20 MB
+# read, 66.5 MB of tile-grab atomics and counters), from the rocprofv3 byte counters calibrated on a
 # copy of known size.
-K_FLOW_HBM_BYTES_PER_LAUNCH = 9.92e7
+K_FLOW_HBM_BYTES_PER_LAUNCH = 6.67e7
 
 
 def log(*a):
@@ -193,6 +193,8 @@ def main():
         "config": {"workload": "n13_full: all 13! = 6227020800 deals, flow kernel (persistent), as war_gpu.py 13 "
                                "runs it", "n": 13, "mode": "full", "rule": 0, "positions": n13,
                    "kernel": "persistent", "detector": DET13, "tuning": tun,
+                   "tiles": "ranks_per_refill rounded to whole blocks of 24 deals per lane run (1536 per "
+                            "tile); the kernel's deal-block variant steps each block with one card swap per deal",
                    "parity": "a(13), longest 57, cycle-length histogram equal the published values every step"},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": peak_lane_ops / 1e12,
                      "unit": "Tlane-op/s", "frac": achieved / peak_lane_ops, "traffic": K_FLOW_HBM_BYTES_PER_LAUNCH,
