@@ -953,15 +953,15 @@ __global__ __launch_bounds__(256) void k_refill(StageArgs<W> A) {
 }
 
 // ------------------------------------------------------------------------------------------
-// Flow kernel: no stages (one request per batch). Every lane plays its own game,
-// and a lane whose game is over takes the next position at the start of the next span of
-// FLOW_SPAN turns, so a wave never waits for the slowest game of a group and no survivor records
-// are written. Births fall on span starts (the clock a multiple of FLOW_SPAN); the age is clock -
-// birth and the cycle length clock - R, per lane. Because every age is a multiple of FLOW_SPAN at a
+// Flow kernel: no stages. Every lane plays its own game, and a lane whose game is over takes the
+// next position at the start of the next span of SPAN turns (FLOW_SPAN; twice that for Brent over
+// deal blocks), so a wave never waits for the slowest game of a group and no survivor records
+// are written. Births fall on span starts (the clock a multiple of SPAN); the age is clock -
+// birth and the cycle length clock - R, per lane. Because every age is a multiple of SPAN at a
 // span start, each detector's tortoise moves fall on fixed turns of the span:
 // * Brent (DET 0, any n and rule): the tortoise takes the state at ages 2^k - 1 (after that turn's
 //   repeat test), exactly as the reference's war_sim (gpu/war_gpu.py:77-96), so `turns` is the
-//   reference's count. Age 2^k - 1 is turn 2^k - 2 of the lane's first span, and turn FLOW_SPAN - 2
+//   reference's count. Age 2^k - 1 is turn 2^k - 2 of the lane's first span, and turn SPAN - 2
 //   of a later span (when age + 1 is a power of 2).
 // * the alternation detector (DET 2, odd n, rule 0): checkpoints every ALT_K turns.
 // Positions come from tiles of the job list as in the first stage: lane l of a tile owns positions
@@ -1128,9 +1128,14 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
     u32 ghome = 0;
     u64 alive = 0, newm = 0, prevm = 0, streakm = 0, endm = 0, cycm = 0;
     u64 preveqm = 0, youngm = 0;   // pair-run detector: last turn's same-winner mask; games <= PAIR_P old
-    constexpr int FB = __builtin_ctz(FLOW_SPAN);
+    // the span: FLOW_SPAN turns (two alternation checkpoints); Brent over deal blocks runs spans
+    // twice as long (long games: half the span ends, 4 % faster at n = 13; canonical positions,
+    // most of which end within 30 turns, lose more to lanes idle until the span end, and the
+    // alternation detector measured slower with 32)
+    constexpr u32 SPAN = (BLK && DET == FLOW_BRENT) ? 2u * FLOW_SPAN : FLOW_SPAN;
+    constexpr int FB = __builtin_ctz(SPAN);
     static_assert(FLOW_SPAN == 2u * ALT_K && PAIR_P % FLOW_SPAN == 0u, "checkpoints at turns 7 and 15 of a span");
-    static_assert((FLOW_SPAN & (FLOW_SPAN - 1u)) == 0u && (AGE_CAP % FLOW_SPAN) == 0u,
+    static_assert((SPAN & (SPAN - 1u)) == 0u && (AGE_CAP % SPAN) == 0u && SPAN <= 1024u,
                   "span: a power of 2 dividing AGE_CAP");
     u64 fb[FB];                // the turn of the span each finished game ended in, bit by bit
     for (int b = 0; b < FB; b++) fb[b] = 0;
@@ -1236,7 +1241,7 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
             youngm = ballot(clk0 - birth <= PAIR_P - FLOW_SPAN);
         }
 #pragma unroll
-        for (u32 tix = 0; tix < FLOW_SPAN; tix++) {       // turn tix of the span
+        for (u32 tix = 0; tix < SPAN; tix++) {       // turn tix of the span
             u64 over;
             bool p1;
             if constexpr (W == 4) p1 = turn_top<RULE, !MULTI>(s, sb, MULTI ? gC : C, zt, over);
@@ -1277,10 +1282,10 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
             }
             if constexpr (DET == FLOW_BRENT) {
                 // Brent's tortoise moves at ages 2^k - 1: turn 2^k - 2 of a lane's first span; turn
-                // FLOW_SPAN - 2 of a later span when age + 1 is a power of 2
+                // SPAN - 2 of a later span when age + 1 is a power of 2
                 if (((tix + 2u) & (tix + 1u)) == 0u) {           // folded: the loop is unrolled
                     u64 jm = newm & alive;
-                    if (tix == FLOW_SPAN - 2u) {
+                    if (tix == SPAN - 2u) {
                         const u32 age1 = clk0 + tix + 2u - birth;    // age + 1
                         jm |= ballot((age1 & (age1 - 1u)) == 0u) & alive;
                     }
@@ -1306,7 +1311,7 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
                 }
             }
         }
-        clk += FLOW_SPAN;
+        clk += SPAN;
         newm = 0;
         endm = alive0 & ~alive & ~cycm;
         if (endm | cycm) settle(clk0);
