@@ -39,12 +39,46 @@ using namespace war;
 #define I_SHL32(i) "v_lshlrev_b32 %" #i ", %8, %" #i "\n"
 #define I_SHR64(i) "v_lshrrev_b64 %" #i ", %8, %" #i "\n"
 #define I_LSHLADD(i) "v_lshl_add_u64 %" #i ", %" #i ", 4, %8\n"
+// further forms (K >= 10): candidates for rebuilding the turn around dual-issuable instructions
+#define I_XOR(i) "v_xor_b32 %" #i ", %" #i ", %8\n"
+#define I_SUB(i) "v_sub_u32 %" #i ", %" #i ", %8\n"
+#define I_MOV(i) "v_mov_b32 %" #i ", %8\n"
+#define I_MAX(i) "v_max_u32 %" #i ", %" #i ", %8\n"
+#define I_MIN(i) "v_min_u32 %" #i ", %" #i ", %8\n"
+#define I_CNDVCC(i) "v_cndmask_b32_e32 %" #i ", %" #i ", %8, vcc\n"
+#define I_SHR32(i) "v_lshrrev_b32 %" #i ", %8, %" #i "\n"
+#define I_SHR16(i) "v_lshrrev_b16 %" #i ", %8, %" #i "\n"
+#define I_PKSHR16(i) "v_pk_lshrrev_b16 %" #i ", %8, %" #i "\n"
+#define I_PKADD16(i) "v_pk_add_u16 %" #i ", %" #i ", %8\n"
+#define I_BFI(i) "v_bfi_b32 %" #i ", %8, %" #i ", %8\n"
+#define I_PERM(i) "v_perm_b32 %" #i ", %" #i ", %8, %9\n"
+#define I_MOV64(i) "v_mov_b64 %" #i ", %8\n"
+#define I_ADDS(i) "v_add_u32 %" #i ", %9, %" #i "\n"
+#define I_ANDLIT(i) "v_and_b32 %" #i ", 0xf0000000, %" #i "\n"
+#define I_CMPVCC(i) "v_cmp_gt_u32 vcc, %" #i ", %8\n"
+#define I_SUBCOVCC(i) "v_sub_co_u32 %" #i ", vcc, %8, %" #i "\n"
+#define I_ADDC(i) "v_addc_co_u32 %" #i ", vcc, %" #i ", %8, vcc\n"
+#define I_CND0(i) "v_cndmask_b32_e64 %" #i ", 0, %" #i ", %9\n"
+#define I_ADD3(i) "v_add3_u32 %" #i ", %" #i ", %8, %8\n"
+#define I_OR3(i) "v_or3_b32 %" #i ", %" #i ", %8, %8\n"
+#define I_MULU24(i) "v_mul_u32_u24 %" #i ", %" #i ", %8\n"
+#define I_SHL64(i) "v_lshlrev_b64 %" #i ", %8, %" #i "\n"
+#define I_ALIGN(i) "v_alignbit_b32 %" #i ", %" #i ", %8, 4\n"
+#define I_CMP64VCC(i) "v_cmp_eq_u64 vcc, %" #i ", %8\n"
 
-constexpr int NFORM = 10;
+constexpr int NFORM = 35;
 const char *form_name[NFORM] = {"v_or_b32", "v_and_b32 (inline constant)", "v_add_u32", "v_lshl_or_b32",
                                 "v_cndmask_b32_e64 (scalar mask)", "v_lshlrev_b32", "v_lshrrev_b64",
                                 "v_lshl_add_u64", "v_cmp_gt_u32 (to a scalar pair)",
-                                "v_sub_co_u32 (borrow to a scalar pair)"};
+                                "v_sub_co_u32 (borrow to a scalar pair)",
+                                "v_xor_b32", "v_sub_u32", "v_mov_b32", "v_max_u32", "v_min_u32",
+                                "v_cndmask_b32_e32 (vcc)", "v_lshrrev_b32", "v_lshrrev_b16", "v_pk_lshrrev_b16",
+                                "v_pk_add_u16", "v_bfi_b32", "v_perm_b32", "v_mov_b64",
+                                "v_add_u32 (scalar operand)", "v_and_b32 (literal 0xf0000000)",
+                                "v_cmp_gt_u32 (to vcc)", "v_sub_co_u32 (borrow to vcc)",
+                                "v_addc_co_u32 (carry in and out in vcc)", "v_cndmask_b32_e64 (0, scalar mask)",
+                                "v_add3_u32", "v_or3_b32", "v_mul_u32_u24", "v_lshlrev_b64", "v_alignbit_b32",
+                                "v_cmp_eq_u64 (to vcc)"};
 
 template <int K>
 __global__ __launch_bounds__(256) void k_rate(u64 *out, int iters, u32 seed) {
@@ -77,6 +111,31 @@ __global__ __launch_bounds__(256) void k_rate(u64 *out, int iters, u32 seed) {
                              : "v"(c[0]), "v"(c[1]), "v"(c[2]), "v"(c[3]), "v"(x), "v"(sh));
                 m ^= s0 ^ s1 ^ s2 ^ s3;
             }
+            if constexpr (K == 10) asm volatile(X8(I_XOR) : C32 : "v"(x));
+            if constexpr (K == 11) asm volatile(X8(I_SUB) : C32 : "v"(x));
+            if constexpr (K == 12) asm volatile(X8(I_MOV) : C32 : "v"(x));
+            if constexpr (K == 13) asm volatile(X8(I_MAX) : C32 : "v"(x));
+            if constexpr (K == 14) asm volatile(X8(I_MIN) : C32 : "v"(x));
+            if constexpr (K == 15) asm volatile("s_mov_b64 vcc, %9\n" X8(I_CNDVCC) : C32 : "v"(x), "s"(m) : "vcc");
+            if constexpr (K == 16) asm volatile(X8(I_SHR32) : C32 : "v"(sh));
+            if constexpr (K == 17) asm volatile(X8(I_SHR16) : C32 : "v"(sh));
+            if constexpr (K == 18) asm volatile(X8(I_PKSHR16) : C32 : "v"(sh));
+            if constexpr (K == 19) asm volatile(X8(I_PKADD16) : C32 : "v"(x));
+            if constexpr (K == 20) asm volatile(X8(I_BFI) : C32 : "v"(x));
+            if constexpr (K == 21) asm volatile(X8(I_PERM) : C32 : "v"(x), "s"(0x05010400u));
+            if constexpr (K == 22) asm volatile(X8(I_MOV64) : C64 : "v"(y));
+            if constexpr (K == 23) asm volatile(X8(I_ADDS) : C32 : "v"(x), "s"(seed));
+            if constexpr (K == 24) asm volatile(X8(I_ANDLIT) : C32);
+            if constexpr (K == 25) asm volatile(X8(I_CMPVCC) : C32 : "v"(x) : "vcc");
+            if constexpr (K == 26) asm volatile(X8(I_SUBCOVCC) : C32 : "v"(x) : "vcc");
+            if constexpr (K == 27) asm volatile("s_mov_b64 vcc, 0\n" X8(I_ADDC) : C32 : "v"(x) : "vcc");
+            if constexpr (K == 28) asm volatile(X8(I_CND0) : C32 : "v"(x), "s"(m));
+            if constexpr (K == 29) asm volatile(X8(I_ADD3) : C32 : "v"(x));
+            if constexpr (K == 30) asm volatile(X8(I_OR3) : C32 : "v"(x));
+            if constexpr (K == 31) asm volatile(X8(I_MULU24) : C32 : "v"(x));
+            if constexpr (K == 32) asm volatile(X8(I_SHL64) : C64 : "v"(sh));
+            if constexpr (K == 33) asm volatile(X8(I_ALIGN) : C32 : "v"(x));
+            if constexpr (K == 34) asm volatile(X8(I_CMP64VCC) : C64 : "v"(y) : "vcc");
             if constexpr (K == 9) {
                 u64 s0;
                 asm volatile("v_sub_co_u32 %0, %8, %9, %0\n v_sub_co_u32 %1, %8, %9, %1\n v_sub_co_u32 %2, %8, %9, %2\n"
@@ -170,9 +229,13 @@ int main(int argc, char **argv) {
         }
         return 0;
     };
-    void (*rates[NFORM])(int, u64 *, int) = {launch_rate<0>, launch_rate<1>, launch_rate<2>, launch_rate<3>,
-                                            launch_rate<4>, launch_rate<5>, launch_rate<6>, launch_rate<7>,
-                                            launch_rate<8>, launch_rate<9>};
+    void (*rates[NFORM])(int, u64 *, int) = {
+        launch_rate<0>,  launch_rate<1>,  launch_rate<2>,  launch_rate<3>,  launch_rate<4>,  launch_rate<5>,
+        launch_rate<6>,  launch_rate<7>,  launch_rate<8>,  launch_rate<9>,  launch_rate<10>, launch_rate<11>,
+        launch_rate<12>, launch_rate<13>, launch_rate<14>, launch_rate<15>, launch_rate<16>, launch_rate<17>,
+        launch_rate<18>, launch_rate<19>, launch_rate<20>, launch_rate<21>, launch_rate<22>, launch_rate<23>,
+        launch_rate<24>, launch_rate<25>, launch_rate<26>, launch_rate<27>, launch_rate<28>, launch_rate<29>,
+        launch_rate<30>, launch_rate<31>, launch_rate<32>, launch_rate<33>, launch_rate<34>};
     const int iters = 10000;
     printf("\n| instruction form | cycles per wave-instruction per SIMD |\n|---|---:|\n");
     for (int k = 0; k < NFORM; k++) {
