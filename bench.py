@@ -45,16 +45,17 @@ C14 = {"cyc": 2872129173, "maxfin": 465, "hist": {96: 43669233, 112: 2828459940}
 # counted in the gfx950 assembly of k_flow (k_stage's age-synchronous loop also has 17); the
 # algorithmic unit is one turn of the detector run.
 VALU_OPS_PER_TURN = 17
-# VALU issue on gfx950, measured with an issue-rate probe at 7-8 waves per SIMD: VOP2 v_add_u32,
-# v_and_b32 and v_or_b32 issue two wave64 instructions per SIMD per 4 clocks (dual issue); 64-bit
-# shifts, adds and compares, VOP3 forms, selects and compares issue one per 4 clocks. The peak is
+# VALU issue on gfx950, measured with an issue-rate probe at 7 waves per SIMD: the VOP2
+# v_add/v_sub/v_and/v_or/v_xor/v_mov_b32 and the 32- and 16-bit shifts right issue two wave64
+# instructions per SIMD per 4 clocks (dual issue); 64-bit shifts, adds and compares, VOP3 forms,
+# selects, compares and VOP2 forms with a scalar operand issue one per 4 clocks. The peak is
 # therefore 2 x 64 lanes per SIMD per 4 clocks = CUs x 128 lane-ops per clock (78.6 T/s at 256 CUs
 # and 2.4 GHz, the rate the published 157.3 TFLOP/s FP32 vector figure implies). Of the turn's 17
-# instructions 5 can dual-issue (2 ANDs, 2 ORs, the length add) and 12 cannot, so the turn itself
-# needs at least 12 x 4 + 5 x 2 = 58 clocks per wave: its instruction mix caps VALU issue at
-# 17 x 4 / 58 = 1.17 instructions per SIMD per 4 clocks, 0.586 of the dual-issue peak.
+# instructions 7 can dual-issue (2 ANDs, 2 shifts right, 2 ORs, the length add) and 10 cannot, so
+# the turn itself needs at least 10 x 4 + 7 x 2 = 54 clocks per wave: its instruction mix caps VALU
+# issue at 17 x 4 / 54 = 1.26 instructions per SIMD per 4 clocks, 0.630 of the dual-issue peak.
 VALU_LANE_OPS_PER_CU_CLOCK = 128
-TURN_MIX_CEILING = (VALU_OPS_PER_TURN * 4.0 / (12 * 4 + 5 * 2)) / 2.0
+TURN_MIX_CEILING = (VALU_OPS_PER_TURN * 4.0 / (10 * 4 + 7 * 2)) / 2.0
 # HBM bytes per k_flow launch on the n = 13 workload (2 launches; no records are written: 0.20 MB
 # read, 66.5 MB of tile-grab atomics and counters), from the rocprofv3 byte counters calibrated on a
 # copy of known size.
