@@ -14,6 +14,8 @@ every step. One step = one whole enumeration. Beside it, on the same GPU in the 
                  (4-bit piles, card 16 apart) and 5-bit piles, Brent and the alternation detector;
   serving        a continuous-batching run of mixed queries with p50/p99 latency, beside the same
                  queries one by one through the plain and the persistent kernels;
+  mixed_batch    240 requests of n = 5..15 (full, canonical, distinct) in shared persistent launches
+                 (war_run_batch), beside the same requests one by one;
   roofline       integer VALU: useful lane-ops/s (turns x VALU instructions per turn of the flow
                  kernel's loop x 64 lanes) against the VALU's dual-issue peak, CUs x 128 lane-ops
                  per clock, with the fraction of the single-issue rate and of the turn's own
@@ -291,6 +293,12 @@ def main():
         # ---- serving: continuous batching of mixed queries
         out["serving"] = serving(eng)
 
+        # ---- mixed-n batch: many requests of n = 5..15 in shared persistent launches
+        out["mixed_batch"] = mixed_batch(eng)
+        log("mixed batch: %.3e positions/s in %d launches; one by one %.3e"
+            % (out["mixed_batch"]["value"], out["mixed_batch"]["launches"],
+               out["mixed_batch"]["one_by_one_persistent_positions_per_s"]))
+
         # ---- CPU baseline: the oracle on sampled n = 13 chunks, compared chunk by chunk
         threads = min(16, os.cpu_count() or 1)
         ch = chunks_spread(n13, a.cpu_chunks, 1 << 22, 13)
@@ -360,6 +368,46 @@ def serving(eng, nreq=64, seed=20261203, rate=100.0):
             "queries": "n uniform 7..17, full or canon (distinct for 30%% of canon n<=13), length log-uniform "
                        "[1e3, 10^8.5] (%d positions in all); counts identical alone, in the mix and in the "
                        "burst" % npos}
+
+
+def mixed_batch(eng, nreq=240, seed=20261203):
+    """The mixed-n single-launch workload: seeded numpy PCG64, `nreq` requests (the engine has 248
+    request slots), n uniform over 5..15, full or canonical, distinct + dhist for a third of the
+    canonical requests of n <= 13, length floor(10^U(3, 9)) clipped to the total, lo uniform. Run
+    as one war_run_batch call (requests of one layout, rule and detector share each launch; with
+    cycle walks a launch holds 2^28 positions), then one by one through war_run_range: every count
+    must be identical."""
+    import numpy as np
+    rng = np.random.Generator(np.random.PCG64(seed))
+    qs = []
+    for _ in range(nreq):
+        n = int(rng.integers(5, 16))
+        mode = "canon" if rng.random() < 0.5 else "full"
+        total = math.factorial(n - 1) * (n - 1) if mode == "canon" else math.factorial(n)
+        ln = min(total, int(10 ** rng.uniform(3, 9)))
+        lo = int(rng.integers(0, total - ln + 1))
+        kw = {"hbins": 128, "lbins": 128}
+        if mode == "canon" and n <= 13 and rng.random() < 1 / 3:
+            kw.update(distinct=True, dbins=256)
+        qs.append((n, mode, lo, lo + ln, kw))
+    npos = sum(hi - lo for _, _, lo, hi, _ in qs)
+    eng.run_batch(qs[:8])                                   # graphs of the batch shapes captured
+    t0 = time.perf_counter()
+    got, launches = eng.run_batch(qs)
+    wall = time.perf_counter() - t0
+    t = 0.0
+    for q, g in zip(qs, got):
+        n, mode, lo, hi, kw = q
+        t1 = time.perf_counter()
+        p = eng.run_range(n, mode, lo, hi, **kw)
+        t += time.perf_counter() - t1
+        for k in ("cyc", "done", "maxfin", "turns", "hist", "lenhist", "oncycle", "distinct_minkey", "dhist"):
+            assert p[k] == g[k], (q, k)
+    return {"workload": "%d requests, n uniform 5..15, full or canon (distinct for 1/3 of canon n<=13), length "
+                        "log-uniform [1e3, 1e9] clipped, numpy PCG64 seed %d: %d positions" % (nreq, seed, npos),
+            "value": npos / wall, "unit": "positions/s", "launches": launches, "wall_s": wall,
+            "one_by_one_persistent_positions_per_s": npos / t,
+            "parity": "every count of every request identical in the batch and alone"}
 
 
 if __name__ == "__main__":

@@ -746,6 +746,88 @@ int run_request_locked(Engine &E, ReqHost &h) {
     return rc;
 }
 
+// Several requests at once, synchronously (engine mutex held by the caller): the serving loop's
+// multi-request launches without its thread. Requests are grouped by pile layout, rule, detector
+// and whether they need cycle walks (so requests without walks keep the larger batches of the
+// kernel without them); each group's ranges are packed, in request order, into as few persistent
+// launches as the batch capacity allows (positions per launch: FLOW_BATCH on the flow kernel, the
+// queue capacity with cycle walks or the staged pipeline; at most MAXJOBS ranges), so a group
+// whose positions fit is one launch. *launches counts them.
+int run_batch_locked(Engine &E, std::vector<ReqHost> &hs, int *launches) {
+    int rc = 0;
+    for (auto &h : hs) {
+        war_tuning tt;
+        resolve_tuning(E, h.rq, tt);
+        rc = check_tuning(tt);
+        if (!rc) rc = layout_of(h.rq, tt, &h.layout);
+        if (!rc) {
+            h.slot = alloc_slot(E);
+            if (h.slot < 0) rc = fail(40, "more requests than free request slots");
+        }
+        if (!rc) rc = req_alloc(E, h);
+        if (rc) break;
+    }
+    for (int W : {4, 5, 6}) {
+        for (int rule : {0, 1}) {
+            for (int alt : {0, 1}) {
+              for (int wk : {0, 1}) {
+                if (rc) break;
+                std::vector<ReqHost *> grp;
+                for (auto &h : hs)
+                    if (h.layout == W && h.rq.rule == rule && (h.rq.detector == WAR_DET_ALT) == (alt != 0) &&
+                        (h.dev.walk != 0) == (wk != 0) && h.rq.lo < h.rq.hi)
+                        grp.push_back(&h);
+                if (grp.empty()) continue;
+                war_tuning gt;
+                resolve_tuning(E, grp[0]->rq, gt);
+                const u64 tp = (u64)gt.ranks_per_refill;
+                int hb = 0, lb = 0;
+                bool walk = false;
+                for (ReqHost *h : grp) {
+                    hb = std::max(hb, h->rq.hbins); lb = std::max(lb, h->rq.lbins);
+                    walk = walk || h->dev.walk;
+                }
+                const u64 cap = (walk || gt.stage_kind != 3) ? E.batch : FLOW_BATCH;
+                const int det = alt ? WAR_DET_ALT : WAR_DET_BRENT;
+                std::vector<Job> jobs;
+                u64 used = 0;      // tile-rounded positions of the launch being packed
+                double dev = 0;
+                auto flush = [&]() -> int {
+                    if (jobs.empty()) return 0;
+                    float ms = 0;
+                    const int r2 = (W == 4) ? run_batch_dispatch<4>(E, rule, jobs, gt, hb, lb, walk, true, &ms, det)
+                                 : (W == 6) ? run_batch_dispatch<6>(E, rule, jobs, gt, hb, lb, false, true, &ms, det)
+                                            : run_batch_dispatch<5>(E, rule, jobs, gt, hb, lb, walk, true, &ms, det);
+                    dev += ms;
+                    ++*launches;
+                    jobs.clear();
+                    used = 0;
+                    return r2;
+                };
+                for (ReqHost *h : grp) {
+                    for (u64 lo = h->rq.lo; lo < h->rq.hi && !rc;) {
+                        if ((int)jobs.size() == MAXJOBS || cap - used < tp) { rc = flush(); continue; }
+                        const u64 len = std::min<u64>(h->rq.hi - lo, (cap - used) / tp * tp);
+                        Job j{};
+                        j.lo = lo; j.hi = lo + len; j.req = (u32)h->slot;
+                        jobs.push_back(j);
+                        used += (len + tp - 1) / tp * tp;
+                        lo += len;
+                    }
+                }
+                if (!rc) rc = flush();
+                for (ReqHost *h : grp) h->dev_ms = dev;     // device time of the group's launches
+              }
+            }
+        }
+    }
+    for (auto &h : hs) {
+        if (!rc && h.slot >= 0) rc = req_collect(E, h);
+        req_free(E, h);
+    }
+    return rc;
+}
+
 // ---------------------------------------------------------------------------------------------
 // Serving loop: continuous batching. Each step takes, from every admitted request, its next
 // share of positions (step_positions split evenly, whole tiles), runs the groups of one layout and
@@ -1012,6 +1094,35 @@ int war_run_range(const war_request *req, war_result *res) {
         rc = run_request_locked(*g_eng, h);
     }
     res->latency_ms = now_ms() - t0;
+    return rc;
+}
+
+int war_run_batch(const war_request *reqs, int32_t count, war_result *res, int32_t *launches) {
+    t_err.clear();
+    if (!g_eng) return fail(60, "war_init has not been called");
+    if (count < 0 || (count > 0 && (!reqs || !res))) return fail(1, "null requests or results");
+    int rc = 0;
+    for (int i = 0; i < count && !rc; i++) {
+        rc = check_request(&reqs[i]);
+        if (!rc && reqs[i].kernel != WAR_KERNEL_PERSISTENT)
+            rc = fail(82, "request %d: a batch runs the persistent kernel only", i);
+        if (!rc && reqs[i].detector != WAR_DET_BRENT && reqs[i].detector != WAR_DET_ALT)
+            rc = fail(82, "request %d: a batch runs Brent or the alternation detector", i);
+    }
+    if (rc) return rc;
+    std::vector<ReqHost> hs((size_t)count);
+    for (int i = 0; i < count; i++) { hs[(size_t)i].rq = reqs[i]; hs[(size_t)i].res = &res[i]; }
+    int nl = 0;
+    const double t0 = now_ms();
+    {
+        std::lock_guard<std::mutex> g(g_eng->mu);
+        if ((size_t)count > g_eng->free_slots.size())
+            return fail(40, "%d requests, %zu free request slots", count, g_eng->free_slots.size());
+        rc = run_batch_locked(*g_eng, hs, &nl);
+    }
+    const double wall = now_ms() - t0;
+    for (int i = 0; i < count; i++) res[i].latency_ms = wall;
+    if (launches) *launches = nl;
     return rc;
 }
 

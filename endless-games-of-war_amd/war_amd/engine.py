@@ -131,6 +131,21 @@ class Engine:
         check(self.lib.war_run_range(ctypes.byref(req), ctypes.byref(buf.res)))
         return buf.as_dict()
 
+    def run_batch(self, queries):
+        """Several requests in shared persistent launches (war_run_batch). queries: [(n, mode, lo,
+        hi, kw), ...] with run_range's keywords. Returns (results, launches): one result dict per
+        query, each equal to run_range of that query, and the number of launches made."""
+        reqs = [make_request(n, mode, lo, hi, **kw) for n, mode, lo, hi, kw in queries]
+        count = len(reqs)
+        arr = (WarRequest * max(1, count))(*reqs)
+        bufs = [_ResultBuf(r) for r in reqs]
+        res = (WarResult * max(1, count))(*[b.res for b in bufs])
+        launches = ctypes.c_int32(0)
+        check(self.lib.war_run_batch(arr, count, res, ctypes.byref(launches)))
+        for i, b in enumerate(bufs):
+            b.res = res[i]
+        return [b.as_dict() for b in bufs], launches.value
+
     # ---- tools around the enumeration
     def reach(self, n, lo=0, hi=None, cap=200_000_000, nbins=4096):
         """Distinct cycles of canonical positions [lo, hi) and whether ordinary deals enter them."""

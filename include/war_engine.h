@@ -161,6 +161,15 @@ int war_device_info(war_devinfo *out);
 /* Synchronous range query: plays every position of [lo, hi) on the GPU and fills *res. */
 int war_run_range(const war_request *req, war_result *res);
 
+/* Several requests of mixed n, mode and rule in shared persistent launches, synchronously: the
+ * reference launches war_count / war_canon once per range (gpu/war_gpu.py:309-328); here requests
+ * of one pile layout, rule and detector share each launch (one, when their positions fit a batch:
+ * 2^32 positions on the flow kernel, 2^28 with cycle walks) and every result equals the request
+ * run alone. Persistent kernel, Brent or the alternation detector; count <= the free request slots
+ * (248 less the serving loop's). res[i] receives request i's result, device_ms the device time of
+ * its group's launches and latency_ms the whole call; *launches (may be null) the launch count. */
+int war_run_batch(const war_request *reqs, int32_t count, war_result *res, int32_t *launches);
+
 /* Reachability of the distinct cycles of canonical positions [lo, hi) of n cards (2 <= n <= 15,
  * rule 0) by ordinary deals: replaces cpu/reach_cycles.c's main loop (:73-107) and search
  * (:48-69). cap: positions one cycle's backward search may visit (the reference's default is
