@@ -883,15 +883,17 @@ void serve_loop(Engine *Ep) {
             active.push_back(h);
         }
         if (active.empty()) continue;
-        // one step per (layout, rule) group
+        // one launch per (layout, rule, detector, cycle walks) group: requests without walks keep
+        // the kernel without them
         int rc = 0;
         for (int W : {4, 5, 6}) {
             for (int rule : {0, 1})
-            for (int alt : {0, 1}) {
+            for (int alt : {0, 1})
+            for (int wk : {0, 1}) {
                 std::vector<std::shared_ptr<ReqHost>> grp;
                 for (auto &h : active)
                     if (h->layout == W && h->rq.rule == rule && (h->rq.detector == WAR_DET_ALT) == (alt != 0) &&
-                        h->next < h->rq.hi)
+                        (h->dev.walk != 0) == (wk != 0) && h->next < h->rq.hi)
                         grp.push_back(h);
                 if (grp.empty()) continue;
                 war_tuning gt;
