@@ -66,7 +66,7 @@ using namespace war;
 #define I_ALIGN(i) "v_alignbit_b32 %" #i ", %" #i ", %8, 4\n"
 #define I_CMP64VCC(i) "v_cmp_eq_u64 vcc, %" #i ", %8\n"
 
-constexpr int NFORM = 37;
+constexpr int NFORM = 40;
 const char *form_name[NFORM] = {"v_or_b32", "v_and_b32 (inline constant)", "v_add_u32", "v_lshl_or_b32",
                                 "v_cndmask_b32_e64 (scalar mask)", "v_lshlrev_b32", "v_lshrrev_b64",
                                 "v_lshl_add_u64", "v_cmp_gt_u32 (to a scalar pair)",
@@ -80,7 +80,10 @@ const char *form_name[NFORM] = {"v_or_b32", "v_and_b32 (inline constant)", "v_ad
                                 "v_add3_u32", "v_or3_b32", "v_mul_u32_u24", "v_lshlrev_b64", "v_alignbit_b32",
                                 "v_cmp_eq_u64 (to vcc)",
                                 "v_cndmask_b32_e32 after a VALU compare to vcc (1 compare + 8 selects, per select)",
-                                "v_cndmask_b32_e64 after a VALU compare to a scalar pair (1 + 8, per select)"};
+                                "v_cndmask_b32_e64 after a VALU compare to a scalar pair (1 + 8, per select)",
+                                "v_cndmask_b32_e64 reading vcc after a VALU compare to vcc (1 + 8, per select)",
+                                "v_cndmask_b32_e32 after a VALU compare to vcc, 8 ORs between (1 + 8 + 8, per select)",
+                                "v_or_b32 x 8 after a VALU compare to vcc (1 + 8, per OR; reference for the row above)"};
 
 template <int K>
 __global__ __launch_bounds__(256) void k_rate(u64 *out, int iters, u32 seed) {
@@ -151,6 +154,17 @@ __global__ __launch_bounds__(256) void k_rate(u64 *out, int iters, u32 seed) {
                              : "v"(x), "v"(c[0]));
                 m ^= s0;
             }
+            if constexpr (K == 37)
+                asm volatile("v_cmp_gt_u32 vcc, %8, %9\n s_nop 1\n"
+                             "v_cndmask_b32_e64 %0, %0, %8, vcc\n v_cndmask_b32_e64 %1, %1, %8, vcc\n"
+                             "v_cndmask_b32_e64 %2, %2, %8, vcc\n v_cndmask_b32_e64 %3, %3, %8, vcc\n"
+                             "v_cndmask_b32_e64 %4, %4, %8, vcc\n v_cndmask_b32_e64 %5, %5, %8, vcc\n"
+                             "v_cndmask_b32_e64 %6, %6, %8, vcc\n v_cndmask_b32_e64 %7, %7, %8, vcc"
+                             : C32 : "v"(x), "v"(c[0]) : "vcc");
+            if constexpr (K == 38)
+                asm volatile("v_cmp_gt_u32 vcc, %8, %9\n" X8(I_OR) X8(I_CNDVCC) : C32 : "v"(x), "v"(c[0]) : "vcc");
+            if constexpr (K == 39)
+                asm volatile("v_cmp_gt_u32 vcc, %8, %9\n s_nop 1\n" X8(I_OR) : C32 : "v"(x), "v"(c[0]) : "vcc");
             if constexpr (K == 9) {
                 u64 s0;
                 asm volatile("v_sub_co_u32 %0, %8, %9, %0\n v_sub_co_u32 %1, %8, %9, %1\n v_sub_co_u32 %2, %8, %9, %2\n"
@@ -251,7 +265,7 @@ int main(int argc, char **argv) {
         launch_rate<18>, launch_rate<19>, launch_rate<20>, launch_rate<21>, launch_rate<22>, launch_rate<23>,
         launch_rate<24>, launch_rate<25>, launch_rate<26>, launch_rate<27>, launch_rate<28>, launch_rate<29>,
         launch_rate<30>, launch_rate<31>, launch_rate<32>, launch_rate<33>, launch_rate<34>, launch_rate<35>,
-        launch_rate<36>};
+        launch_rate<36>, launch_rate<37>, launch_rate<38>, launch_rate<39>};
     const int iters = 10000;
     printf("\n| instruction form | cycles per wave-instruction per SIMD |\n|---|---:|\n");
     for (int k = 0; k < NFORM; k++) {

@@ -219,19 +219,39 @@ __device__ __forceinline__ Zt make_zt(u32 zero) {
     asm volatile("v_mov_b32 %0, %1" : "=v"(z2) : "s"(zero));
     Zt z; z.h1 = z1; z.h2 = z2; return z;
 }
-template <int RULE, bool UC = true>
-__device__ __forceinline__ bool turn_top(St<4> &s, u32 &sb, u32 C, Zt &z, u64 &over) {
+template <int RULE, bool UC = true, bool SELASM = false>
+__device__ __forceinline__ u64 turn_top(St<4> &s, u32 &sb, u32 C, Zt &z, u64 &over) {
     const u32 ah = (u32)(s.a >> 32) & 0xF0000000u, bh = (u32)(s.b >> 32) & 0xF0000000u;   // top cards << 28
-    const bool p1 = ah > bh;
-    const u32 wa = ah | (bh >> 4), wb = bh | (ah >> 4);         // player 1's card above / player 2's above
-    const u32 x1 = p1 ? (RULE ? wb : wa) : 0u, x2 = p1 ? 0u : (RULE ? wa : wb);
+    u64 pm;                    // the wave's mask of lanes whose player 1 won the turn
+    u32 x1, x2, d;
+    if constexpr (SELASM) {
+        // The compare and the three selects as written: compiled from C, the first select can come
+        // out as the VOP2 form reading vcc right after the compare wrote it, which gfx950 issues
+        // several times slower than the VOP3 form (measured). Two wait states between the compare
+        // and the first read of its mask.
+        const u32 wa = ah | (bh >> 4), wb = bh | (ah >> 4);
+        asm("v_cmp_gt_u32_e64 %0, %4, %5\n\t"
+            "s_nop 1\n\t"
+            "v_cndmask_b32_e64 %1, 0, %6, %0\n\t"
+            "v_cndmask_b32_e64 %2, %7, 0, %0\n\t"
+            "v_cndmask_b32_e64 %3, -4, 4, %0"
+            : "=&s"(pm), "=&v"(x1), "=&v"(x2), "=&v"(d)
+            : "v"(ah), "v"(bh), "v"(RULE ? wb : wa), "v"(RULE ? wa : wb));
+    } else {
+        const bool p1 = ah > bh;
+        const u32 wa = ah | (bh >> 4), wb = bh | (ah >> 4);     // player 1's card above / player 2's above
+        x1 = p1 ? (RULE ? wb : wa) : 0u;
+        x2 = p1 ? 0u : (RULE ? wa : wb);
+        d = p1 ? 4u : (u32)-4;
+        pm = __builtin_amdgcn_ballot_w64(p1);
+    }
     z.h1 = (z.h1 & 0xFFFFFFFFull) | ((u64)x1 << 32);
     z.h2 = (z.h2 & 0xFFFFFFFFull) | ((u64)x2 << 32);
     s.a = (s.a << 4) + (z.h1 >> (s.sa & 63u));
     s.b = (s.b << 4) + (z.h2 >> (sb & 63u));
-    s.sa += p1 ? 4u : (u32)-4;
+    s.sa += d;
     sub_borrow<UC>(C, s.sa, sb, over);
-    return p1;
+    return pm;
 }
 
 // The repeat test compares (w1, w2, l1): card 0 and an empty slot look alike (gpu/war_gpu.py:91).

@@ -1133,6 +1133,10 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
     // most of which end within 30 turns, lose more to lanes idle until the span end, and the
     // alternation detector measured slower with 32)
     constexpr u32 SPAN = (BLK && DET == FLOW_BRENT) ? 2u * FLOW_SPAN : FLOW_SPAN;
+    // Brent's turn takes turn_top's written compare and selects (the compiled ones put a VOP2 select
+    // right after the compare): 2 % faster on canonical n = 14, 3 % on n = 13 deals; the
+    // alternation detector's compiled turn already has the VOP3 selects and runs 4 % slower with it
+    constexpr bool SELASM = DET == FLOW_BRENT;
     constexpr int FB = __builtin_ctz(SPAN);
     static_assert(FLOW_SPAN == 2u * ALT_K && PAIR_P % FLOW_SPAN == 0u, "checkpoints at turns 7 and 15 of a span");
     static_assert((SPAN & (SPAN - 1u)) == 0u && (AGE_CAP % SPAN) == 0u && SPAN <= 1024u,
@@ -1242,10 +1246,9 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
         }
 #pragma unroll
         for (u32 tix = 0; tix < SPAN; tix++) {       // turn tix of the span
-            u64 over;
-            bool p1;
-            if constexpr (W == 4) p1 = turn_top<RULE, !MULTI>(s, sb, MULTI ? gC : C, zt, over);
-            else p1 = turn_sb<W, RULE, !MULTI>(s, sb, MULTI ? gC : C, z, over);
+            u64 over, pm;          // pm: the lanes whose player 1 won the turn
+            if constexpr (W == 4) pm = turn_top<RULE, !MULTI, SELASM>(s, sb, MULTI ? gC : C, zt, over);
+            else pm = ballot(turn_sb<W, RULE, !MULTI>(s, sb, MULTI ? gC : C, z, over));
             // the common case (no lane ended or matched the tortoise's player-1 pile) costs two
             // scalar operations and the branch; the scalar unit is busy about 70 % of the time here
             u64 cand = ballot(s.a == T_.t.a);
@@ -1254,19 +1257,17 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
             }
             if constexpr (DET == FLOW_ALT) {
                 // a repeated winner on a turn of odd age (tix even: births fall on span starts)
-                const u64 pm = ballot(p1);
                 if ((tix & 1u) == 0u) streakm |= ~(pm ^ prevm);
                 prevm = pm;
             } else if constexpr (DET == FLOW_PAIR) {
                 // a run violation: the winner repeated twice in a row (a run of three or more) or
                 // changed twice in a row (a run of one)
-                const u64 pm = ballot(p1);
                 const u64 eqm = ~(pm ^ prevm);
                 streakm |= ~(eqm ^ preveqm);
                 prevm = pm;
                 preveqm = eqm;
             } else {
-                (void)p1;
+                (void)pm;
             }
             if (const u64 x = (over | cand) & alive) {
                 // ended games: x & over; the ended set of the span is rebuilt at its end (live at
