@@ -68,9 +68,9 @@ typedef struct war_tuning {
                                   finishes takes the next record at once), 2 = lockstep;
                                   3 = flow: no stages, one kernel per batch in which a lane whose
                                   game is over takes the next position at the start of the next
-                                  16-turn span (single requests and the serving loop's mixed
-                                  batches, with or without cycle walks; rule 1 with a detector
-                                  other than Brent runs as 2); 0 = default (3) */
+                                  span of 16 turns (32 for Brent over blocks of 24 deals; single
+                                  requests and mixed batches, with or without cycle walks; rule 1
+                                  with a detector other than Brent runs as 2); 0 = default (3) */
     int32_t pile_layout;       /* 0 = default; 4 = 4-bit piles (n <= 16); 5 = 5-bit piles (any n);
                                   6 = n = 17 with card 16 held apart and 4-bit piles (the layout
                                   of gpu/war17.py; persistent kernel, Brent, no cycle walks).
