@@ -118,7 +118,7 @@ def test_flow_brent_sampled(eng, orc):
 def test_flow_deal_blocks(eng, orc, n):
     """The flow kernel steps 4-bit deals through blocks of 24 orders that differ in their last four
     cards, one swap per deal: ranges on and off a block boundary, ending mid-block, of every lane-run
-    length the tile sizes give, with both detectors."""
+    length the tile sizes give, with both detectors and both rules."""
     rng = random.Random(24 * n)
     total = orc.total(n, "full")
     for ln in (1, 23, 24, 25, 47, 1535, 1537, 100_003):
@@ -129,6 +129,8 @@ def test_flow_deal_blocks(eng, orc, n):
             for det in (["brent", "alt"] if n % 2 else ["brent"]):
                 print("case deal blocks", n, lo, ln, tuning, det, flush=True)
                 check(eng, orc, n, "full", lo, lo + ln, tuning=tuning, detector=det)
+        print("case deal blocks rule 1", n, lo, ln, flush=True)
+        check(eng, orc, n, "full", lo, lo + ln, rule=1)
 
 
 def test_5bit_ranges(eng, orc):
