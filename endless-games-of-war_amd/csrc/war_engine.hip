@@ -458,7 +458,7 @@ void launch_refill(int flags, int blocks, size_t lds, hipStream_t st, const Stag
 
 // The flow kernel for a batch's detector (WAR_DET_*): Brent for either rule, the alternation and
 // pair-run detectors for rule 0 (pair-run: single requests, layouts 4 and 5). BLK: the variant for
-// single 4-bit deal requests whose batch and lane runs are whole blocks of 24 deals.
+// 4-bit deals whose jobs and lane runs are whole blocks of 24 deals.
 template <int W, int RULE, bool WALK, bool MULTI, bool BLK = false>
 void launch_flow(int det, int blocks, size_t lds, hipStream_t st, const StageArgs<W> &A) {
     if (det == WAR_DET_BRENT) {
@@ -496,7 +496,10 @@ int build_graph(Engine &E, const GraphKey &key, hipGraphExec_t *out) {
                 else launch_flow<W, RULE, true, false>(key.det, blocks, lds, E.st, A);
             }
         } else if (key.blk) {
-            if constexpr (W == 4) launch_flow<4, RULE, false, false, true>(key.det, blocks, lds, E.st, A);
+            if constexpr (W == 4) {
+                if (key.multi) launch_flow<4, RULE, false, true, true>(key.det, blocks, lds, E.st, A);
+                else launch_flow<4, RULE, false, false, true>(key.det, blocks, lds, E.st, A);
+            }
         } else {
             if (key.multi) launch_flow<W, RULE, false, true>(key.det, blocks, lds, E.st, A);
             else launch_flow<W, RULE, false, false>(key.det, blocks, lds, E.st, A);
@@ -564,7 +567,7 @@ int run_persistent_batch(Engine &E, std::vector<Job> &jobs, const war_tuning &t,
     // flow: cap0 holds the refill threshold (free lanes of 64), brent_cap's stage windows unused
     const int cap0 = flow ? FLOW_MIN_FREE : t.brent_cap;
     GraphKey key{W, RULE, 1, t.waves_per_cu, t.ranks_per_refill / 64, cap0, walk ? 1 : 0, multi ? 1 : 0,
-                 round_pow2(hb), round_pow2(lb), kind, det, (blk && flow && W == 4 && !walk && !multi) ? 1 : 0};
+                 round_pow2(hb), round_pow2(lb), kind, det, (blk && flow && W == 4 && !walk) ? 1 : 0};
     if ((!flow || walk) && (tiles > qbufs<W>(E).tmax || tiles * tile_pos > qbufs<W>(E).cap))
         return fail(32, "batch exceeds the queue capacity");
     JobHdr h{};
@@ -749,10 +752,12 @@ int run_request_locked(Engine &E, ReqHost &h) {
 // Several requests at once, synchronously (engine mutex held by the caller): the serving loop's
 // multi-request launches without its thread. Requests are grouped by pile layout, rule, detector
 // and whether they need cycle walks (so requests without walks keep the larger batches of the
-// kernel without them); each group's ranges are packed, in request order, into as few persistent
-// launches as the batch capacity allows (positions per launch: FLOW_BATCH on the flow kernel, the
-// queue capacity with cycle walks or the staged pipeline; at most MAXJOBS ranges), so a group
-// whose positions fit is one launch. *launches counts them.
+// kernel without them); in a group of 4-bit piles without walks the deal ranges' whole blocks of
+// 24 orders run apart, on the kernel's deal-block variant. Each set of ranges is packed, in
+// request order, into as few persistent launches as the batch capacity allows (positions per
+// launch: FLOW_BATCH on the flow kernel, the queue capacity with cycle walks or the staged
+// pipeline; at most MAXJOBS ranges), so a group whose positions fit is one launch, or two with
+// deal blocks. *launches counts them.
 int run_batch_locked(Engine &E, std::vector<ReqHost> &hs, int *launches) {
     int rc = 0;
     for (auto &h : hs) {
@@ -780,42 +785,73 @@ int run_batch_locked(Engine &E, std::vector<ReqHost> &hs, int *launches) {
                 if (grp.empty()) continue;
                 war_tuning gt;
                 resolve_tuning(E, grp[0]->rq, gt);
-                const u64 tp = (u64)gt.ranks_per_refill;
                 int hb = 0, lb = 0;
                 bool walk = false;
                 for (ReqHost *h : grp) {
                     hb = std::max(hb, h->rq.hbins); lb = std::max(lb, h->rq.lbins);
                     walk = walk || h->dev.walk;
                 }
-                const u64 cap = (walk || gt.stage_kind != 3) ? E.batch : FLOW_BATCH;
                 const int det = alt ? WAR_DET_ALT : WAR_DET_BRENT;
-                std::vector<Job> jobs;
-                u64 used = 0;      // tile-rounded positions of the launch being packed
                 double dev = 0;
-                auto flush = [&]() -> int {
-                    if (jobs.empty()) return 0;
-                    float ms = 0;
-                    const int r2 = (W == 4) ? run_batch_dispatch<4>(E, rule, jobs, gt, hb, lb, walk, true, &ms, det)
-                                 : (W == 6) ? run_batch_dispatch<6>(E, rule, jobs, gt, hb, lb, false, true, &ms, det)
-                                            : run_batch_dispatch<5>(E, rule, jobs, gt, hb, lb, walk, true, &ms, det);
-                    dev += ms;
-                    ++*launches;
-                    jobs.clear();
-                    used = 0;
+                // the group's ranges; 4-bit deals split at blocks of 24 orders, whose middles run
+                // the kernel's deal-block variant (lane runs of whole blocks: 24 per lane)
+                struct Rng { ReqHost *h; u64 lo, hi; };
+                std::vector<Rng> rest, blocks;
+                war_tuning bt = gt;
+                {
+                    const u64 per = std::max<u64>(1, ((u64)gt.ranks_per_refill / 64 + 12) / 24) * 24;
+                    if (64 * per <= MAX_TILE) bt.ranks_per_refill = (int)(64 * per);
+                }
+                const bool blk_ok = W == 4 && !walk && gt.stage_kind == 3 && (bt.ranks_per_refill / 64) % 24 == 0;
+                for (ReqHost *h : grp) {
+                    const war_request &q = h->rq;
+                    if (blk_ok && q.mode == WAR_MODE_FULL && q.n >= 5) {
+                        const u64 a = (q.lo + 23) / 24 * 24, b = q.hi / 24 * 24;
+                        if (a < b) {
+                            if (q.lo < a) rest.push_back({h, q.lo, a});
+                            blocks.push_back({h, a, b});
+                            if (b < q.hi) rest.push_back({h, b, q.hi});
+                            continue;
+                        }
+                    }
+                    rest.push_back({h, q.lo, q.hi});
+                }
+                // pack ranges, in order, into as few launches as the capacity allows
+                auto pack = [&](const std::vector<Rng> &rs, const war_tuning &tt, bool blk) -> int {
+                    const u64 tp = (u64)tt.ranks_per_refill;
+                    const u64 cap = (walk || tt.stage_kind != 3) ? E.batch : FLOW_BATCH;
+                    std::vector<Job> jobs;
+                    u64 used = 0;      // tile-rounded positions of the launch being packed
+                    int r2 = 0;
+                    auto flush = [&]() -> int {
+                        if (jobs.empty()) return 0;
+                        float ms = 0;
+                        const int r3 =
+                            (W == 4) ? run_batch_dispatch<4>(E, rule, jobs, tt, hb, lb, walk, true, &ms, det, blk)
+                          : (W == 6) ? run_batch_dispatch<6>(E, rule, jobs, tt, hb, lb, false, true, &ms, det)
+                                     : run_batch_dispatch<5>(E, rule, jobs, tt, hb, lb, walk, true, &ms, det);
+                        dev += ms;
+                        ++*launches;
+                        jobs.clear();
+                        used = 0;
+                        return r3;
+                    };
+                    for (const Rng &x : rs) {
+                        for (u64 lo = x.lo; lo < x.hi && !r2;) {
+                            if ((int)jobs.size() == MAXJOBS || cap - used < tp) { r2 = flush(); continue; }
+                            const u64 len = std::min<u64>(x.hi - lo, (cap - used) / tp * tp);
+                            Job j{};
+                            j.lo = lo; j.hi = lo + len; j.req = (u32)x.h->slot;
+                            jobs.push_back(j);
+                            used += (len + tp - 1) / tp * tp;
+                            lo += len;
+                        }
+                    }
+                    if (!r2) r2 = flush();
                     return r2;
                 };
-                for (ReqHost *h : grp) {
-                    for (u64 lo = h->rq.lo; lo < h->rq.hi && !rc;) {
-                        if ((int)jobs.size() == MAXJOBS || cap - used < tp) { rc = flush(); continue; }
-                        const u64 len = std::min<u64>(h->rq.hi - lo, (cap - used) / tp * tp);
-                        Job j{};
-                        j.lo = lo; j.hi = lo + len; j.req = (u32)h->slot;
-                        jobs.push_back(j);
-                        used += (len + tp - 1) / tp * tp;
-                        lo += len;
-                    }
-                }
-                if (!rc) rc = flush();
+                rc = pack(blocks, bt, true);
+                if (!rc) rc = pack(rest, gt, false);
                 for (ReqHost *h : grp) h->dev_ms = dev;     // device time of the group's launches
               }
             }

@@ -1009,7 +1009,7 @@ constexpr u32 FLOW_SPAN = ALT_K * FLOW_BLOCKS;
 // keeps its request and C per lane, and accounting groups the finished lanes by request.
 template <int W, int RULE, int DET, bool WALK, bool MULTI, bool BLK = false>
 __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
-    static_assert(!BLK || (W == 4 && !WALK && !MULTI), "deal blocks: 4-bit single requests without walks");
+    static_assert(!BLK || (W == 4 && !WALK), "deal blocks: 4-bit deals without walks");
     static_assert(DET == FLOW_BRENT || ((DET == FLOW_ALT || DET == FLOW_PAIR) && RULE == 0),
                   "the alternation and pair-run detectors are for rule 0");
     static_assert(!WALK || W != 6, "no cycle walks in layout 6");
@@ -1085,7 +1085,7 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
             while (hi - lo > 1) { u32 mid = (lo + hi) >> 1; if (A.jobs[mid].tile_base <= g0) lo = mid; else hi = mid; }
             const Job J = A.jobs[lo];
             if constexpr (MULTI) {
-                preq = J.req; n = reqs[preq].n; mode = reqs[preq].mode; C = c_of<W>(n);
+                preq = J.req; n = reqs[preq].n; mode = BLK ? 0u : reqs[preq].mode; C = c_of<W>(n);
             }
             const u64 first = J.lo + (g0 - J.tile_base) * (u64)TP + (u64)lane * per;
             left = first < J.hi ? (u32)((J.hi - first) < per ? (J.hi - first) : per) : 0u;

@@ -43,7 +43,7 @@ def test_mixed_n_one_launch_per_group_equals_oracle(eng, orc):
     qs = mixed_queries()
     assert sum(hi - lo for _, _, lo, hi, _ in qs) < 1 << 28       # fits one batch with cycle walks
     got, launches = eng.run_batch(qs)
-    assert launches == 2              # one launch without cycle walks, one with them
+    assert launches == 3              # without cycle walks: deal blocks, the rest; with walks
     for (n, mode, lo, hi, kw), g in zip(qs, got):
         want = orc.run(n, mode, lo, hi, **kw)
         for k in KEYS:
@@ -54,8 +54,9 @@ def test_mixed_n_one_launch_per_group_equals_oracle(eng, orc):
 
 def test_groups_and_packing_equal_single_requests(eng):
     """Rules, detectors, layouts and cycle walks form separate groups (one launch each when they
-    fit); a request larger than a batch is split over launches (here 5e9 deals of n = 14: two
-    launches of the flow kernel's 2^32), and cycle walks bound a launch at 2^28 positions."""
+    fit, deal blocks of 24 orders apart from the rest); a request larger than a batch is split
+    over launches (here 5e9 deals of n = 14: two launches of the flow kernel's 2^32), and cycle
+    walks bound a launch at 2^28 positions."""
     qs = [
         (14, "full", 10**9, 10**9 + 5 * 10**9, {"hbins": 256, "lbins": 256}),
         (13, "full", 7, 7 + 3_000_001, {"hbins": 64, "detector": "alt"}),
@@ -67,10 +68,12 @@ def test_groups_and_packing_equal_single_requests(eng):
     ]
     qs = [(n, m, lo, total(n, m) if hi is None else hi, kw) for n, m, lo, hi, kw in qs]
     got, launches = eng.run_batch(qs)
-    # layout 4, rule 0, Brent, no walks (n14 full, n9): 5e9 + 362880 positions, two launches of at
-    # most 2^32; the same with walks (the two distinct n13 ranges): 3e8 positions, two launches of
-    # at most 2^28; layout 4 rule 0 alternation: 1; layout 4 rule 1: 1; layout 6 (n = 17): 1
-    assert launches == 2 + 2 + 1 + 1 + 1
+    # layout 4, rule 0, Brent, no walks (n14 full, n9): the deal blocks, 5e9 - 8 + 362880
+    # positions, two launches of at most 2^32, and the 8 deals before n14's first block boundary,
+    # one; the same with walks (the two distinct n13 ranges): 3e8 positions, two launches of at
+    # most 2^28; layout 4 rule 0 alternation: blocks 1, the ends of the range 1; layout 4 rule 1
+    # (canonical): 1; layout 6 (n = 17): 1
+    assert launches == 3 + 2 + 2 + 1 + 1
     for (n, mode, lo, hi, kw), g in zip(qs, got):
         alone = eng.run_range(n, mode, lo, hi, **kw)
         for k in KEYS:
