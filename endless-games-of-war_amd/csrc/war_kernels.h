@@ -1602,7 +1602,6 @@ __device__ __forceinline__ u32 hash_insert(const ReqDev &R, u64 K) {
 
 template <int W, int RULE>
 __global__ __launch_bounds__(256) void k_walk(WalkArgs<W> A) {
-    typedef typename Lay<W>::T T;
     const u32 lane = lane_id();
     Zh<W> z = make_zh<W>(A.zero);
     const u64 T_ = A.hdr->ntiles;
