@@ -69,6 +69,12 @@ def test_errors_without_engine():
     if rc == 0:
         pytest.skip("an engine is live in this process")
     assert lib.war_last_error()
+    launches = ctypes.c_int32(-1)
+    reqs = (_lib.WarRequest * 1)(req)
+    out = (_lib.WarResult * 1)()
+    assert lib.war_run_batch(reqs, 1, out, ctypes.byref(launches)) != 0
+    assert b"war_init" in lib.war_last_error()
+    assert launches.value == -1
 
 
 def test_make_request_totals():
