@@ -749,15 +749,78 @@ int run_request_locked(Engine &E, ReqHost &h) {
     return rc;
 }
 
+// One group's ranges (one pile layout, rule, detector and walk flag) as multi-request persistent
+// launches. With `blocks`, 4-bit deal ranges without walks are split at blocks of 24 orders and
+// their whole blocks run on the kernel's deal-block variant (lane runs of 24 deals), a launch of
+// their own: worth it for whole requests (war_run_batch), not for the serving loop's small steps,
+// where the second launch per step cost more than it saved (measured). Each set of ranges is
+// packed, in order, into as few launches as the batch capacity allows (FLOW_BATCH positions on
+// the flow kernel, the queue capacity with cycle walks or the staged pipeline; MAXJOBS ranges).
+struct GroupRange { u32 slot, n, mode; u64 lo, hi; };
+int run_group(Engine &E, int W, int rule, int det, bool walk, const war_tuning &gt, int hb, int lb,
+              const std::vector<GroupRange> &rs, bool blocks_apart, double *dev, int *launches) {
+    std::vector<GroupRange> rest, blocks;
+    war_tuning bt = gt;
+    {
+        const u64 per = std::max<u64>(1, ((u64)gt.ranks_per_refill / 64 + 12) / 24) * 24;
+        if (64 * per <= MAX_TILE) bt.ranks_per_refill = (int)(64 * per);
+    }
+    const bool blk_ok =
+        blocks_apart && W == 4 && !walk && gt.stage_kind == 3 && (bt.ranks_per_refill / 64) % 24 == 0;
+    for (const GroupRange &x : rs) {
+        if (blk_ok && x.mode == WAR_MODE_FULL && x.n >= 5) {
+            const u64 a = (x.lo + 23) / 24 * 24, b = x.hi / 24 * 24;
+            if (a < b) {
+                if (x.lo < a) rest.push_back({x.slot, x.n, x.mode, x.lo, a});
+                blocks.push_back({x.slot, x.n, x.mode, a, b});
+                if (b < x.hi) rest.push_back({x.slot, x.n, x.mode, b, x.hi});
+                continue;
+            }
+        }
+        if (x.lo < x.hi) rest.push_back(x);
+    }
+    auto pack = [&](const std::vector<GroupRange> &v, const war_tuning &tt, bool blk) -> int {
+        const u64 tp = (u64)tt.ranks_per_refill;
+        const u64 cap = (walk || tt.stage_kind != 3) ? E.batch : FLOW_BATCH;
+        std::vector<Job> jobs;
+        u64 used = 0;          // tile-rounded positions of the launch being packed
+        int rc = 0;
+        auto flush = [&]() -> int {
+            if (jobs.empty()) return 0;
+            float ms = 0;
+            const int r2 = (W == 4) ? run_batch_dispatch<4>(E, rule, jobs, tt, hb, lb, walk, true, &ms, det, blk)
+                         : (W == 6) ? run_batch_dispatch<6>(E, rule, jobs, tt, hb, lb, false, true, &ms, det)
+                                    : run_batch_dispatch<5>(E, rule, jobs, tt, hb, lb, walk, true, &ms, det);
+            *dev += ms;
+            if (launches) ++*launches;
+            jobs.clear();
+            used = 0;
+            return r2;
+        };
+        for (const GroupRange &x : v) {
+            for (u64 lo = x.lo; lo < x.hi && !rc;) {
+                if ((int)jobs.size() == MAXJOBS || cap - used < tp) { rc = flush(); continue; }
+                const u64 len = std::min<u64>(x.hi - lo, (cap - used) / tp * tp);
+                Job j{};
+                j.lo = lo; j.hi = lo + len; j.req = x.slot;
+                jobs.push_back(j);
+                used += (len + tp - 1) / tp * tp;
+                lo += len;
+            }
+        }
+        if (!rc) rc = flush();
+        return rc;
+    };
+    int rc = pack(blocks, bt, true);
+    if (!rc) rc = pack(rest, gt, false);
+    return rc;
+}
+
 // Several requests at once, synchronously (engine mutex held by the caller): the serving loop's
 // multi-request launches without its thread. Requests are grouped by pile layout, rule, detector
 // and whether they need cycle walks (so requests without walks keep the larger batches of the
-// kernel without them); in a group of 4-bit piles without walks the deal ranges' whole blocks of
-// 24 orders run apart, on the kernel's deal-block variant. Each set of ranges is packed, in
-// request order, into as few persistent launches as the batch capacity allows (positions per
-// launch: FLOW_BATCH on the flow kernel, the queue capacity with cycle walks or the staged
-// pipeline; at most MAXJOBS ranges), so a group whose positions fit is one launch, or two with
-// deal blocks. *launches counts them.
+// kernel without them), and each group's whole ranges run as run_group launches: one while they
+// fit, or two with deal blocks. *launches counts them.
 int run_batch_locked(Engine &E, std::vector<ReqHost> &hs, int *launches) {
     int rc = 0;
     for (auto &h : hs) {
@@ -793,65 +856,9 @@ int run_batch_locked(Engine &E, std::vector<ReqHost> &hs, int *launches) {
                 }
                 const int det = alt ? WAR_DET_ALT : WAR_DET_BRENT;
                 double dev = 0;
-                // the group's ranges; 4-bit deals split at blocks of 24 orders, whose middles run
-                // the kernel's deal-block variant (lane runs of whole blocks: 24 per lane)
-                struct Rng { ReqHost *h; u64 lo, hi; };
-                std::vector<Rng> rest, blocks;
-                war_tuning bt = gt;
-                {
-                    const u64 per = std::max<u64>(1, ((u64)gt.ranks_per_refill / 64 + 12) / 24) * 24;
-                    if (64 * per <= MAX_TILE) bt.ranks_per_refill = (int)(64 * per);
-                }
-                const bool blk_ok = W == 4 && !walk && gt.stage_kind == 3 && (bt.ranks_per_refill / 64) % 24 == 0;
-                for (ReqHost *h : grp) {
-                    const war_request &q = h->rq;
-                    if (blk_ok && q.mode == WAR_MODE_FULL && q.n >= 5) {
-                        const u64 a = (q.lo + 23) / 24 * 24, b = q.hi / 24 * 24;
-                        if (a < b) {
-                            if (q.lo < a) rest.push_back({h, q.lo, a});
-                            blocks.push_back({h, a, b});
-                            if (b < q.hi) rest.push_back({h, b, q.hi});
-                            continue;
-                        }
-                    }
-                    rest.push_back({h, q.lo, q.hi});
-                }
-                // pack ranges, in order, into as few launches as the capacity allows
-                auto pack = [&](const std::vector<Rng> &rs, const war_tuning &tt, bool blk) -> int {
-                    const u64 tp = (u64)tt.ranks_per_refill;
-                    const u64 cap = (walk || tt.stage_kind != 3) ? E.batch : FLOW_BATCH;
-                    std::vector<Job> jobs;
-                    u64 used = 0;      // tile-rounded positions of the launch being packed
-                    int r2 = 0;
-                    auto flush = [&]() -> int {
-                        if (jobs.empty()) return 0;
-                        float ms = 0;
-                        const int r3 =
-                            (W == 4) ? run_batch_dispatch<4>(E, rule, jobs, tt, hb, lb, walk, true, &ms, det, blk)
-                          : (W == 6) ? run_batch_dispatch<6>(E, rule, jobs, tt, hb, lb, false, true, &ms, det)
-                                     : run_batch_dispatch<5>(E, rule, jobs, tt, hb, lb, walk, true, &ms, det);
-                        dev += ms;
-                        ++*launches;
-                        jobs.clear();
-                        used = 0;
-                        return r3;
-                    };
-                    for (const Rng &x : rs) {
-                        for (u64 lo = x.lo; lo < x.hi && !r2;) {
-                            if ((int)jobs.size() == MAXJOBS || cap - used < tp) { r2 = flush(); continue; }
-                            const u64 len = std::min<u64>(x.hi - lo, (cap - used) / tp * tp);
-                            Job j{};
-                            j.lo = lo; j.hi = lo + len; j.req = (u32)x.h->slot;
-                            jobs.push_back(j);
-                            used += (len + tp - 1) / tp * tp;
-                            lo += len;
-                        }
-                    }
-                    if (!r2) r2 = flush();
-                    return r2;
-                };
-                rc = pack(blocks, bt, true);
-                if (!rc) rc = pack(rest, gt, false);
+                std::vector<GroupRange> rs;
+                for (ReqHost *h : grp) rs.push_back({(u32)h->slot, (u32)h->rq.n, (u32)h->rq.mode, h->rq.lo, h->rq.hi});
+                rc = run_group(E, W, rule, det, walk, gt, hb, lb, rs, true, &dev, launches);
                 for (ReqHost *h : grp) h->dev_ms = dev;     // device time of the group's launches
               }
             }
@@ -866,8 +873,8 @@ int run_batch_locked(Engine &E, std::vector<ReqHost> &hs, int *launches) {
 
 // ---------------------------------------------------------------------------------------------
 // Serving loop: continuous batching. Each step takes, from every admitted request, its next
-// share of positions (step_positions split evenly, whole tiles), runs the groups of one layout and
-// rule as one persistent batch, and retires finished requests. Requests that arrive during a step
+// share of positions (step_positions split evenly, whole tiles), runs each group of one layout,
+// rule, detector and walk flag as one run_group launch, and retires finished requests. Requests that arrive during a step
 // are admitted at the next one, while slots and device memory allow.
 void serve_loop(Engine *Ep) {
     Engine &E = *Ep;
@@ -937,22 +944,20 @@ void serve_loop(Engine *Ep) {
                 const u64 tile = (u64)gt.ranks_per_refill;
                 u64 share = std::max<u64>(tile, E.step_positions / grp.size());
                 share = (share + tile - 1) / tile * tile;
-                std::vector<Job> jobs;
+                std::vector<GroupRange> rs;
                 int hb = 0, lb = 0;
                 bool walk = false;
                 for (auto &h : grp) {
-                    Job j{};
-                    j.lo = h->next; j.hi = std::min<u64>(h->rq.hi, h->next + share); j.req = (u32)h->slot;
-                    h->next = j.hi;
-                    jobs.push_back(j);
+                    const u64 hi = std::min<u64>(h->rq.hi, h->next + share);
+                    rs.push_back({(u32)h->slot, (u32)h->rq.n, (u32)h->rq.mode, h->next, hi});
+                    h->next = hi;
                     hb = std::max(hb, h->rq.hbins); lb = std::max(lb, h->rq.lbins);
                     walk = walk || h->dev.walk;
                 }
-                float ms = 0;
+                double dms = 0;
                 const int det = alt ? WAR_DET_ALT : WAR_DET_BRENT;
-                rc = (W == 4) ? run_batch_dispatch<4>(E, rule, jobs, gt, hb, lb, walk, true, &ms, det)
-                   : (W == 6) ? run_batch_dispatch<6>(E, rule, jobs, gt, hb, lb, false, true, &ms, det)
-                              : run_batch_dispatch<5>(E, rule, jobs, gt, hb, lb, walk, true, &ms, det);
+                rc = run_group(E, W, rule, det, walk, gt, hb, lb, rs, false, &dms, nullptr);
+                const float ms = (float)dms;
                 for (auto &h : grp) h->dev_ms += ms;   // device time of the steps the request took part in
                 if (rc) {
                     for (auto &h : grp) { h->status = rc; h->err = t_err; h->next = h->rq.hi; }
