@@ -1004,9 +1004,10 @@ constexpr int FLOW_BRENT = 0, FLOW_ALT = 2, FLOW_PAIR = 3;
 constexpr u32 PAIR_P = 128;
 constexpr u32 FLOW_SPAN = ALT_K * FLOW_BLOCKS;
 
-// MULTI: the batch holds several requests (the serving loop's steps). A tile belongs to one job,
-// so the pool's positions share one request (its n, mode and C change at a tile grab); a game
-// keeps its request and C per lane, and accounting groups the finished lanes by request.
+// MULTI: the batch holds several requests (the serving loop's steps, war_run_batch). A tile belongs
+// to one job, so the pool's positions share one request (its n, mode and C change at a tile grab);
+// a game keeps its request and C per lane, and accounting groups the finished lanes by request.
+// BLK: every job is 4-bit deals in whole blocks of 24 orders, and so is every lane run.
 template <int W, int RULE, int DET, bool WALK, bool MULTI, bool BLK = false>
 __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
     static_assert(!BLK || (W == 4 && !WALK), "deal blocks: 4-bit deals without walks");
