@@ -32,9 +32,9 @@ using namespace war;
 
 namespace {
 
+// The reason of the last failed call on this thread (empty after a call that succeeded). A serving
+// request's failure travels in its ReqHost and is raised again on the thread that waits for it.
 thread_local std::string t_err;
-std::string g_err_shared;
-std::mutex g_err_mu;
 
 int fail(int code, const char *fmt, ...) {
     char buf[512];
@@ -43,8 +43,6 @@ int fail(int code, const char *fmt, ...) {
     vsnprintf(buf, sizeof buf, fmt, ap);
     va_end(ap);
     t_err = buf;
-    std::lock_guard<std::mutex> g(g_err_mu);
-    g_err_shared = buf;
     return code;
 }
 
@@ -108,6 +106,8 @@ struct QueueBufs {
 };
 
 constexpr u64 MIN_TILE = 64, MAX_TILE = 8192;
+constexpr int MAXB = 16;                 // launches one batch graph holds (one branch each at most)
+constexpr size_t SLOT_BYTES = JOBS_OFF + MAXJOBS * 32;   // one launch's job header + jobs
 constexpr int FLOW_MIN_FREE = 1;        // flow kernel: refill at a checkpoint once this many lanes are free
 constexpr u64 FLOW_BATCH = 1ull << 32;  // flow kernel: positions per batch
 constexpr u32 PLAIN_CYC_TILE = 1024;    // home-tile size of the plain kernel's cycle records
@@ -129,7 +129,7 @@ struct Engine {
     int cus = 0, clock_mhz = 0;
     std::string arch;
     war_tuning tun{0, 0, 0, 0, 0};            // engine-wide override (war_set_tuning); zeros = per-workload default
-    std::map<std::tuple<int, int, int>, war_tuning> tuned;   // (n, mode, rule) -> war_autotune's choice
+    std::map<std::tuple<int, int, int, int>, war_tuning> tuned;   // (n, mode, rule, detector) -> war_autotune's choice
     u64 batch = 1ull << 28;          // positions per persistent batch (= queue capacity)
     Pipe *pipe = nullptr;
     JobHdr *hdr = nullptr;           // device: JobHdr followed by MAXJOBS Jobs
@@ -139,7 +139,15 @@ struct Engine {
     QueueBufs qb4, qb5;
     u64 *reach_stack = nullptr;     // reach searches: REACH_DEPTH entries per thread of a walk grid
     std::map<GraphKey, hipGraphExec_t> graphs;
+    // batch graphs: MAXB slots of pipe counters and job lists, a stream and a join event per branch
+    Pipe *bpipe = nullptr;
+    char *bhdr = nullptr;
+    void *bstage = nullptr;
+    hipStream_t bst[MAXB] = {};
+    hipEvent_t bfork = nullptr, bjoin[MAXB] = {};
+    std::map<std::vector<int>, hipGraphExec_t> bgraphs;
     std::vector<int> free_slots;
+    uint32_t slot_cap[MAXSLOTS] = {};  // each slot's explicit step cap (ReqDev::cap), for JobHdr::capck
     std::mutex mu;                   // serialises all GPU work of the engine
 
     // serving
@@ -168,8 +176,12 @@ int check_request(const war_request *r) {
     if (r->hbins < 0 || r->hbins > 4096 || r->lbins < 0 || r->lbins > 4096 || r->dbins < 0 || r->dbins > 4096)
         return fail(8, "bins must be in [0, 4096]");
     if (r->hbins + r->lbins > 8192) return fail(8, "hbins + lbins must be <= 8192");
-    if ((r->distinct || r->dbins) && (r->mode != WAR_MODE_CANON || r->n > 15))
-        return fail(9, "distinct / dbins need canon mode and n <= 15");
+    if (r->distinct && (r->mode != WAR_MODE_CANON || r->n > 15))
+        return fail(9, "distinct needs canon mode and n <= 15 (its keys are 4-bit cards under a 4-bit length)");
+    if (r->dbins && (r->mode != WAR_MODE_CANON || r->n > 16))
+        return fail(9, "dbins needs canon mode and n <= 16 (4-bit piles)");
+    if (r->step_cap > WAR_SAFETY_CAP) return fail(13, "step_cap must be <= 2^24 (got %u)", r->step_cap);
+    if (r->flags & ~WAR_FLAG_UNDET_ERROR) return fail(13, "unknown flags 0x%x", r->flags);
     if (r->shape_blk && r->mode != WAR_MODE_CANON) return fail(10, "shape needs canon mode");
     if (r->shape_blk < 0 || r->shape_blk > 512) return fail(10, "shape_blk must be in [0, 512]");
     if (r->detector != WAR_DET_BRENT && r->detector != WAR_DET_STREAK && r->detector != WAR_DET_ALT &&
@@ -213,7 +225,7 @@ war_tuning default_tuning(int n, int mode, int det) {
 // for this (n, mode, rule), then the default.
 void resolve_tuning(const Engine &E, const war_request &r, war_tuning &t) {
     t = default_tuning(r.n, r.mode, r.detector);
-    auto it = E.tuned.find(std::make_tuple(r.n, r.mode, r.rule));
+    auto it = E.tuned.find(std::make_tuple(r.n, r.mode, r.rule, r.detector));
     if (it != E.tuned.end()) {
         t = it->second;
         if (t.pile_layout == 6 && !held_ok(r)) t.pile_layout = 0;   // tuned on requests without walks
@@ -226,6 +238,17 @@ void resolve_tuning(const Engine &E, const war_request &r, war_tuning &t) {
         if (src->stage_kind > 0) t.stage_kind = src->stage_kind;
         if (src->pile_layout > 0) t.pile_layout = src->pile_layout;
     }
+    // the pair-run detector exists on the flow kernel only
+    if (r.detector == WAR_DET_PAIR && r.tuning.stage_kind == 0 && E.tun.stage_kind == 0) t.stage_kind = 3;
+}
+
+// An explicit step cap needs a kernel that tests it exactly: the plain kernel (any cap), or the flow
+// kernel (a multiple of its spans of 16 and 32 turns, tested at every span end).
+int check_cap(const war_request &r, const war_tuning &t) {
+    if (!r.step_cap || r.kernel == WAR_KERNEL_PLAIN) return 0;
+    if (r.step_cap % 32u) return fail(13, "step_cap on the persistent kernel must be a multiple of 32 (got %u)", r.step_cap);
+    if (t.stage_kind != 3) return fail(13, "step_cap on the persistent kernel needs the flow schedule (stage_kind 3)");
+    return 0;
 }
 
 int check_tuning(const war_tuning &t) {
@@ -290,6 +313,8 @@ int req_alloc(Engine &E, ReqHost &h) {
     d.reach = h.reach ? 1u : 0u;
     d.reach_cap = h.reach_cap;
     d.nolen = nr ? d.dhist + nd + nt : nullptr;
+    d.cap = r.step_cap;
+    E.slot_cap[h.slot] = r.step_cap;
     HIPCHK(hipMemcpyAsync(&E.reqs[h.slot], &d, sizeof(ReqDev), hipMemcpyHostToDevice, E.st));
     return 0;
 }
@@ -322,7 +347,11 @@ int req_collect(Engine &E, ReqHost &h) {
         o->device_ms = h.dev_ms;
     }
     if (buf[C_HOVF]) return fail(20, "distinct hash set overflowed (%llu keys dropped)", buf[C_HOVF]);
-    if (buf[C_UNDET]) return fail(21, "%llu games still running after 2^24 turns", buf[C_UNDET]);
+    // undetermined games are a count; an error where the caller asks for it, and for the default cap
+    // of the exact detectors (a game that runs 2^24 turns means a corrupted state)
+    if (buf[C_UNDET] && ((r.flags & WAR_FLAG_UNDET_ERROR) || (!r.step_cap && r.detector != WAR_DET_STREAK)))
+        return fail(21, "%llu games not resolved within %u turns", buf[C_UNDET],
+                    r.step_cap ? r.step_cap : (r.detector == WAR_DET_STREAK ? WAR_STREAK_CAP : WAR_SAFETY_CAP));
     const u64 want = r.hi - r.lo;
     if (r.mode != WAR_MODE_HALF && buf[C_DONE] != want)
         return fail(22, "played %llu positions, expected %llu", buf[C_DONE], want);
@@ -331,7 +360,7 @@ int req_collect(Engine &E, ReqHost &h) {
 
 void req_free(Engine &E, ReqHost &h) {
     if (h.dmem) { (void)hipFree(h.dmem); h.dmem = nullptr; }
-    if (h.slot >= 0) { E.free_slots.push_back(h.slot); h.slot = -1; }
+    if (h.slot >= 0) { E.slot_cap[h.slot] = 0; E.free_slots.push_back(h.slot); h.slot = -1; }
 }
 
 // ---- queue buffers of one layout, allocated on first use
@@ -471,20 +500,21 @@ void launch_flow(int det, int blocks, size_t lds, hipStream_t st, const StageArg
     }
 }
 
+// Enqueue one batch's kernels on stream st (inside a capture): k_setup, then the flow kernel or the
+// staged pipeline, then the cycle walks. pipe / hdr / jobs: the batch's counters and job list.
 template <int W, int RULE>
-int build_graph(Engine &E, const GraphKey &key, hipGraphExec_t *out) {
+void record_launch(Engine &E, const GraphKey &key, hipStream_t st, Pipe *pipe, JobHdr *hdr, Job *jobsd) {
     typedef typename Lay<W>::T T;
     QueueBufs &Q = qbufs<W>(E);
     const int blocks = std::max(1, E.cus * key.wpc / 4);
     const size_t lds = (size_t)4 * (key.hb + key.lb) * sizeof(u32);
     const std::vector<u32> caps = key.kind == 3 ? std::vector<u32>{}
                                   : key.det == WAR_DET_ALT ? stage_caps_alt(key.cap0) : stage_caps(key.cap0);
-    HIPCHK(hipStreamBeginCapture(E.st, hipStreamCaptureModeThreadLocal));
-    hipLaunchKernelGGL(k_setup, dim3(256), dim3(256), 0, E.st, E.pipe, (const JobHdr *)E.hdr,
+    hipLaunchKernelGGL(k_setup, dim3(256), dim3(256), 0, st, pipe, (const JobHdr *)hdr,
                        key.walk ? Q.ccnt : (u32 *)nullptr);
     if (key.kind == 3) {                // the flow kernel: one launch plays the whole batch
         StageArgs<W> A{};
-        A.pipe = E.pipe; A.hdr = E.hdr; A.jobs = E.jobs; A.reqs = E.reqs;
+        A.pipe = pipe; A.hdr = hdr; A.jobs = jobsd; A.reqs = E.reqs;
         A.per = (u32)key.per; A.tile_pos = 64u * (u32)key.per;
         A.hb = (u32)key.hb; A.lb = (u32)key.lb;
         A.flow_min = (u32)key.cap0;
@@ -492,23 +522,23 @@ int build_graph(Engine &E, const GraphKey &key, hipGraphExec_t *out) {
         if (key.walk) {
             A.cq = cyc_queue<W>(E);
             if constexpr (CW) {
-                if (key.multi) launch_flow<W, RULE, true, true>(key.det, blocks, lds, E.st, A);
-                else launch_flow<W, RULE, true, false>(key.det, blocks, lds, E.st, A);
+                if (key.multi) launch_flow<W, RULE, true, true>(key.det, blocks, lds, st, A);
+                else launch_flow<W, RULE, true, false>(key.det, blocks, lds, st, A);
             }
         } else if (key.blk) {
             if constexpr (W == 4) {
-                if (key.multi) launch_flow<4, RULE, false, true, true>(key.det, blocks, lds, E.st, A);
-                else launch_flow<4, RULE, false, false, true>(key.det, blocks, lds, E.st, A);
+                if (key.multi) launch_flow<4, RULE, false, true, true>(key.det, blocks, lds, st, A);
+                else launch_flow<4, RULE, false, false, true>(key.det, blocks, lds, st, A);
             }
         } else {
-            if (key.multi) launch_flow<W, RULE, false, true>(key.det, blocks, lds, E.st, A);
-            else launch_flow<W, RULE, false, false>(key.det, blocks, lds, E.st, A);
+            if (key.multi) launch_flow<W, RULE, false, true>(key.det, blocks, lds, st, A);
+            else launch_flow<W, RULE, false, false>(key.det, blocks, lds, st, A);
         }
     }
     u32 a0 = 0;
     for (size_t s = 0; s < caps.size(); s++) {
         StageArgs<W> A{};
-        A.pipe = E.pipe; A.hdr = E.hdr; A.jobs = E.jobs; A.reqs = E.reqs;
+        A.pipe = pipe; A.hdr = hdr; A.jobs = jobsd; A.reqs = E.reqs;
         const int in = (int)((s + 1) & 1), out = (int)(s & 1);
         A.qin.a = (T *)Q.qa[in]; A.qin.b = (T *)Q.qb[in]; A.qin.req = Q.qreq[in]; A.qin.pos = Q.qpos[in];
         A.qin.home = Q.qhome[in]; A.qin.cnt = Q.qcnt[in];
@@ -526,18 +556,32 @@ int build_graph(Engine &E, const GraphKey &key, hipGraphExec_t *out) {
         // and the first (enumerating) stage are lockstep
         const bool refill = W != 6 && key.det != WAR_DET_ALT && key.kind == 1 && s > 0 && caps[s] != UNBOUNDED;
         A.in_refill = (W != 6 && key.det != WAR_DET_ALT && key.kind == 1 && s > 1) ? 1u : 0u;
-        if (refill) launch_refill<W, RULE>(flags, blocks, lds, E.st, A);
-        else launch_stage<W, RULE>(s == 0, flags, blocks, lds, E.st, A);
+        if (refill) launch_refill<W, RULE>(flags, blocks, lds, st, A);
+        else launch_stage<W, RULE>(s == 0, flags, blocks, lds, st, A);
         a0 = caps[s];
     }
     if constexpr (W != 6) {
         if (key.walk) {
             WalkArgs<W> WA{};
-            WA.pipe = E.pipe; WA.hdr = E.hdr; WA.reqs = E.reqs; WA.cq = cyc_queue<W>(E); WA.tile_pos = 64u * (u32)key.per;
+            WA.pipe = pipe; WA.hdr = hdr; WA.reqs = E.reqs; WA.cq = cyc_queue<W>(E); WA.tile_pos = 64u * (u32)key.per;
             WA.stack = E.reach_stack;
-            hipLaunchKernelGGL((k_walk<W, RULE>), dim3(blocks), dim3(256), 0, E.st, WA);
+            hipLaunchKernelGGL((k_walk<W, RULE>), dim3(blocks), dim3(256), 0, st, WA);
         }
     }
+}
+
+void record_launch_any(Engine &E, const GraphKey &key, hipStream_t st, Pipe *pipe, JobHdr *hdr, Job *jobsd) {
+    switch (key.W * 2 + key.rule) {
+        case 8: record_launch<4, 0>(E, key, st, pipe, hdr, jobsd); break;
+        case 9: record_launch<4, 1>(E, key, st, pipe, hdr, jobsd); break;
+        case 10: record_launch<5, 0>(E, key, st, pipe, hdr, jobsd); break;
+        case 11: record_launch<5, 1>(E, key, st, pipe, hdr, jobsd); break;
+        case 12: record_launch<6, 0>(E, key, st, pipe, hdr, jobsd); break;
+        default: record_launch<6, 1>(E, key, st, pipe, hdr, jobsd); break;
+    }
+}
+
+int end_capture(Engine &E, hipGraphExec_t *out) {
     hipGraph_t g = nullptr;
     hipError_t ce = hipStreamEndCapture(E.st, &g);
     if (ce != hipSuccess) return fail(30, "graph capture failed: %s", hipGetErrorString(ce));
@@ -546,11 +590,24 @@ int build_graph(Engine &E, const GraphKey &key, hipGraphExec_t *out) {
     return 0;
 }
 
-// One batch of jobs (all of layout W and rule RULE) through the persistent pipeline.
 template <int W, int RULE>
-int run_persistent_batch(Engine &E, std::vector<Job> &jobs, const war_tuning &t, int hb, int lb, bool walk,
-                         bool multi, float *ms, int det, bool blk = false) {
-    if (jobs.empty()) return 0;
+int build_graph(Engine &E, const GraphKey &key, hipGraphExec_t *out) {
+    HIPCHK(hipStreamBeginCapture(E.st, hipStreamCaptureModeThreadLocal));
+    record_launch<W, RULE>(E, key, E.st, E.pipe, E.hdr, E.jobs);
+    return end_capture(E, out);
+}
+
+// One persistent launch planned for a batch: the graph key of its kernels and its job list.
+struct Launch {
+    GraphKey key{};
+    std::vector<Job> jobs;
+    bool flow = false;
+};
+
+// Plan one launch of jobs (all of layout W and rule RULE): the checks, tiles and graph key.
+template <int W, int RULE>
+int plan_launch(Engine &E, std::vector<Job> &jobs, const war_tuning &t, int hb, int lb, bool walk, bool multi,
+                int det, bool blk, Launch *out) {
     if ((int)jobs.size() > MAXJOBS) return fail(31, "too many jobs in one batch");
     const bool alt = det == WAR_DET_ALT;
     // the flow kernel (stage_kind 3, the default) plays every batch, of one request or several;
@@ -561,31 +618,58 @@ int run_persistent_batch(Engine &E, std::vector<Job> &jobs, const war_tuning &t,
     int rc = (flow && !walk) ? 0 : ensure_queues<W>(E, walk, alt);
     if (rc) return rc;
     const u64 tile_pos = (u64)t.ranks_per_refill;
-    u64 tiles = 0, npos = 0;
-    for (Job &j : jobs) { j.tile_base = tiles; tiles += (j.hi - j.lo + tile_pos - 1) / tile_pos; npos += j.hi - j.lo; }
+    u64 tiles = 0;
+    for (Job &j : jobs) { j.tile_base = tiles; tiles += (j.hi - j.lo + tile_pos - 1) / tile_pos; }
     const int kind = flow ? 3 : (t.stage_kind == 3 ? 2 : t.stage_kind);
     // flow: cap0 holds the refill threshold (free lanes of 64), brent_cap's stage windows unused
     const int cap0 = flow ? FLOW_MIN_FREE : t.brent_cap;
-    GraphKey key{W, RULE, 1, t.waves_per_cu, t.ranks_per_refill / 64, cap0, walk ? 1 : 0, multi ? 1 : 0,
-                 round_pow2(hb), round_pow2(lb), kind, det, (blk && flow && W == 4 && !walk) ? 1 : 0};
+    out->key = GraphKey{W, RULE, 1, t.waves_per_cu, t.ranks_per_refill / 64, cap0, walk ? 1 : 0, multi ? 1 : 0,
+                        round_pow2(hb), round_pow2(lb), kind, det, (blk && flow && W == 4 && !walk) ? 1 : 0};
     if ((!flow || walk) && (tiles > qbufs<W>(E).tmax || tiles * tile_pos > qbufs<W>(E).cap))
         return fail(32, "batch exceeds the queue capacity");
+    out->jobs = jobs;
+    out->flow = flow;
+    return 0;
+}
+
+// Write a launch's job list (header, then the jobs) into pinned staging at sp.
+void stage_jobs(Engine &E, const Launch &L, char *sp) {
     JobHdr h{};
-    h.njobs = (u32)jobs.size(); h.req0 = jobs[0].req; h.ntiles = tiles;
+    u64 tiles = 0;
+    const u64 tp = 64u * (u64)L.key.per;
+    for (const Job &j : L.jobs) tiles += (j.hi - j.lo + tp - 1) / tp;
+    h.njobs = (u32)L.jobs.size(); h.req0 = L.jobs[0].req; h.ntiles = tiles;
+    for (const Job &j : L.jobs) h.capck |= E.slot_cap[j.req] ? 1u : 0u;
+    memcpy(sp, &h, sizeof h);
+    memcpy(sp + JOBS_OFF, L.jobs.data(), L.jobs.size() * sizeof(Job));
+}
+
+int get_graph(Engine &E, const GraphKey &key, hipGraphExec_t *ge) {
+    auto it = E.graphs.find(key);
+    if (it != E.graphs.end()) { *ge = it->second; return 0; }
+    int rc;
+    switch (key.W * 2 + key.rule) {
+        case 8: rc = build_graph<4, 0>(E, key, ge); break;
+        case 9: rc = build_graph<4, 1>(E, key, ge); break;
+        case 10: rc = build_graph<5, 0>(E, key, ge); break;
+        case 11: rc = build_graph<5, 1>(E, key, ge); break;
+        case 12: rc = build_graph<6, 0>(E, key, ge); break;
+        default: rc = build_graph<6, 1>(E, key, ge); break;
+    }
+    if (!rc) E.graphs[key] = *ge;
+    return rc;
+}
+
+// One planned launch as its own graph (the engine's slot-0 pipe and job list), synchronously.
+int exec_single(Engine &E, const Launch &L, float *ms) {
+    if (L.jobs.empty()) return 0;
     // copy the job list in (the staging buffer is free: the previous batch was synchronised)
     char *sp = (char *)E.stage_host;
-    memcpy(sp, &h, sizeof h);
-    memcpy(sp + JOBS_OFF, jobs.data(), jobs.size() * sizeof(Job));
-    HIPCHK(hipMemcpyAsync(E.hdr, sp, JOBS_OFF + jobs.size() * sizeof(Job), hipMemcpyHostToDevice, E.st));
-    auto it = E.graphs.find(key);
+    stage_jobs(E, L, sp);
+    HIPCHK(hipMemcpyAsync(E.hdr, sp, JOBS_OFF + L.jobs.size() * sizeof(Job), hipMemcpyHostToDevice, E.st));
     hipGraphExec_t ge = nullptr;
-    if (it == E.graphs.end()) {
-        rc = build_graph<W, RULE>(E, key, &ge);
-        if (rc) return rc;
-        E.graphs[key] = ge;
-    } else {
-        ge = it->second;
-    }
+    int rc = get_graph(E, L.key, &ge);
+    if (rc) return rc;
     HIPCHK(hipEventRecord(E.ev0, E.st));
     HIPCHK(hipGraphLaunch(ge, E.st));
     HIPCHK(hipEventRecord(E.ev1, E.st));
@@ -595,9 +679,113 @@ int run_persistent_batch(Engine &E, std::vector<Job> &jobs, const war_tuning &t,
     float m = 0;
     HIPCHK(hipEventElapsedTime(&m, E.ev0, E.ev1));
     if (ms) *ms = m;
-    (void)npos;
     if (ovf) return fail(33, "persistent pipeline ran out of queue regions (%llu refused)", ovf);
     return 0;
+}
+
+// Several planned launches. Flow launches (up to MAXB of them) run as ONE graph: each launch on a
+// branch of its own with its own pipe counters and job list, forked from and joined to the engine's
+// stream; launches with cycle walks of one pile layout share that layout's cycle records and
+// follow each other on one branch. Anything else (staged launches, more than MAXB) runs launch by
+// launch. *ms: device time; *graphs: graph launches made.
+int exec_plan(Engine &E, std::vector<Launch> &plan, double *ms, int *graphs) {
+    size_t nl = 0;
+    for (const Launch &L : plan) nl += L.jobs.empty() ? 0 : 1;
+    if (nl == 0) return 0;
+    bool all_flow = true;
+    for (const Launch &L : plan) all_flow = all_flow && (L.jobs.empty() || L.flow);
+    if (nl == 1 || !all_flow || nl > (size_t)MAXB) {
+        for (const Launch &L : plan) {
+            if (L.jobs.empty()) continue;
+            float m = 0;
+            const int rc = exec_single(E, L, &m);
+            if (ms) *ms += m;
+            if (graphs) ++*graphs;
+            if (rc) return rc;
+        }
+        return 0;
+    }
+    // branches: each launch without walks alone; the launches with walks, one branch per layout
+    std::vector<const Launch *> ls;
+    for (const Launch &L : plan) if (!L.jobs.empty()) ls.push_back(&L);
+    std::vector<std::vector<int>> br;
+    int walk_branch[7] = {-1, -1, -1, -1, -1, -1, -1};
+    for (int i = 0; i < (int)ls.size(); i++) {
+        const GraphKey &k = ls[(size_t)i]->key;
+        if (k.walk) {
+            int &b = walk_branch[k.W];
+            if (b < 0) { b = (int)br.size(); br.emplace_back(); }
+            br[(size_t)b].push_back(i);
+        } else {
+            br.push_back({i});
+        }
+    }
+    // the graph's shape: branch by branch, the keys of its launches (slot = position in ls)
+    std::vector<int> bkey;
+    for (const auto &b : br) {
+        bkey.push_back(-1);
+        for (int i : b) {
+            const GraphKey &k = ls[(size_t)i]->key;
+            for (int v : {i, k.W, k.rule, k.kernel, k.wpc, k.per, k.cap0, k.walk, k.multi, k.hb, k.lb, k.kind, k.det, k.blk})
+                bkey.push_back(v);
+        }
+    }
+    // job lists of every slot, one copy
+    char *sp = (char *)E.bstage;
+    for (size_t i = 0; i < ls.size(); i++) stage_jobs(E, *ls[i], sp + i * SLOT_BYTES);
+    HIPCHK(hipMemcpyAsync(E.bhdr, sp, ls.size() * SLOT_BYTES, hipMemcpyHostToDevice, E.st));
+    hipGraphExec_t ge = nullptr;
+    auto it = E.bgraphs.find(bkey);
+    if (it != E.bgraphs.end()) {
+        ge = it->second;
+    } else {
+        HIPCHK(hipStreamBeginCapture(E.st, hipStreamCaptureModeThreadLocal));
+        HIPCHK(hipEventRecord(E.bfork, E.st));
+        for (size_t b = 0; b < br.size(); b++) {
+            hipStream_t st = E.bst[b];
+            HIPCHK(hipStreamWaitEvent(st, E.bfork, 0));
+            for (int i : br[b]) {
+                char *hd = E.bhdr + (size_t)i * SLOT_BYTES;
+                record_launch_any(E, ls[(size_t)i]->key, st, E.bpipe + i, (JobHdr *)hd, (Job *)(hd + JOBS_OFF));
+            }
+            HIPCHK(hipEventRecord(E.bjoin[b], st));
+            HIPCHK(hipStreamWaitEvent(E.st, E.bjoin[b], 0));
+        }
+        const int rc = end_capture(E, &ge);
+        if (rc) return rc;
+        E.bgraphs[bkey] = ge;
+    }
+    HIPCHK(hipEventRecord(E.ev0, E.st));
+    HIPCHK(hipGraphLaunch(ge, E.st));
+    HIPCHK(hipEventRecord(E.ev1, E.st));
+    std::vector<Pipe> pp(ls.size());
+    HIPCHK(hipMemcpyAsync(pp.data(), E.bpipe, ls.size() * sizeof(Pipe), hipMemcpyDeviceToHost, E.st));
+    HIPCHK(hipStreamSynchronize(E.st));
+    float m = 0;
+    HIPCHK(hipEventElapsedTime(&m, E.ev0, E.ev1));
+    if (ms) *ms += m;
+    if (graphs) ++*graphs;
+    for (const Pipe &p : pp)
+        if (p.ovf) return fail(33, "persistent pipeline ran out of queue regions (%llu refused)", p.ovf);
+    return 0;
+}
+
+// One batch of jobs (all of layout W and rule RULE) through the persistent pipeline, synchronously.
+template <int W, int RULE>
+int run_persistent_batch(Engine &E, std::vector<Job> &jobs, const war_tuning &t, int hb, int lb, bool walk,
+                         bool multi, float *ms, int det, bool blk = false) {
+    if (jobs.empty()) return 0;
+    Launch L;
+    const int rc = plan_launch<W, RULE>(E, jobs, t, hb, lb, walk, multi, det, blk, &L);
+    if (rc) return rc;
+    return exec_single(E, L, ms);
+}
+
+template <int W>
+int plan_dispatch(Engine &E, int rule, std::vector<Job> &jobs, const war_tuning &t, int hb, int lb, bool walk,
+                  bool multi, int det, bool blk, Launch *out) {
+    return rule ? plan_launch<W, 1>(E, jobs, t, hb, lb, walk, multi, WAR_DET_BRENT, blk, out)
+                : plan_launch<W, 0>(E, jobs, t, hb, lb, walk, multi, det, blk, out);
 }
 
 template <int W, int RULE>
@@ -682,6 +870,7 @@ int run_request_locked(Engine &E, ReqHost &h) {
     war_tuning t;
     resolve_tuning(E, r, t);
     int rc = check_tuning(t);
+    if (!rc) rc = check_cap(r, t);
     if (rc) return rc;
     int W = 4;
     rc = layout_of(r, t, &W);
@@ -757,8 +946,8 @@ int run_request_locked(Engine &E, ReqHost &h) {
 // packed, in order, into as few launches as the batch capacity allows (FLOW_BATCH positions on
 // the flow kernel, the queue capacity with cycle walks or the staged pipeline; MAXJOBS ranges).
 struct GroupRange { u32 slot, n, mode; u64 lo, hi; };
-int run_group(Engine &E, int W, int rule, int det, bool walk, const war_tuning &gt, int hb, int lb,
-              const std::vector<GroupRange> &rs, bool blocks_apart, double *dev, int *launches) {
+int plan_group(Engine &E, int W, int rule, int det, bool walk, const war_tuning &gt, int hb, int lb,
+               const std::vector<GroupRange> &rs, bool blocks_apart, std::vector<Launch> *plan) {
     std::vector<GroupRange> rest, blocks;
     war_tuning bt = gt;
     {
@@ -787,12 +976,11 @@ int run_group(Engine &E, int W, int rule, int det, bool walk, const war_tuning &
         int rc = 0;
         auto flush = [&]() -> int {
             if (jobs.empty()) return 0;
-            float ms = 0;
-            const int r2 = (W == 4) ? run_batch_dispatch<4>(E, rule, jobs, tt, hb, lb, walk, true, &ms, det, blk)
-                         : (W == 6) ? run_batch_dispatch<6>(E, rule, jobs, tt, hb, lb, false, true, &ms, det)
-                                    : run_batch_dispatch<5>(E, rule, jobs, tt, hb, lb, walk, true, &ms, det);
-            *dev += ms;
-            if (launches) ++*launches;
+            plan->emplace_back();
+            Launch *L = &plan->back();
+            const int r2 = (W == 4) ? plan_dispatch<4>(E, rule, jobs, tt, hb, lb, walk, true, det, blk, L)
+                         : (W == 6) ? plan_dispatch<6>(E, rule, jobs, tt, hb, lb, false, true, det, false, L)
+                                    : plan_dispatch<5>(E, rule, jobs, tt, hb, lb, walk, true, det, false, L);
             jobs.clear();
             used = 0;
             return r2;
@@ -819,14 +1007,16 @@ int run_group(Engine &E, int W, int rule, int det, bool walk, const war_tuning &
 // Several requests at once, synchronously (engine mutex held by the caller): the serving loop's
 // multi-request launches without its thread. Requests are grouped by pile layout, rule, detector
 // and whether they need cycle walks (so requests without walks keep the larger batches of the
-// kernel without them), and each group's whole ranges run as run_group launches: one while they
-// fit, or two with deal blocks. *launches counts them.
-int run_batch_locked(Engine &E, std::vector<ReqHost> &hs, int *launches) {
+// kernel without them); each group's whole ranges are planned as launches (one while they fit, or
+// two with deal blocks), and every launch of the batch runs as branches of one graph (exec_plan).
+// *launches counts the graph launches, *dev_total the device time.
+int run_batch_locked(Engine &E, std::vector<ReqHost> &hs, int *launches, double *dev_total = nullptr) {
     int rc = 0;
     for (auto &h : hs) {
         war_tuning tt;
         resolve_tuning(E, h.rq, tt);
         rc = check_tuning(tt);
+        if (!rc) rc = check_cap(h.rq, tt);
         if (!rc) rc = layout_of(h.rq, tt, &h.layout);
         if (!rc) {
             h.slot = alloc_slot(E);
@@ -835,6 +1025,7 @@ int run_batch_locked(Engine &E, std::vector<ReqHost> &hs, int *launches) {
         if (!rc) rc = req_alloc(E, h);
         if (rc) break;
     }
+    std::vector<Launch> plan;
     for (int W : {4, 5, 6}) {
         for (int rule : {0, 1}) {
             for (int alt : {0, 1}) {
@@ -855,15 +1046,17 @@ int run_batch_locked(Engine &E, std::vector<ReqHost> &hs, int *launches) {
                     walk = walk || h->dev.walk;
                 }
                 const int det = alt ? WAR_DET_ALT : WAR_DET_BRENT;
-                double dev = 0;
                 std::vector<GroupRange> rs;
                 for (ReqHost *h : grp) rs.push_back({(u32)h->slot, (u32)h->rq.n, (u32)h->rq.mode, h->rq.lo, h->rq.hi});
-                rc = run_group(E, W, rule, det, walk, gt, hb, lb, rs, true, &dev, launches);
-                for (ReqHost *h : grp) h->dev_ms = dev;     // device time of the group's launches
+                rc = plan_group(E, W, rule, det, walk, gt, hb, lb, rs, true, &plan);
               }
             }
         }
     }
+    double dev = 0;
+    if (!rc) rc = exec_plan(E, plan, &dev, launches);
+    for (auto &h : hs) h.dev_ms = dev;     // device time of the batch's graph
+    if (dev_total) *dev_total = dev;
     for (auto &h : hs) {
         if (!rc && h.slot >= 0) rc = req_collect(E, h);
         req_free(E, h);
@@ -874,7 +1067,8 @@ int run_batch_locked(Engine &E, std::vector<ReqHost> &hs, int *launches) {
 // ---------------------------------------------------------------------------------------------
 // Serving loop: continuous batching. Each step takes, from every admitted request, its next
 // share of positions (step_positions split evenly, whole tiles), runs each group of one layout,
-// rule, detector and walk flag as one run_group launch, and retires finished requests. Requests that arrive during a step
+// rule, detector and walk flag as planned launches, runs every group's launches as one graph
+// (exec_plan), and retires finished requests. Requests that arrive during a step
 // are admitted at the next one, while slots and device memory allow.
 void serve_loop(Engine *Ep) {
     Engine &E = *Ep;
@@ -908,6 +1102,7 @@ void serve_loop(Engine *Ep) {
                 war_tuning tt;
                 resolve_tuning(E, h->rq, tt);
                 rc = check_tuning(tt);
+                if (!rc) rc = check_cap(h->rq, tt);
                 if (!rc) rc = layout_of(h->rq, tt, &h->layout);
             }
             if (!rc) {
@@ -928,6 +1123,8 @@ void serve_loop(Engine *Ep) {
         if (active.empty()) continue;
         // one launch per (layout, rule, detector, cycle walks) group: requests without walks keep
         // the kernel without them
+        std::vector<Launch> plan;
+        std::vector<std::shared_ptr<ReqHost>> stepped;
         int rc = 0;
         for (int W : {4, 5, 6}) {
             for (int rule : {0, 1})
@@ -954,16 +1151,27 @@ void serve_loop(Engine *Ep) {
                     hb = std::max(hb, h->rq.hbins); lb = std::max(lb, h->rq.lbins);
                     walk = walk || h->dev.walk;
                 }
-                double dms = 0;
                 const int det = alt ? WAR_DET_ALT : WAR_DET_BRENT;
-                rc = run_group(E, W, rule, det, walk, gt, hb, lb, rs, false, &dms, nullptr);
-                const float ms = (float)dms;
-                for (auto &h : grp) h->dev_ms += ms;   // device time of the steps the request took part in
+                const size_t np = plan.size();
+                rc = plan_group(E, W, rule, det, walk, gt, hb, lb, rs, false, &plan);
                 if (rc) {
+                    plan.resize(np);
                     for (auto &h : grp) { h->status = rc; h->err = t_err; h->next = h->rq.hi; }
                     rc = 0;
+                } else {
+                    stepped.insert(stepped.end(), grp.begin(), grp.end());
                 }
             }
+        }
+        // the step's launches, every group's, as one graph
+        {
+            double dms = 0;
+            rc = exec_plan(E, plan, &dms, nullptr);
+            for (auto &h : stepped) {
+                h->dev_ms += dms;   // device time of the steps the request took part in
+                if (rc) { h->status = rc; h->err = t_err; h->next = h->rq.hi; }
+            }
+            rc = 0;
         }
         // retire
         for (size_t i = 0; i < active.size();) {
@@ -996,13 +1204,7 @@ extern "C" {
 
 int war_abi_version(void) { return WAR_ABI_VERSION; }
 
-const char *war_last_error(void) {
-    if (!t_err.empty()) return t_err.c_str();
-    std::lock_guard<std::mutex> g(g_err_mu);
-    static thread_local std::string copy;
-    copy = g_err_shared;
-    return copy.c_str();
-}
+const char *war_last_error(void) { return t_err.c_str(); }
 
 int war_init(int32_t device, uint32_t flags) {
     t_err.clear();
@@ -1033,6 +1235,15 @@ int war_init(int32_t device, uint32_t flags) {
     E->jobs = (Job *)((char *)jb + JOBS_OFF);
     HIPCHK(hipHostMalloc(&E->stage_host, JOBS_OFF + MAXJOBS * sizeof(Job), hipHostMallocDefault));
     HIPCHK(hipMalloc((void **)&E->reqs, MAXSLOTS * sizeof(ReqDev)));
+    static_assert(sizeof(Job) == 32, "Job is 32 bytes (SLOT_BYTES)");
+    HIPCHK(hipMalloc((void **)&E->bpipe, MAXB * sizeof(Pipe)));
+    HIPCHK(hipMalloc((void **)&E->bhdr, MAXB * SLOT_BYTES));
+    HIPCHK(hipHostMalloc(&E->bstage, MAXB * SLOT_BYTES, hipHostMallocDefault));
+    for (int b = 0; b < MAXB; b++) {
+        HIPCHK(hipStreamCreateWithFlags(&E->bst[b], hipStreamNonBlocking));   // default priority
+        HIPCHK(hipEventCreateWithFlags(&E->bjoin[b], hipEventDisableTiming));
+    }
+    HIPCHK(hipEventCreateWithFlags(&E->bfork, hipEventDisableTiming));
     HIPCHK(hipMemsetAsync(E->reqs, 0, MAXSLOTS * sizeof(ReqDev), E->st));
     HIPCHK(hipMemsetAsync(E->pipe, 0, sizeof(Pipe), E->st));
     HIPCHK(hipStreamSynchronize(E->st));
@@ -1051,6 +1262,12 @@ int war_shutdown(void) {
         std::lock_guard<std::mutex> g(E->mu);
         (void)hipStreamSynchronize(E->st);
         for (auto &kv : E->graphs) (void)hipGraphExecDestroy(kv.second);
+        for (auto &kv : E->bgraphs) (void)hipGraphExecDestroy(kv.second);
+        for (int b = 0; b < MAXB; b++) { (void)hipStreamDestroy(E->bst[b]); (void)hipEventDestroy(E->bjoin[b]); }
+        (void)hipEventDestroy(E->bfork);
+        (void)hipFree(E->bpipe);
+        (void)hipFree(E->bhdr);
+        (void)hipHostFree(E->bstage);
         for (QueueBufs *Q : {&E->qb4, &E->qb5}) {
             for (int i = 0; i < 2; i++) {
                 (void)hipFree(Q->qa[i]); (void)hipFree(Q->qb[i]); (void)hipFree(Q->qreq[i]); (void)hipFree(Q->qpos[i]);
@@ -1265,84 +1482,161 @@ int war_cycle_info_batch(const war_pile_position *pos, int32_t count, int32_t ru
 int war_autotune(int32_t n, int32_t mode, int32_t rule, double budget_ms, war_tuning *best) {
     t_err.clear();
     if (!g_eng) return fail(60, "war_init has not been called");
-    Engine &E = *g_eng;
     war_request r{};
     r.n = n; r.mode = mode; r.rule = rule; r.kernel = WAR_KERNEL_PERSISTENT;
+    if (n < 1 || n > 20 || mode < 0 || mode > 2) return fail(2, "autotune: n must be in [1, 20] and mode FULL, HALF or CANON");
     const u64 total = range_total(n, mode);
     // sample: a slice from the middle of the range, big enough to fill the GPU many times over
     const u64 want = std::min<u64>(total, 1ull << 26);
     r.lo = total / 2 - std::min(total / 2, want / 2);
     r.hi = r.lo + want;
-    int rc = check_request(&r);
-    if (rc) return rc;
+    return war_autotune_batch(&r, 1, budget_ms, nullptr, best);
+}
+
+int war_autotune_batch(const war_request *reqs, int32_t count, double budget_ms, const war_result *expect,
+                       war_tuning *best) {
+    t_err.clear();
+    if (!g_eng) return fail(60, "war_init has not been called");
+    if (count < 1 || count > MAXSLOTS - 8 || !reqs) return fail(1, "autotune: 1 to %d requests", MAXSLOTS - 8);
+    Engine &E = *g_eng;
+    for (int i = 0; i < count; i++) {
+        int rc = check_request(&reqs[i]);
+        if (!rc && reqs[i].kernel != WAR_KERNEL_PERSISTENT) rc = fail(82, "autotune: request %d is not for the persistent kernel", i);
+        if (!rc && reqs[i].detector != WAR_DET_BRENT && reqs[i].detector != WAR_DET_ALT && count > 1)
+            rc = fail(82, "autotune: a batch runs Brent or the alternation detector (request %d)", i);
+        if (!rc && reqs[i].detector == WAR_DET_STREAK) rc = fail(82, "autotune: the streak test runs on the plain kernel");
+        if (rc) return rc;
+    }
+    // result buffers: one set per request for the reference and one for the candidate
+    struct Buf {
+        war_result r{};
+        std::vector<uint64_t> h, l, d;
+        void init(const war_request &q) {
+            h.assign((size_t)std::max(1, q.hbins), 0); l.assign((size_t)std::max(1, q.lbins), 0);
+            d.assign((size_t)std::max(1, q.dbins), 0);
+            r = war_result{};
+            r.hist = h.data(); r.hbins = q.hbins; r.lenhist = l.data(); r.lbins = q.lbins; r.dhist = d.data(); r.dbins = q.dbins;
+        }
+    };
+    std::vector<Buf> ref((size_t)count), cand((size_t)count);
+    for (int i = 0; i < count; i++) { ref[(size_t)i].init(reqs[i]); cand[(size_t)i].init(reqs[i]); }
+    // the counts every schedule must reproduce: the caller's (expect), else the plain kernel's, an
+    // independent code path (one position per lane, the reference's structure)
+    if (!expect) {
+        std::lock_guard<std::mutex> g(E.mu);
+        for (int i = 0; i < count; i++) {
+            ReqHost h;
+            h.rq = reqs[i]; h.rq.kernel = WAR_KERNEL_PLAIN;
+            if (h.rq.detector == WAR_DET_PAIR) h.rq.detector = WAR_DET_BRENT;   // no plain form; turns not compared
+            h.rq.tuning = war_tuning{};
+            h.res = &ref[(size_t)i].r;
+            const int c = run_request_locked(E, h);
+            if (c) return c;
+        }
+    }
+    auto same_counts = [&](int i, const war_result &a) -> bool {
+        const war_result &b = expect ? expect[i] : ref[(size_t)i].r;
+        const war_request &q = reqs[i];
+        if (a.cyc != b.cyc || a.done != b.done || a.maxfin != b.maxfin || a.undetermined != b.undetermined) return false;
+        if (q.detector != WAR_DET_PAIR && a.turns != b.turns) return false;
+        if (q.dbins || q.distinct) { if (a.oncycle != b.oncycle) return false; }
+        if (q.distinct && a.distinct_minkey != b.distinct_minkey) return false;
+        if (q.shape_blk && (a.shape_nonstd != b.shape_nonstd || a.shape_unchk != b.shape_unchk)) return false;
+        for (int k = 0; k < q.hbins && b.hist; k++) if (a.hist[k] != b.hist[k]) return false;
+        for (int k = 0; k < q.lbins && b.lenhist; k++) if (a.lenhist[k] != b.lenhist[k]) return false;
+        for (int k = 0; k < q.dbins && b.dhist; k++) if (a.dhist[k] != b.dhist[k]) return false;
+        return true;
+    };
+    // one schedule: every request with tuning t, as one batch (or one range); timed on its second
+    // run (the first captures its graphs); *ms = device time
+    auto trial = [&](const war_tuning &t, double *ms) -> int {
+        double dev = 0;
+        for (int rep = 0; rep < 2; rep++) {
+            std::lock_guard<std::mutex> g(E.mu);
+            int c = 0;
+            if (count == 1) {
+                ReqHost h;
+                h.rq = reqs[0]; h.rq.tuning = t; cand[0].init(reqs[0]); h.res = &cand[0].r;
+                c = run_request_locked(E, h);
+                dev = cand[0].r.device_ms;
+            } else {
+                std::vector<ReqHost> hs((size_t)count);
+                for (int i = 0; i < count; i++) {
+                    hs[(size_t)i].rq = reqs[i]; hs[(size_t)i].rq.tuning = t;
+                    cand[(size_t)i].init(reqs[i]); hs[(size_t)i].res = &cand[(size_t)i].r;
+                }
+                if ((size_t)count > E.free_slots.size()) return fail(40, "autotune: %d requests, %zu free slots", count, E.free_slots.size());
+                int nl = 0;
+                c = run_batch_locked(E, hs, &nl, &dev);
+            }
+            if (c) return c;
+        }
+        for (int i = 0; i < count; i++)
+            if (!same_counts(i, cand[(size_t)i].r))
+                return fail(70, "autotune: schedule (waves %d, tile %d, cap %d, kind %d, layout %d) changed the counts of "
+                                "request %d (exactness violated)", t.waves_per_cu, t.ranks_per_refill, t.brent_cap,
+                            t.stage_kind, t.pile_layout, i);
+        *ms = dev;
+        return 0;
+    };
     war_tuning cur;
     {
         std::lock_guard<std::mutex> g(E.mu);
-        resolve_tuning(E, r, cur);
+        resolve_tuning(E, reqs[0], cur);
     }
-    war_result ref_res{};
-    bool have_ref = false;
     const double t_start = now_ms();
-    // time one schedule (second run: its graph is then already captured); every schedule must give
-    // the first one's counts exactly
-    auto trial = [&](const war_tuning &t, double *ms) -> int {
-        war_request q = r;
-        q.tuning = t;
-        war_result o{};
-        ReqHost h;
-        h.rq = q; h.res = &o;
-        int c;
-        {
-            std::lock_guard<std::mutex> g(E.mu);
-            c = run_request_locked(E, h);
-            if (!c) { ReqHost h2; h2.rq = q; h2.res = &o; c = run_request_locked(E, h2); }
-        }
-        if (c) return c;
-        if (!have_ref) { ref_res = o; have_ref = true; }
-        else if (o.cyc != ref_res.cyc || o.done != ref_res.done || o.maxfin != ref_res.maxfin || o.turns != ref_res.turns)
-            return fail(70, "autotune: schedule changed the counts (exactness violated)");
-        *ms = o.device_ms;
-        return 0;
-    };
     double best_ms = 1e30;
-    rc = trial(cur, &best_ms);
+    int rc = trial(cur, &best_ms);
     if (rc) return rc;
-    // coordinate descent over (brent_cap, ranks_per_refill, waves_per_cu, stage_kind, pile layout);
-    // the layout dimension exists for n = 17 (5-bit piles, or 4-bit with card 16 held apart)
+    // coordinate descent, two passes, over: the schedule (staged refill / lockstep / flow), the tile
+    // (positions per grab: a multiple of the 64 lanes of a wave, so 64 is the smallest; 32 would
+    // leave half a wave without a position), waves per CU (1..8 as the occupancy sweep, and the
+    // larger counts the flow kernel prefers), the first stage boundary (staged schedules only: the
+    // flow kernel has no stages) and, for n = 17, the pile layout (5-bit or card 16 held apart)
+    const int kinds[] = {3, 1, 2};
+    const int tiles[] = {64, 128, 256, 512, 1024, 2048, 4096, 8192};
+    const int wpcs[] = {1, 2, 3, 4, 5, 6, 7, 8, 12, 16, 20, 24, 28, 32};
     const int caps[] = {7, 15, 31, 63, 127};
-    const int rprs[] = {256, 1024, 2048, 4096};
-    const int wpcs[] = {8, 16, 24, 28, 32};
-    const int kinds[] = {1, 2, 3};
     const int lays[] = {5, 6};
-    const int ndim = held_ok(r) ? 5 : 4;
+    bool held = count == 1 && held_ok(reqs[0]);
+    bool any_cap = false, any_pair = false;
+    for (int i = 0; i < count; i++) { any_cap |= reqs[i].step_cap != 0; any_pair |= reqs[i].detector == WAR_DET_PAIR; }
+    int trials = 1;
     for (int pass = 0; pass < 2; pass++) {
-        for (int dim = 0; dim < ndim; dim++) {
-            const int nc = dim == 0 ? 5 : dim == 1 ? 4 : dim == 2 ? 5 : dim == 3 ? 3 : 2;
+        for (int dim = 0; dim < 5; dim++) {
+            const int *vals = dim == 0 ? kinds : dim == 1 ? tiles : dim == 2 ? wpcs : dim == 3 ? caps : lays;
+            const int nv = dim == 0 ? 3 : dim == 1 ? 8 : dim == 2 ? 14 : dim == 3 ? 5 : 2;
+            if (dim == 0 && (any_cap || any_pair)) continue;    // explicit caps and the pair-run detector: flow only
+            if (dim == 3 && cur.stage_kind == 3) continue;      // no stage boundaries in the flow kernel
+            if (dim == 4 && !held) continue;
             war_tuning bestt = cur;
-            for (int i = 0; i < nc; i++) {
+            for (int i = 0; i < nv; i++) {
                 if (now_ms() - t_start > budget_ms) break;
                 war_tuning t = cur;
-                if (dim == 0) t.brent_cap = caps[i];
-                else if (dim == 1) t.ranks_per_refill = rprs[i];
-                else if (dim == 2) t.waves_per_cu = wpcs[i];
-                else if (dim == 3) t.stage_kind = kinds[i];
-                else t.pile_layout = lays[i];
+                if (dim == 0) t.stage_kind = vals[i];
+                else if (dim == 1) t.ranks_per_refill = vals[i];
+                else if (dim == 2) t.waves_per_cu = vals[i];
+                else if (dim == 3) t.brent_cap = vals[i];
+                else t.pile_layout = vals[i];
                 if (memcmp(&t, &cur, sizeof t) == 0) continue;
                 double ms = 0;
                 rc = trial(t, &ms);
                 if (rc) return rc;
+                trials++;
                 if (E.verbose)
-                    fprintf(stderr, "autotune n=%d mode=%d cap=%d tile=%d wpc=%d kind=%d layout=%d: %.3f ms\n", n,
-                            mode, t.brent_cap, t.ranks_per_refill, t.waves_per_cu, t.stage_kind, t.pile_layout, ms);
+                    fprintf(stderr, "autotune n=%d mode=%d: kind=%d tile=%d wpc=%d cap=%d layout=%d: %.3f ms\n", reqs[0].n,
+                            reqs[0].mode, t.stage_kind, t.ranks_per_refill, t.waves_per_cu, t.brent_cap, t.pile_layout, ms);
                 if (ms < best_ms) { best_ms = ms; bestt = t; }
             }
             cur = bestt;
         }
     }
-    {
+    if (count == 1) {
         std::lock_guard<std::mutex> g(E.mu);
-        E.tuned[std::make_tuple(n, mode, rule)] = cur;
+        const war_request &q = reqs[0];
+        E.tuned[std::make_tuple(q.n, q.mode, q.rule, q.detector)] = cur;
     }
+    if (E.verbose) fprintf(stderr, "autotune: %d schedules in %.0f ms, best %.3f ms\n", trials, now_ms() - t_start, best_ms);
     if (best) *best = cur;
     return 0;
 }

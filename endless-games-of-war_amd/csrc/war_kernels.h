@@ -14,7 +14,7 @@
 //                   state). The FRESH stage enumerates positions from the job list (a global tile
 //                   counter refills waves); later stages drain queues (a global record counter).
 //                   The last stage has no cap (Brent's exactness), only a 2^24-turn safety stop
-//                   that counts as "undetermined" and is reported as an error.
+//                   that counts as "undetermined" (an error for the default cap).
 //  k_walk           per cycling position of a --distinct / --dbins / --shape request: the
 //                   Theorem-3 run test from the meeting state (gpu/war_gpu.py:51-73), and the walk
 //                   of lam turns from the start that decides whether it lies on its cycle, then the
@@ -36,7 +36,8 @@ constexpr int MAXSTAGE = 16;
 
 struct ReqDev {
     u32 n, mode, rule, hbins, lbins, dbins, distinct, shape_blk;
-    u32 walk, hlog2, reach, pad1;
+    u32 walk, hlog2, reach;
+    u32 cap;            // the request's step cap (war_request.step_cap); 0 = the default
     u64 *cnt, *hist, *lenhist, *dhist, *htab;
     u64 reach_cap;      // reach: positions a cycle's backward search may visit
     u64 *nolen;         // reach: cycle lengths of the cycles no deal enters (4096 bins)
@@ -47,6 +48,7 @@ struct Job { u64 lo, hi, tile_base; u32 req, pad; };
 struct JobHdr {
     u32 njobs, req0;        // req0: the request of job 0
     u64 ntiles;
+    u32 capck, pad;         // some request of the batch has an explicit step cap (ReqDev::cap)
 };
 
 struct Pipe {
@@ -1032,6 +1034,8 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
     const u32 per = A.per, TP = A.tile_pos;
     const u64 ntiles = A.hdr->ntiles;
     const u32 min_dead = A.flow_min;       // refill once this many lanes are free (or none is alive)
+    const bool capck = A.hdr->capck != 0u; // some request of the batch has an explicit step cap
+    const u32 rcap = reqs[req].cap;
     u64 *grab = &A.pipe->grab[0];
     u64 nxt = 0;
     if (lane == 0) nxt = atomicAdd((unsigned long long *)grab, 1ull);
@@ -1335,6 +1339,30 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
                 alive &= ~um;
             }
         }
+        // an explicit step cap (war_request.step_cap), tested at every span end when a request of
+        // the batch has one: ages at span ends are multiples of the span, which divides the cap, so
+        // a game stops exactly when it has played the cap's turns unresolved, charged those turns
+        if (capck) {
+            u32 gcap = rcap;
+            if constexpr (MULTI) gcap = reqs[greq].cap;
+            const u64 um = ballot(gcap != 0u && clk - birth >= gcap) & alive;
+            if (um) {
+                for (u64 rem = um; rem;) {
+                    u64 mr = rem;
+                    u32 cp = rcap;
+                    if constexpr (MULTI) {
+                        const u32 r = lanev(greq, (u32)(__ffsll((long long)rem) - 1));
+                        mr = ballot(greq == r) & rem;
+                        acc.select(r, reqs);
+                        cp = reqs[r].cap;
+                    }
+                    rem &= ~mr;
+                    const u32 c = (u32)__popcll(mr);
+                    acc.undet += c; acc.done += c; acc.turns += (u64)c * cp; acc.dirty = true;
+                }
+                alive &= ~um;
+            }
+        }
         if (more && (u32)__popcll(~alive) >= min_dead) refill();
     }
     if constexpr (!MULTI) {    // MULTI batches accounted turns and longest games at each span end
@@ -1369,7 +1397,8 @@ template <int W> struct PlainArgs {
 // The streak test (gpu/war24.py:98-125; odd n, rule 0): anchor at a turn whose winner-to-be holds an
 // even number ws of cards and the other ls >= 2; if the players then alternate for
 // lcm(n+1, ws, ls-1) turns and the state equals the anchor, the game never ends (Spivey Thm 6). A
-// repeated winner drops the anchor. More than 2^22 turns: undetermined (the reference's CAP).
+// repeated winner drops the anchor. Still running with more than the cap's turns played at the top of
+// its loop (2^22 by default, the reference's CAP): undetermined, as gpu/war24.py:103.
 constexpr u32 STREAK_CAP = 1u << 22;
 __device__ __forceinline__ u32 gcd_u(u32 a, u32 b) { while (b) { const u32 t = a % b; a = b; b = t; } return a; }
 __device__ __forceinline__ u32 lcm_u(u32 a, u32 b) { return a / gcd_u(a, b) * b; }
@@ -1393,6 +1422,9 @@ __global__ __launch_bounds__(256) void k_plain(PlainArgs<W> P) {
         else q = unrank<W>(first, (int)n);
     }
     u64 cyc = 0, done = 0, maxfin = 0, turns = 0, undet = 0, pushed = 0;
+    // step caps: Brent and the alternation detector stop a game unresolved after `cap` turns (the
+    // request's, else the 2^24-turn safety stop); the streak test after more than `scap`
+    const u32 cap = P.R.cap ? P.R.cap : AGE_CAP, scap = P.R.cap ? P.R.cap : STREAK_CAP;
     for (u32 j = 0; j < cnt; j++) {
         St<W> s = (mode == 2) ? canon<W>(q, (int)n, k) : deal<W>(q, (int)n);
         const bool take = (mode == 1) ? top_card_to_p1<W>(q, (int)n) : true;
@@ -1412,7 +1444,7 @@ __global__ __launch_bounds__(256) void k_plain(PlainArgs<W> P) {
                     l++;
                     if (same<W>(s, t)) { cy = true; lam = l; break; }
                     if (l == power) { t = s; power <<= 1; l = 0; }
-                    if (rounds >= AGE_CAP) { und = true; break; }
+                    if (rounds >= cap) { und = true; break; }
                 }
             } else if constexpr (DET == 2) {
                 // the alternation detector, one lane at a time (same schedule as the persistent form)
@@ -1430,14 +1462,14 @@ __global__ __launch_bounds__(256) void k_plain(PlainArgs<W> P) {
                         else if (streak) { t = s; R = (u32)rounds; }
                         streak = false;
                     }
-                    if (rounds >= AGE_CAP) { und = true; break; }
+                    if (rounds >= cap) { und = true; break; }
                 }
             } else {
                 constexpr u32 M = (1u << W) - 1u;
                 bool anch = false;
                 u32 lam_s = 0, since = 0, prevw = 2u;     // prevw 2: no previous turn since the anchor
                 for (;;) {
-                    if (rounds > STREAK_CAP) { und = true; break; }
+                    if (rounds > scap) { und = true; break; }
                     if (!anch) {
                         const u32 ta = (u32)s.a & M, tb = (u32)s.b & M;
                         const u32 l1 = s.sa / (u32)W + 1u;

@@ -11,10 +11,13 @@ gfx950) through its C ABI; nothing is computed on the CPU.
 Deal ranks are lexicographic over the n! orders of cards 0..n-1 (the order war24.py enumerates);
 deck positions 0, 2, 4, ... go to player 1, top card first. --det names the cycle detector:
 floyd (any n) or streak (odd n only, as in the reference). Both are exact, so they give the same
-cycling count, longest finite game and undetermined count; the engine runs the streak test as
-written for --det streak (one deal per lane), and its exact age-synchronous Brent pipeline for
---det floyd. --chunk and --threads shaped the reference's launches; they are accepted and checked
-as the reference checks them.
+cycling count and longest finite game; the engine runs the streak test as written for --det streak
+(one deal per lane), and Brent on its persistent flow kernel for --det floyd (Floyd and Brent find
+the same cycles and game lengths). A game still unresolved after the reference's step cap of 2^22
+turns counts as undetermined and is printed as such (gpu/war24.py:57,96,103,127): the streak test
+keeps the reference's test exactly; Brent stops a game unresolved after 2^22 turns of its own
+(no valid game of n <= 20 comes near it). --chunk and --threads shaped the reference's launches;
+they are accepted and checked as the reference checks them.
 
 Reference behaviour followed (gpu/war24.py): main() :157-207 (arguments :159-167, asserts
 :169-173, files :177-179, checkpoint keys :180-181, :193-195, progress line :197-198, result line
@@ -34,6 +37,7 @@ if _HERE not in sys.path:
 from war_amd import Engine  # noqa: E402
 
 SPAN_DEALS = 1 << 32        # deals per engine call (between checkpoints)
+STEP_CAP = 1 << 22          # the reference's CAP (gpu/war24.py:57)
 
 _engine = None
 
@@ -52,8 +56,11 @@ def files(n, det, lo, hi, tag, where=None):
     return os.path.join(where, "war24_%s.progress" % t), os.path.join(where, "war24_%s.ckpt" % t)
 
 
-def run(n, det="streak", lo=0, hi=None, resume=False, tag="", device=0, where=None, span=None, log=True):
-    """Count deals [lo, hi) of n cards. Returns (cyc, maxfin, deals, undetermined, elapsed_s, line)."""
+def run(n, det="streak", lo=0, hi=None, resume=False, tag="", device=0, where=None, span=None, log=True,
+        step_cap=STEP_CAP):
+    """Count deals [lo, hi) of n cards. Returns (cyc, maxfin, deals, undetermined, elapsed_s, line).
+    step_cap: the turns after which an unresolved game counts as undetermined (the reference's 2^22;
+    a multiple of 32 for --det floyd)."""
     hi = math.factorial(n) if hi is None else hi
     span = SPAN_DEALS if span is None else span
     prog, ckpt = files(n, det, lo, hi, tag, where)
@@ -71,7 +78,7 @@ def run(n, det="streak", lo=0, hi=None, resume=False, tag="", device=0, where=No
     pf = open(prog, "a") if log else None
     while pos < hi:
         stop = min(hi, pos + span)
-        r = eng.run_range(n, "full", pos, stop, kernel=kernel, detector=detector)
+        r = eng.run_range(n, "full", pos, stop, kernel=kernel, detector=detector, step_cap=step_cap)
         st["cyc"] += r["cyc"]
         st["max"] = max(st["max"], r["maxfin"])
         st["deals"] += r["done"]

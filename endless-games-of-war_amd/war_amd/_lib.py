@@ -15,6 +15,8 @@ MODE_FULL, MODE_HALF, MODE_CANON = 0, 1, 2
 KERNEL_PLAIN, KERNEL_PERSISTENT = 0, 1
 KERNELS = {"plain": KERNEL_PLAIN, "persistent": KERNEL_PERSISTENT}
 DET_BRENT, DET_STREAK, DET_ALT, DET_PAIR = 0, 1, 2, 3
+FLAG_UNDET_ERROR = 1
+STREAK_CAP, SAFETY_CAP = 1 << 22, 1 << 24
 DETECTORS = {"brent": DET_BRENT, "streak": DET_STREAK, "alt": DET_ALT, "pair": DET_PAIR}
 
 
@@ -29,7 +31,7 @@ class WarRequest(ctypes.Structure):
                 ("kernel", ctypes.c_int32), ("lo", ctypes.c_uint64), ("hi", ctypes.c_uint64),
                 ("hbins", ctypes.c_int32), ("lbins", ctypes.c_int32), ("dbins", ctypes.c_int32),
                 ("distinct", ctypes.c_int32), ("shape_blk", ctypes.c_int32), ("detector", ctypes.c_int32),
-                ("tuning", WarTuning)]
+                ("tuning", WarTuning), ("step_cap", ctypes.c_uint32), ("flags", ctypes.c_uint32)]
 
 
 _U64P = ctypes.POINTER(ctypes.c_uint64)
@@ -100,6 +102,8 @@ _SIGS = {
     "war_tuning_for": ([ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(WarTuning)], ctypes.c_int),
     "war_autotune": ([ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.POINTER(WarTuning)],
                      ctypes.c_int),
+    "war_autotune_batch": ([ctypes.POINTER(WarRequest), ctypes.c_int32, ctypes.c_double, ctypes.POINTER(WarResult),
+                            ctypes.POINTER(WarTuning)], ctypes.c_int),
     "war_reach": ([ctypes.c_int32, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(WarReachResult)],
                   ctypes.c_int),
     "war_cycle_info_batch": ([ctypes.POINTER(WarPilePosition), ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64,

@@ -47,13 +47,16 @@ def _tuning_dict(t):
 
 
 def make_request(n, mode="full", lo=0, hi=None, rule=0, kernel="persistent", hbins=0, lbins=0, dbins=0,
-                 distinct=False, shape_blk=0, tuning=None, detector="brent"):
+                 distinct=False, shape_blk=0, tuning=None, detector="brent", step_cap=0, undet_error=False):
+    """step_cap: turns after which an unresolved game counts as undetermined (0 = the default);
+    undet_error: fail the request instead if any game is undetermined."""
     mode = MODES.get(mode, mode)
     if hi is None:
         hi = total_positions(n, mode)
     return WarRequest(int(n), int(mode), int(rule), int(KERNELS.get(kernel, kernel)), int(lo), int(hi),
                       int(hbins), int(lbins), int(dbins), int(bool(distinct)), int(shape_blk),
-                      int(DETECTORS.get(detector, detector)), _tuning(tuning))
+                      int(DETECTORS.get(detector, detector)), _tuning(tuning), int(step_cap),
+                      _lib.FLAG_UNDET_ERROR if undet_error else 0)
 
 
 class _ResultBuf:
@@ -122,6 +125,32 @@ class Engine:
     def autotune(self, n, mode="canon", rule=0, budget_ms=20000.0):
         t = WarTuning()
         check(self.lib.war_autotune(int(n), int(MODES.get(mode, mode)), int(rule), float(budget_ms), ctypes.byref(t)))
+        return _tuning_dict(t)
+
+    def autotune_batch(self, queries, budget_ms=20000.0, expect=None):
+        """war_autotune_batch over queries [(n, mode, lo, hi, kw), ...] (run_range's keywords).
+        expect: one result dict per query (keys as run_range returns them) that every candidate
+        schedule must reproduce; None = the plain kernel's counts. Returns the chosen tuning."""
+        reqs = [make_request(n, mode, lo, hi, **kw) for n, mode, lo, hi, kw in queries]
+        arr = (WarRequest * len(reqs))(*reqs)
+        exp = None
+        keep = []
+        if expect is not None:
+            exp = (WarResult * len(reqs))()
+            for i, (q, e) in enumerate(zip(reqs, expect)):
+                r = exp[i]
+                for k in ("cyc", "done", "maxfin", "turns", "oncycle", "shape_nonstd", "shape_unchk"):
+                    setattr(r, k, int(e.get(k, 0)))
+                r.distinct_minkey = int(e.get("distinct_minkey", 0))
+                r.undetermined = int(e.get("undetermined", 0))
+                for name, nb, key in (("hist", q.hbins, "hist"), ("lenhist", q.lbins, "lenhist"), ("dhist", q.dbins, "dhist")):
+                    if nb:
+                        a = (ctypes.c_uint64 * nb)(*[int(v) for v in e[key]])
+                        keep.append(a)
+                        setattr(r, name, ctypes.cast(a, ctypes.POINTER(ctypes.c_uint64)))
+                        setattr(r, {"hist": "hbins", "lenhist": "lbins", "dhist": "dbins"}[name], nb)
+        t = WarTuning()
+        check(self.lib.war_autotune_batch(arr, len(reqs), float(budget_ms), exp, ctypes.byref(t)))
         return _tuning_dict(t)
 
     # ---- synchronous range query

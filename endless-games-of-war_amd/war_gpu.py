@@ -8,9 +8,9 @@ Cards are 0..n-1 (0 lowest). Deck positions 0, 2, 4, ... go to player 1 and 1, 3
 2, top card first. Each turn the higher top card wins, and the winner puts the winning card and
 then the losing card under its own pile (--rule 1 reverses the order).
 
-  python war_gpu.py 12                        a(12)
-  python war_gpu.py 15 --hist 128             a(15) and the cycle lengths
-  python war_gpu.py 16 --canon --out n16c     every 16-card canonical position, resumable
+The command line is the reference's; for instance `war_gpu.py 15 --hist 128` prints a(15) with its
+cycle-length histogram, and `--canon --out PREFIX` scans the canonical positions with a progress
+file that `--resume` continues from.
 
 Reference behaviour followed (gpu/war_gpu.py): run() :272-352 (progress keys :331-338, resume
 asserts :297,306, --half doubling :346-352), CLI :355-398 (asserts :380-390, output :384-398).
@@ -60,13 +60,13 @@ def run(n, half=False, per=256, threads=256, batch_threads=1 << 18, out=None, re
     are accepted for compatibility. det: the cycle test (detector_for)."""
     eng = engine(device)
     detector = detector_for(n, rule, kernel, det)
-    # canon: iterate over the (n-1)! orders of the other cards; each yields n-1 positions
+    # units of `start`: deal ranks, or (canon) orders of the n-1 cards below the top one, n-1 splits each
     total = math.factorial(n - 1) if canon else math.factorial(n)
     unit = max(1, n - 1) if canon else 1               # positions per unit of `start`
     if not canon:
         dbins = 0
-    blk = (n & -n) if (canon and shape) else 0          # 2^v, the largest power of 2 dividing n
-    if blk and os.environ.get("WAR_BLK_OVERRIDE"):      # negative control only: force a wrong block length
+    blk = (n & -n) if (canon and shape) else 0          # the run length Theorem 3 predicts: lowest set bit of n
+    if blk and os.environ.get("WAR_BLK_OVERRIDE"):      # the reference's check that the test can fail
         blk = int(os.environ["WAR_BLK_OVERRIDE"])
     cyc = done = maxfin = 0
     hist = [0] * max(hbins, 1)

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define WAR_ABI_VERSION 2
+#define WAR_ABI_VERSION 3
 
 /* mode */
 #define WAR_MODE_FULL  0   /* deal ranks r in [0, n!), deck positions 0,2,4.. to player 1     */
@@ -77,6 +77,16 @@ typedef struct war_tuning {
                                   The default is 6 where it applies, else 4 for n <= 16, else 5 */
 } war_tuning;
 
+/* war_request.flags */
+#define WAR_FLAG_UNDET_ERROR 1u   /* fail the request (status 21) if any game is undetermined, instead
+                                     of only counting it in war_result.undetermined               */
+
+/* war_request.step_cap = 0: the default cap. The streak test stops a game after 2^22 turns, as the
+ * reference's CAP (gpu/war24.py:57,103); Brent and the checkpoint detectors are exact and end every
+ * valid game, and a game still running after 2^24 turns (a corrupted state) is an error (status 21). */
+#define WAR_STREAK_CAP (1u << 22)
+#define WAR_SAFETY_CAP (1u << 24)
+
 typedef struct war_request {
     int32_t  n;            /* cards: 1..16 (4-bit piles), 17..20 (5-bit piles)                 */
     int32_t  mode;         /* WAR_MODE_*                                                         */
@@ -85,11 +95,19 @@ typedef struct war_request {
     uint64_t lo, hi;       /* half-open range: deal ranks (FULL/HALF) or positions (CANON)      */
     int32_t  hbins;        /* cycle-length histogram bins, last = overflow (0 = off, <= 4096)   */
     int32_t  lbins;        /* ending-game-length histogram bins, last = overflow (0 = off)      */
-    int32_t  dbins;        /* distinct weight bins (CANON, n <= 15; 0 = off, <= 4096)           */
+    int32_t  dbins;        /* distinct weight bins (CANON, n <= 16; 0 = off, <= 4096)           */
     int32_t  distinct;     /* CANON, n <= 15: min-key count + device hash set of cycle keys    */
     int32_t  shape_blk;    /* CANON: Theorem-3 run length to test (0 = off)                      */
     int32_t  detector;     /* WAR_DET_* (0 = Brent)                                              */
     war_tuning tuning;     /* zeros = engine defaults                                            */
+    uint32_t step_cap;     /* 0 = the default cap (above). Otherwise a game that its detector has
+                              not resolved (ended or seen repeating) within step_cap turns counts
+                              as undetermined, with step_cap turns in `turns`; the streak test keeps
+                              the reference's form (still running with more than step_cap turns
+                              played at the top of its loop: gpu/war24.py:103), so step_cap + 1
+                              turns. <= 2^24; the persistent kernel needs a multiple of 32 and the
+                              flow schedule (stage_kind 3)                                       */
+    uint32_t flags;        /* WAR_FLAG_*                                                         */
 } war_request;
 
 typedef struct war_result {
@@ -103,8 +121,9 @@ typedef struct war_result {
     uint64_t distinct_hashset; /* distinct: different cycle-minimum keys met (device hash set)  */
     uint64_t shape_nonstd;     /* shape: cycles not made of winner runs of exactly shape_blk    */
     uint64_t shape_unchk;      /* shape: cycles longer than 512 turns (not checked)             */
-    uint64_t undetermined;     /* games still running at the safety cap (2^24 turns; the streak
-                                  test: 2^22, as gpu/war24.py:58); must be 0                     */
+    uint64_t undetermined;     /* games not resolved within the step cap (war_request.step_cap):
+                                  counted in done, not in cyc, maxfin or the histograms; the
+                                  reference's war24 counter prints it (gpu/war24.py:150-152,203-205) */
     uint64_t *hist;    int32_t hbins;   /* caller-owned, hbins entries (or NULL)               */
     uint64_t *lenhist; int32_t lbins;   /* caller-owned, lbins entries (or NULL)               */
     uint64_t *dhist;   int32_t dbins;   /* caller-owned, dbins entries (or NULL)               */
@@ -163,11 +182,14 @@ int war_run_range(const war_request *req, war_result *res);
 
 /* Several requests of mixed n, mode and rule in shared persistent launches, synchronously: the
  * reference launches war_count / war_canon once per range (gpu/war_gpu.py:309-328); here requests
- * of one pile layout, rule and detector share each launch (one, when their positions fit a batch:
- * 2^32 positions on the flow kernel, 2^28 with cycle walks) and every result equals the request
- * run alone. Persistent kernel, Brent or the alternation detector; count <= the free request slots
- * (248 less the serving loop's). res[i] receives request i's result, device_ms the device time of
- * its group's launches and latency_ms the whole call; *launches (may be null) the launch count. */
+ * of one pile layout, rule, detector and cycle-walk need share each kernel launch (one, when their
+ * positions fit a batch: 2^32 positions on the flow kernel, 2^28 with cycle walks), and all the
+ * kernel launches of the call (up to 16) are branches of ONE captured hipGraph, launched once;
+ * every result equals the request run alone. Persistent kernel, Brent or the alternation detector;
+ * count <= the free request slots (248 less the serving loop's). res[i] receives request i's
+ * result, device_ms the device time of the call's graph and latency_ms the whole call; *launches
+ * (may be null) the number of graph launches (1 unless the call needs more than 16 kernel
+ * launches or a staged schedule, which run one graph per kernel launch). */
 int war_run_batch(const war_request *reqs, int32_t count, war_result *res, int32_t *launches);
 
 /* Reachability of the distinct cycles of canonical positions [lo, hi) of n cards (2 <= n <= 15,
@@ -191,10 +213,24 @@ int war_get_tuning(war_tuning *t);
 /* The tuning a request of (n, mode, rule) with zero tuning fields would run with. */
 int war_tuning_for(int32_t n, int32_t mode, int32_t rule, war_tuning *out);
 
-/* Autotuner: times candidate schedules of the persistent kernel on a 2^26-position slice of
- * (n, mode, rule), checks that every schedule gives identical counts, and keeps the fastest as the
- * default for that workload. budget_ms bounds the search; *best receives the choice. */
+/* Autotuner (replaces the reference's fixed launch shapes, gpu/war_gpu.py:272-280,310-317): times
+ * candidate schedules of the persistent kernel on a 2^26-position slice of (n, mode, rule) with
+ * Brent, and keeps the fastest as the default for that workload. Coordinate descent, two passes,
+ * over the schedule (flow / staged refill / staged lockstep), the tile (64..8192 positions per grab;
+ * 64 = one position per lane of a wave is the smallest), waves per CU (1..8, 12..32), the first
+ * stage boundary (staged schedules) and, for n = 17, the pile layout. Every candidate must
+ * reproduce the plain kernel's counts on the slice exactly (an independent code path), else the
+ * call fails. budget_ms bounds the search; *best receives the choice. */
 int war_autotune(int32_t n, int32_t mode, int32_t rule, double budget_ms, war_tuning *best);
+
+/* The same search over a given set of requests (count >= 1), run as one war_run_batch (count > 1)
+ * or one war_run_range (count = 1) per candidate with the candidate as every request's tuning.
+ * expect (may be null): the counts each request must reproduce (cyc, done, maxfin, undetermined,
+ * turns, oncycle / minkey / shape where asked, and every histogram the request asks for); null =
+ * the plain kernel's counts. A single request's choice is kept as the default for its
+ * (n, mode, rule, detector); a batch's is returned only (pass it as the requests' tuning). */
+int war_autotune_batch(const war_request *reqs, int32_t count, double budget_ms, const war_result *expect,
+                       war_tuning *best);
 
 /* Continuous-batching request loop. Requests submitted while others run are admitted into free
  * slots (bounded by slots and by free device memory) and share the persistent launches.

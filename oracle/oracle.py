@@ -47,6 +47,10 @@ def lib():
         L.orc_run_det.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64,
                                   ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(OrcRes)]
         L.orc_run_det.restype = ctypes.c_int
+        L.orc_run_cap.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64,
+                                  ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
+                                  ctypes.POINTER(OrcRes)]
+        L.orc_run_cap.restype = ctypes.c_int
         L.orc_last_error.restype = ctypes.c_char_p
         L.orc_total.argtypes = [ctypes.c_int, ctypes.c_int]
         L.orc_total.restype = ctypes.c_uint64
@@ -64,11 +68,13 @@ def total(n, mode="full"):
 
 
 def run(n, mode="full", lo=0, hi=None, rule=0, hbins=0, lbins=0, dbins=0, distinct=False, shape_blk=0, threads=0,
-        detector="brent"):
+        detector="brent", step_cap=0):
     """Play every position of [lo, hi) on the CPU; returns a dict shaped like the engine's result.
     detector: "brent" (every output), "streak" (odd n, rule 0, no cycle-length outputs), "alt"
     (Brent with alternation checkpoints: odd n, rule 0, every output) or "pair" (Brent with pair-run
-    checkpoints: n = 2 mod 4, rule 0, every output)."""
+    checkpoints: n = 2 mod 4, rule 0, every output). step_cap: a game unresolved after step_cap turns
+    counts as undetermined (0: none, or the reference's 2^22 for the streak test, whose test is "more
+    than step_cap turns played" at the top of its loop)."""
     L = lib()
     m = _MODES.get(mode, mode)
     if hi is None:
@@ -81,8 +87,8 @@ def run(n, mode="full", lo=0, hi=None, rule=0, hbins=0, lbins=0, dbins=0, distin
     r.lenhist = ctypes.cast(Ln, _U64P); r.lbins = lbins
     r.dhist = ctypes.cast(D, _U64P); r.dbins = dbins
     det = {"brent": 0, "streak": 1, "alt": 2, "pair": 3}[detector]
-    rc = L.orc_run_det(int(n), m, int(rule), int(lo), int(hi), int(threads), int(bool(distinct)), int(shape_blk),
-                       det, ctypes.byref(r))
+    rc = L.orc_run_cap(int(n), m, int(rule), int(lo), int(hi), int(threads), int(bool(distinct)), int(shape_blk),
+                       det, int(step_cap), ctypes.byref(r))
     if rc:
         raise ValueError("oracle: %s" % L.orc_last_error().decode())
     return {"cyc": r.cyc, "done": r.done, "maxfin": r.maxfin, "turns": r.turns, "oncycle": r.oncycle,

@@ -59,7 +59,7 @@ struct orc_res {
     u64* hist;  int hbins;    // lambda histogram of endless positions (last bin = overflow)
     u64* lenhist; int lbins;  // histogram of ending-game lengths (last bin = overflow)
     u64* dhist; int dbins;    // distinct weights (canon + distinct)
-    u64 undet;                // streak test: games still undecided after 2^22 turns
+    u64 undet;                // games not resolved within the step cap (orc_run_cap)
 };
 }
 
@@ -100,33 +100,37 @@ template <int W> static inline int step(St<W>& s, int rule) {
     return 0;
 }
 
-// Brent (war_gpu.py:77-96). Returns true if the game never ends; rounds = turns played until
-// the end or until the repeat is seen; lam = cycle length; meet = state at detection.
-template <int W> static inline bool brent(St<W> s, int rule, u64& rounds, u64& lam_out, St<W>* meet) {
-    if (s.la == 0 || s.lb == 0) { rounds = 0; return false; }
+// Brent (war_gpu.py:77-96). Returns 1 if the game never ends, 0 if it ends; rounds = turns played
+// until the end or until the repeat is seen; lam = cycle length; meet = state at detection. With a
+// step cap (not in the reference, whose Brent has none): 2 = still unresolved after cap turns
+// (rounds = cap).
+template <int W> static inline int brent(St<W> s, int rule, u64& rounds, u64& lam_out, St<W>* meet, u64 cap = 0) {
+    if (s.la == 0 || s.lb == 0) { rounds = 0; return 0; }
     St<W> t = s;
     u64 power = 1, lam = 0, r = 0;
     for (;;) {
         step<W>(s, rule);
         r++;
-        if (s.la == 0 || s.lb == 0) { rounds = r; return false; }
+        if (s.la == 0 || s.lb == 0) { rounds = r; return 0; }
         lam++;
-        if (same<W>(s, t)) { rounds = r; lam_out = lam; if (meet) *meet = s; return true; }
+        if (same<W>(s, t)) { rounds = r; lam_out = lam; if (meet) *meet = s; return 1; }
         if (lam == power) { t = s; power <<= 1; lam = 0; }
+        if (cap && r >= cap) { rounds = r; return 2; }
     }
 }
 
 // Streak test for odd n (gpu/war24.py:98-125, Spivey Thm 6): anchor the state at a turn whose
 // winner-to-be holds an even number ws of cards while the other holds ls >= 2; if the players then
 // alternate for lcm(n+1, ws, ls-1) turns and the state equals the anchor, the game never ends.
-// A repeated winner drops the anchor. Returns 0 = ends, 1 = endless, 2 = undetermined (more than
-// 2^22 turns, as the reference's CAP); steps = turns played (the game length for a game that ends).
+// A repeated winner drops the anchor. Returns 0 = ends, 1 = endless, 2 = undetermined (still running
+// with more than cap turns played at the top of the loop: the reference's CAP = 2^22,
+// gpu/war24.py:57,103); steps = turns played (the game length for a game that ends).
 static inline int gcd_i(int a, int b) { while (b) { int t = a % b; a = b; b = t; } return a; }
 static inline int lcm_i(int a, int b) { return a / gcd_i(a, b) * b; }
-template <int W> static inline int streak(St<W> s, int n, u64& steps) {
+template <int W> static inline int streak(St<W> s, int n, u64& steps, u64 cap = 0) {
     typedef typename Lay<W>::T T;
     const T M = (T)((1u << W) - 1u);
-    const u64 CAP = 1ULL << 22;
+    const u64 CAP = cap ? cap : (1ULL << 22);
     steps = 0;
     bool anch = false; int lam = 0, since = 0, prevw = -1;
     St<W> a = s;
@@ -167,8 +171,8 @@ template <int W> static inline int streak(St<W> s, int n, u64& steps) {
 // cycle length, so every output is exact; the turns to the repeat are fewer than Brent's (about
 // 30 % at n = 13).
 constexpr u64 ALT_K = 8, ALT_L0 = 128;
-template <int W> static inline bool alt_brent(St<W> s, u64& rounds, u64& lam_out, St<W>* meet) {
-    if (s.la == 0 || s.lb == 0) { rounds = 0; return false; }
+template <int W> static inline int alt_brent(St<W> s, u64& rounds, u64& lam_out, St<W>* meet, u64 cap = 0) {
+    if (s.la == 0 || s.lb == 0) { rounds = 0; return 0; }
     St<W> t = s;
     u64 R = 0, age = 0, L = ALT_L0;
     bool streak = false;
@@ -176,8 +180,8 @@ template <int W> static inline bool alt_brent(St<W> s, u64& rounds, u64& lam_out
     for (;;) {
         const int w = step<W>(s, 0);
         age++;
-        if (s.la == 0 || s.lb == 0) { rounds = age; return false; }
-        if (same<W>(s, t)) { rounds = age; lam_out = age - R; if (meet) *meet = s; return true; }
+        if (s.la == 0 || s.lb == 0) { rounds = age; return 0; }
+        if (same<W>(s, t)) { rounds = age; lam_out = age - R; if (meet) *meet = s; return 1; }
         if (w == prevw && (age & 1)) streak = true;
         prevw = w;
         if (age % ALT_K == 0) {
@@ -185,6 +189,7 @@ template <int W> static inline bool alt_brent(St<W> s, u64& rounds, u64& lam_out
             else if (streak) { t = s; R = age; }
             streak = false;
         }
+        if (cap && age >= cap) { rounds = age; return 2; }
     }
 }
 
@@ -200,8 +205,8 @@ template <int W> static inline bool alt_brent(St<W> s, u64& rounds, u64& lam_out
 // declared on an exact state match and the first match after a jump is the cycle length, so every
 // output is exact; the turns to the repeat are fewer than Brent's (about 8 % at n = 14 canonical).
 constexpr u64 PAIR_P = 128;
-template <int W> static inline bool pair_brent(St<W> s, u64& rounds, u64& lam_out, St<W>* meet) {
-    if (s.la == 0 || s.lb == 0) { rounds = 0; return false; }
+template <int W> static inline int pair_brent(St<W> s, u64& rounds, u64& lam_out, St<W>* meet, u64 cap = 0) {
+    if (s.la == 0 || s.lb == 0) { rounds = 0; return 0; }
     St<W> t = s;
     u64 R = 0, age = 0, L = ALT_L0;
     bool viol = false;
@@ -209,8 +214,8 @@ template <int W> static inline bool pair_brent(St<W> s, u64& rounds, u64& lam_ou
     for (;;) {
         const int w = step<W>(s, 0);
         age++;
-        if (s.la == 0 || s.lb == 0) { rounds = age; return false; }
-        if (same<W>(s, t)) { rounds = age; lam_out = age - R; if (meet) *meet = s; return true; }
+        if (s.la == 0 || s.lb == 0) { rounds = age; return 0; }
+        if (same<W>(s, t)) { rounds = age; lam_out = age - R; if (meet) *meet = s; return 1; }
         const int eq = w == prevw;
         if (eq == preveq) viol = true;
         prevw = w;
@@ -220,6 +225,7 @@ template <int W> static inline bool pair_brent(St<W> s, u64& rounds, u64& lam_ou
             else if (viol && age <= PAIR_P) { t = s; R = age; }
             viol = false;
         }
+        if (cap && age >= cap) { rounds = age; return 2; }
     }
 }
 
@@ -299,17 +305,18 @@ struct Acc {
     Acc(int hb, int lb, int db) : hist(hb > 0 ? hb : 0, 0), lenhist(lb > 0 ? lb : 0, 0), dhist(db > 0 ? db : 0, 0) {}
 };
 
-struct Cfg { int n, mode, rule, hbins, lbins, dbins, distinct, blk, det; };
+struct Cfg { int n, mode, rule, hbins, lbins, dbins, distinct, blk, det; u64 cap; };
 
 template <int W> static void play_cycle(const St<W>& s, const Cfg& c, Acc& A, u64 lam, const St<W>& meet);
 
 template <int W> static void play(const St<W>& s, const Cfg& c, Acc& A) {
     u64 rounds = 0, lam = 0; St<W> meet;
     if (c.det == 2 || c.det == 3) {   // alternation / pair-run checkpoints: every output, own turn count
-        const bool cy = c.det == 2 ? alt_brent<W>(s, rounds, lam, &meet) : pair_brent<W>(s, rounds, lam, &meet);
+        const int r = c.det == 2 ? alt_brent<W>(s, rounds, lam, &meet, c.cap) : pair_brent<W>(s, rounds, lam, &meet, c.cap);
         A.done++;
         A.turns += rounds;
-        if (!cy) {
+        if (r == 2) { A.undet++; return; }
+        if (r == 0) {
             if (rounds > A.maxfin) A.maxfin = rounds;
             if (c.lbins) A.lenhist[rounds < (u64)(c.lbins - 1) ? rounds : (u64)(c.lbins - 1)]++;
             return;
@@ -318,7 +325,7 @@ template <int W> static void play(const St<W>& s, const Cfg& c, Acc& A) {
         return;
     }
     if (c.det == 1) {      // streak detector: counts, game lengths and turns; no cycle lengths
-        int r = streak<W>(s, c.n, rounds);
+        int r = streak<W>(s, c.n, rounds, c.cap);
         A.done++;
         A.turns += rounds;
         if (r == 1) A.cyc++;
@@ -329,9 +336,10 @@ template <int W> static void play(const St<W>& s, const Cfg& c, Acc& A) {
         }
         return;
     }
-    bool cyc = brent<W>(s, c.rule, rounds, lam, &meet);
+    const int cyc = brent<W>(s, c.rule, rounds, lam, &meet, c.cap);
     A.done++;
     A.turns += rounds;
+    if (cyc == 2) { A.undet++; return; }
     if (!cyc) {
         if (rounds > A.maxfin) A.maxfin = rounds;
         if (c.lbins) A.lenhist[rounds < (u64)(c.lbins - 1) ? rounds : (u64)(c.lbins - 1)]++;
@@ -415,8 +423,11 @@ extern "C" u64 orc_total(int n, int mode) { return range_total(n, mode); }
 // hist / dhist / distinct / shape), 2 = Brent with alternation checkpoints (odd n, rule 0; every
 // output, its own turn count), 3 = Brent with pair-run checkpoints (n = 2 mod 4, rule 0; every
 // output, its own turn count).
-extern "C" int orc_run_det(int n, int mode, int rule, u64 lo, u64 hi, int threads,
-                           int distinct, int shape_blk, int det, orc_res* res) {
+// cap: the step cap (0 = none for Brent and the checkpoint detectors, the reference's 2^22 for the
+// streak test): a game unresolved after cap turns (the streak test: with more than cap turns played
+// at the top of its loop) counts as undetermined, its turns in `turns`.
+extern "C" int orc_run_cap(int n, int mode, int rule, u64 lo, u64 hi, int threads,
+                           int distinct, int shape_blk, int det, u64 cap, orc_res* res) {
     if (!res) { g_err = "null result"; return 1; }
     if (n < 1 || n > 20) { g_err = "n must be in [1, 20]"; return 2; }
     if (mode < 0 || mode > 2) { g_err = "bad mode"; return 3; }
@@ -424,7 +435,8 @@ extern "C" int orc_run_det(int n, int mode, int rule, u64 lo, u64 hi, int thread
     u64 total = range_total(n, mode);
     if (lo > hi || hi > total) { g_err = "range outside [0, total]"; return 5; }
     if (mode == ORC_HALF && (n & 1)) { g_err = "half mode needs even n"; return 6; }
-    if ((distinct || res->dbins) && (mode != ORC_CANON || n > 15)) { g_err = "distinct needs canon mode and n <= 15"; return 7; }
+    if (distinct && (mode != ORC_CANON || n > 15)) { g_err = "distinct needs canon mode and n <= 15"; return 7; }
+    if (res->dbins && (mode != ORC_CANON || n > 16)) { g_err = "dbins needs canon mode and n <= 16"; return 7; }
     if (shape_blk && mode != ORC_CANON) { g_err = "shape needs canon mode"; return 8; }
     if (det < 0 || det > 3) {
         g_err = "det must be 0 (Brent), 1 (streak), 2 (alternation checkpoints) or 3 (pair-run checkpoints)"; return 9;
@@ -440,7 +452,7 @@ extern "C" int orc_run_det(int n, int mode, int rule, u64 lo, u64 hi, int thread
         g_err = "the streak test needs odd n, rule 0 and no cycle-length outputs"; return 10;
     }
     Cfg c{n, mode, rule, res->hist ? res->hbins : 0, res->lenhist ? res->lbins : 0,
-          res->dhist ? res->dbins : 0, distinct, shape_blk, det};
+          res->dhist ? res->dbins : 0, distinct, shape_blk, det, cap};
     if (threads > 0) omp_set_num_threads(threads);
     const u64 CH = (mode == ORC_CANON) ? (u64)std::max(1, n - 1) * 512 : 4096;
     u64 nchunks = (hi - lo + CH - 1) / CH;   // 0 for an empty range
@@ -473,6 +485,11 @@ extern "C" int orc_run_det(int n, int mode, int rule, u64 lo, u64 hi, int thread
     for (int i = 0; i < c.lbins; i++) res->lenhist[i] = G.lenhist[(size_t)i];
     for (int i = 0; i < c.dbins; i++) res->dhist[i] = G.dhist[(size_t)i];
     return 0;
+}
+
+extern "C" int orc_run_det(int n, int mode, int rule, u64 lo, u64 hi, int threads,
+                           int distinct, int shape_blk, int det, orc_res* res) {
+    return orc_run_cap(n, mode, rule, lo, hi, threads, distinct, shape_blk, det, 0, res);
 }
 
 extern "C" int orc_run(int n, int mode, int rule, u64 lo, u64 hi, int threads,

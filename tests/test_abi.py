@@ -19,7 +19,7 @@ def test_library_exports_header():
     assert len(names) >= 14
     for nm in names:
         assert hasattr(lib, nm), nm
-    assert lib.war_abi_version() == 2
+    assert lib.war_abi_version() == 3
 
 
 def test_no_python_fallback(monkeypatch, tmp_path):
@@ -31,7 +31,7 @@ STRUCTS = {
     "war_tuning": (_lib.WarTuning, ["waves_per_cu", "ranks_per_refill", "brent_cap", "plain_per", "stage_kind",
                                     "pile_layout"]),
     "war_request": (_lib.WarRequest, ["n", "mode", "rule", "kernel", "lo", "hi", "hbins", "lbins", "dbins",
-                                      "distinct", "shape_blk", "detector", "tuning"]),
+                                      "distinct", "shape_blk", "detector", "tuning", "step_cap", "flags"]),
     "war_reach_result": (_lib.WarReachResult, ["classes", "reached", "not_reached", "undetermined", "nolen",
                                                "nbins", "device_ms"]),
     "war_pile_position": (_lib.WarPilePosition, ["n1", "n2", "p1", "p2"]),

@@ -107,3 +107,30 @@ def test_mismatch_exits_1(monkeypatch, tmp_path, capsys):
         war24.main(["9", "--count", "100"])
     assert ex.value.code == 1
     assert capsys.readouterr().out.strip().endswith("ERROR: deal count mismatch")
+
+
+class UndetEngine(FakeEngine):
+    """Reports a few undetermined deals per call and records the step cap it was asked for."""
+
+    def run_range(self, n, mode, lo, hi, kernel="persistent", detector="brent", **kw):
+        r = super().run_range(n, mode, lo, hi, kernel, detector)
+        self.caps = getattr(self, "caps", []) + [kw.get("step_cap")]
+        r["undetermined"] = 7
+        return r
+
+
+def test_undetermined_is_counted_and_printed(monkeypatch, tmp_path, capsys):
+    """Games unresolved within the reference's step cap (2^22, gpu/war24.py:57) are a count, summed
+    over the calls into the checkpoint's `und` and printed as undetermined=N (:127, :203-205)."""
+    e = UndetEngine()
+    monkeypatch.setattr(war24, "_engine", e)
+    monkeypatch.setattr(war24, "_HERE", str(tmp_path))
+    monkeypatch.setattr(war24, "SPAN_DEALS", 1000)
+    war24.main(["17", "--start", "0", "--count", "3500", "--det", "floyd"])
+    line = capsys.readouterr().out.strip().splitlines()[-1]
+    m = _line_re().match(line)
+    assert m, line
+    assert int(m.group(9)) == 7 * 4                      # four calls of at most 1000 deals
+    assert set(e.caps) == {1 << 22}
+    st = json.loads((tmp_path / "war24_n17_floyd_0_3500.ckpt").read_text())
+    assert st["und"] == 28

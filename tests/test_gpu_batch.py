@@ -1,6 +1,6 @@
-"""war_run_batch: requests of mixed n, mode and rule in shared persistent launches. Each result must
-equal the oracle's for that request (tolerance 0), and a group whose positions fit one batch must
-take one launch.
+"""war_run_batch: requests of mixed n, mode and rule in shared persistent launches, every kernel
+launch of the call a branch of one captured graph, launched once. Each result must equal the
+oracle's for that request (tolerance 0).
 
 The mixed-n workload: seeded numpy PCG64 (seed 20261203), 240 requests (the engine has 248 request
 slots), n uniform over 5..15, mode full or canonical, with distinct + dhist for a third of the
@@ -39,11 +39,12 @@ def mixed_queries(seed=20261203, count=240, top=6.5):
     return out
 
 
-def test_mixed_n_one_launch_per_group_equals_oracle(eng, orc):
+def test_mixed_n_one_graph_launch_equals_oracle(eng, orc):
     qs = mixed_queries()
     assert sum(hi - lo for _, _, lo, hi, _ in qs) < 1 << 28       # fits one batch with cycle walks
     got, launches = eng.run_batch(qs)
-    assert launches == 3              # without cycle walks: deal blocks, the rest; with walks
+    # three kernel launches (without cycle walks: deal blocks, the rest; with walks), one graph
+    assert launches == 1
     for (n, mode, lo, hi, kw), g in zip(qs, got):
         want = orc.run(n, mode, lo, hi, **kw)
         for k in KEYS:
@@ -71,13 +72,26 @@ def test_groups_and_packing_equal_single_requests(eng):
     # layout 4, rule 0, Brent, no walks (n14 full, n9): the deal blocks, 5e9 - 8 + 362880
     # positions, two launches of at most 2^32, and the 8 deals before n14's first block boundary,
     # one; the same with walks (the two distinct n13 ranges): 3e8 positions, two launches of at
-    # most 2^28; layout 4 rule 0 alternation: blocks 1, the ends of the range 1; layout 4 rule 1
-    # (canonical): 1; layout 6 (n = 17): 1
-    assert launches == 3 + 2 + 2 + 1 + 1
+    # most 2^28, one after the other on one branch; layout 4 rule 0 alternation: blocks 1, the ends
+    # of the range 1; layout 4 rule 1 (canonical): 1; layout 6 (n = 17): 1. Nine kernel launches,
+    # one graph launch.
+    assert launches == 1
     for (n, mode, lo, hi, kw), g in zip(qs, got):
         alone = eng.run_range(n, mode, lo, hi, **kw)
         for k in KEYS:
             assert g[k] == alone[k], (n, mode, lo, hi, k)
+
+
+def test_staged_batch_runs_launch_by_launch(eng, orc):
+    """A staged schedule (the queue buffers of a layout are shared) runs each kernel launch as its own
+    graph; the counts are the same."""
+    qs = [(10, "full", 0, 3628800, {"hbins": 64, "tuning": {"stage_kind": "lockstep"}}),
+          (11, "canon", 0, 10**6, {"hbins": 64, "tuning": {"stage_kind": "lockstep"}, "rule": 1})]
+    got, launches = eng.run_batch(qs)
+    assert launches == 2
+    for (n, m, lo, hi, kw), g in zip(qs, got):
+        w = orc.run(n, m, lo, hi, hbins=64, rule=kw.get("rule", 0))
+        assert (g["cyc"], g["turns"], g["hist"]) == (w["cyc"], w["turns"], w["hist"])
 
 
 def test_batch_refusals(eng):

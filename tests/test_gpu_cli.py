@@ -56,3 +56,38 @@ def test_war24_cli_matches_reference_log(eng, goldens, capsys, monkeypatch, tmp_
     war24.main(args)
     got = capsys.readouterr().out.strip().splitlines()[-1]
     assert TIME24.sub("time=<t>s", got) == TIME24.sub("time=<t>s", c["line"]), (c["source"], got)
+
+
+def test_resume_on_the_engine(eng, goldens, tmp_path, monkeypatch):
+    """Resume and progress on the real engine (gpu/war_gpu.py:294-308, 329-341): the n = 13 canonical
+    scan with distinct weights is interrupted after a few engine calls (the .progress file written
+    after every call), resumed from its .progress file, and its totals equal the published ones and
+    an uninterrupted run's."""
+    monkeypatch.setattr(war_gpu, "SPAN_POSITIONS", 1 << 29)        # 11 calls of 12 * 2^29 / 12 positions
+    clock = iter(range(0, 10 ** 9, 11))
+    monkeypatch.setattr(war_gpu.time, "time", lambda: next(clock))  # progress written after every call
+    out = str(tmp_path / "n13c")
+    monkeypatch.setattr(war_gpu, "_engine", eng)
+    calls = []
+    real = eng.run_range
+
+    def killed_after_3(*a, **kw):
+        calls.append(a)
+        if len(calls) > 3:
+            raise KeyboardInterrupt("killed")
+        return real(*a, **kw)
+    monkeypatch.setattr(eng, "run_range", killed_after_3)
+    with pytest.raises(KeyboardInterrupt):
+        war_gpu.run(13, canon=True, out=out, quiet=True, hbins=128, dbins=4096)
+    import json
+    with open(out + ".progress") as f:
+        mid = json.load(f)
+    assert 0 < mid["next_start"] < mid["total"] and mid["cyc"] > 0
+    monkeypatch.setattr(eng, "run_range", real)
+    c, d, mx, el, h, sh, dd = war_gpu.run(13, canon=True, out=out, resume=True, quiet=True, hbins=128, dbins=4096)
+    g = goldens["canon"]["13"]
+    assert (c, d, mx) == (g["cycling"], g["positions"], goldens["canon_gpu"]["13"]["longest"])
+    assert sum(v // k for k, v in dd.items()) == g["cycles"]
+    assert sum(dd.values()) == g["on_cycle"]
+    whole = war_gpu.run(13, canon=True, quiet=True, hbins=128, dbins=4096)
+    assert (c, d, mx, h, dd) == (whole[0], whole[1], whole[2], whole[4], whole[6])

@@ -1,5 +1,7 @@
 """The continuous-batching loop: requests submitted while others run share the persistent launches,
-and every result equals the same request run alone (tolerance 0)."""
+every served result equals the oracle's for the same range, rule and flags, and the same request run
+alone (tolerance 0)."""
+import math
 import random
 
 import pytest
@@ -94,3 +96,53 @@ def test_mixed_detectors_on_both_pipelines(eng, stage_kind):
     for q, a, m in zip(qs, alone, mixed):
         for k in KEYS:
             assert a[k] == m[k], (stage_kind, q, k, a[k], m[k])
+
+
+def oracle_queries(seed=5):
+    """Every n of 7..17 (and 18), each mode that applies, both rules, distinct + dbins + shape on
+    canonical n <= 13, dbins on canonical n = 16, the alternation detector on odd n, the held-card
+    (n = 17, default) and 5-bit (n = 17 forced, n = 18) layouts, a few step caps; lengths
+    log-uniform in [10^2, 10^7] so the oracle checks them all in seconds."""
+    rng = random.Random(seed)
+    out = []
+    for n in range(7, 19):
+        modes = ["full", "canon"] + (["half"] if n % 2 == 0 and n < 17 else [])
+        for mode in modes:
+            for rule in (0, 1):
+                total = math.factorial(n - 1) * (n - 1) if mode == "canon" else math.factorial(n)
+                ln = min(total, int(10 ** rng.uniform(2, 7)))
+                lo = rng.randrange(0, total - ln + 1)
+                kw = {"hbins": 128, "lbins": 256, "rule": rule}
+                if mode == "canon" and n <= 13 and rule == 0:
+                    kw.update(distinct=True, dbins=256, shape_blk=n & -n)
+                if mode == "canon" and n == 16:
+                    kw.update(dbins=64)
+                if n % 2 == 1 and rule == 0 and mode != "canon":
+                    kw["detector"] = "alt"
+                if n == 17 and mode == "full" and rule == 1:
+                    kw["tuning"] = {"pile_layout": 5}
+                if rng.random() < 0.15 and not kw.get("distinct"):
+                    kw["step_cap"] = 32 * rng.randint(1, 4)
+                out.append((n, mode, lo, lo + ln, kw))
+    return out
+
+
+ORC_KEYS = ("cyc", "done", "maxfin", "turns", "undetermined", "hist", "lenhist", "oncycle", "distinct_minkey",
+            "dhist", "shape_nonstd", "shape_unchk")
+
+
+def test_served_results_equal_oracle(eng, orc):
+    qs = oracle_queries()
+    eng.serve_start(max_slots=12, step_positions=1 << 22)
+    try:
+        ids = [eng.submit(n, mode, lo, hi, **kw) for n, mode, lo, hi, kw in qs]
+        served = [eng.wait(i) for i in ids]
+    finally:
+        eng.serve_stop()
+    for (n, mode, lo, hi, kw), g in zip(qs, served):
+        okw = {k: v for k, v in kw.items() if k != "tuning"}
+        want = orc.run(n, mode, lo, hi, **okw)
+        for k in ORC_KEYS:
+            assert g[k] == want[k], (n, mode, lo, hi, kw, k, g[k], want[k])
+        if kw.get("distinct"):
+            assert g["distinct_hashset"] == want["distinct_keys"], (n, mode, lo, hi)
