@@ -22,6 +22,7 @@
 #include <string>
 #include <thread>
 #include <tuple>
+#include <type_traits>
 #include <vector>
 
 #include "war_engine.h"
@@ -112,6 +113,24 @@ constexpr int FLOW_MIN_FREE = 1;        // flow kernel: refill at a checkpoint o
 constexpr u64 FLOW_BATCH = 1ull << 32;  // flow kernel: positions per batch
 constexpr u32 PLAIN_CYC_TILE = 1024;    // home-tile size of the plain kernel's cycle records
 
+// ---- the entry point that allocates nothing (war_run_range_noalloc): its device memory is this
+// file-scope array, one copy per GPU, held until the process ends. Host code never passes the
+// array's name to a kernel or a copy (on the host it is a stand-in, not a device address): war_init
+// resolves the device address once per device with hipGetSymbolAddress and keeps it (Engine::na).
+// Layout, in u64 words: the pipe counters, the job header and its one job, the request descriptor,
+// then the request's counters and its two histograms (at most NA_BINS bins each).
+constexpr u32 NA_BINS = 4096;
+constexpr size_t NA_PIPE = 0, NA_HDR = 64, NA_JOB = NA_HDR + JOBS_OFF / 8, NA_REQ = 160, NA_CNT = 192,
+                 NA_HIST = NA_CNT + C_N, NA_LEN = NA_HIST + NA_BINS, NA_WORDS = NA_LEN + NA_BINS;
+static_assert(sizeof(Pipe) <= 8 * NA_HDR && sizeof(JobHdr) <= JOBS_OFF && NA_JOB + 4 <= NA_REQ &&
+              sizeof(ReqDev) <= 8 * (NA_CNT - NA_REQ), "scratch layout");
+static __device__ __align__(256) u64 scratch[NA_WORDS];
+
+// Fills n words with v (a kernel, not hipMemsetAsync, so a caller may capture the calls).
+__global__ void k_fill(u64 *p, u64 v, u32 n) {
+    for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
 struct GraphKey {
     int W, rule, kernel, wpc, per, cap0, walk, multi, hb, lb, kind, det;   // det: WAR_DET_* of the persistent path
     int blk;                                                                // flow kernel over whole blocks of 24 deals
@@ -146,6 +165,10 @@ struct Engine {
     hipStream_t bst[MAXB] = {};
     hipEvent_t bfork = nullptr, bjoin[MAXB] = {};
     std::map<std::vector<int>, hipGraphExec_t> bgraphs;
+    // war_run_range_noalloc: the device address of `scratch` on this device, and pinned staging
+    // (allocated at init) for its job header, request descriptor and results
+    u64 *na = nullptr;
+    u64 *na_host = nullptr;
     std::vector<int> free_slots;
     uint32_t slot_cap[MAXSLOTS] = {};  // each slot's explicit step cap (ReqDev::cap), for JobHdr::capck
     std::mutex mu;                   // serialises all GPU work of the engine
@@ -503,8 +526,10 @@ void launch_flow(int det, int blocks, size_t lds, hipStream_t st, const StageArg
 // Enqueue one batch's kernels on stream st (inside a capture): k_setup, then the flow kernel or the
 // staged pipeline, then the cycle walks. pipe / hdr / jobs: the batch's counters and job list.
 template <int W, int RULE>
-void record_launch(Engine &E, const GraphKey &key, hipStream_t st, Pipe *pipe, JobHdr *hdr, Job *jobsd) {
+void record_launch(Engine &E, const GraphKey &key, hipStream_t st, Pipe *pipe, JobHdr *hdr, Job *jobsd,
+                   const ReqDev *reqs = nullptr) {
     typedef typename Lay<W>::T T;
+    if (!reqs) reqs = E.reqs;
     QueueBufs &Q = qbufs<W>(E);
     const int blocks = std::max(1, E.cus * key.wpc / 4);
     const size_t lds = (size_t)4 * (key.hb + key.lb) * sizeof(u32);
@@ -514,7 +539,7 @@ void record_launch(Engine &E, const GraphKey &key, hipStream_t st, Pipe *pipe, J
                        key.walk ? Q.ccnt : (u32 *)nullptr);
     if (key.kind == 3) {                // the flow kernel: one launch plays the whole batch
         StageArgs<W> A{};
-        A.pipe = pipe; A.hdr = hdr; A.jobs = jobsd; A.reqs = E.reqs;
+        A.pipe = pipe; A.hdr = hdr; A.jobs = jobsd; A.reqs = reqs;
         A.per = (u32)key.per; A.tile_pos = 64u * (u32)key.per;
         A.hb = (u32)key.hb; A.lb = (u32)key.lb;
         A.flow_min = (u32)key.cap0;
@@ -538,7 +563,7 @@ void record_launch(Engine &E, const GraphKey &key, hipStream_t st, Pipe *pipe, J
     u32 a0 = 0;
     for (size_t s = 0; s < caps.size(); s++) {
         StageArgs<W> A{};
-        A.pipe = pipe; A.hdr = hdr; A.jobs = jobsd; A.reqs = E.reqs;
+        A.pipe = pipe; A.hdr = hdr; A.jobs = jobsd; A.reqs = reqs;
         const int in = (int)((s + 1) & 1), out = (int)(s & 1);
         A.qin.a = (T *)Q.qa[in]; A.qin.b = (T *)Q.qb[in]; A.qin.req = Q.qreq[in]; A.qin.pos = Q.qpos[in];
         A.qin.home = Q.qhome[in]; A.qin.cnt = Q.qcnt[in];
@@ -563,21 +588,22 @@ void record_launch(Engine &E, const GraphKey &key, hipStream_t st, Pipe *pipe, J
     if constexpr (W != 6) {
         if (key.walk) {
             WalkArgs<W> WA{};
-            WA.pipe = pipe; WA.hdr = hdr; WA.reqs = E.reqs; WA.cq = cyc_queue<W>(E); WA.tile_pos = 64u * (u32)key.per;
+            WA.pipe = pipe; WA.hdr = hdr; WA.reqs = reqs; WA.cq = cyc_queue<W>(E); WA.tile_pos = 64u * (u32)key.per;
             WA.stack = E.reach_stack;
             hipLaunchKernelGGL((k_walk<W, RULE>), dim3(blocks), dim3(256), 0, st, WA);
         }
     }
 }
 
-void record_launch_any(Engine &E, const GraphKey &key, hipStream_t st, Pipe *pipe, JobHdr *hdr, Job *jobsd) {
+void record_launch_any(Engine &E, const GraphKey &key, hipStream_t st, Pipe *pipe, JobHdr *hdr, Job *jobsd,
+                       const ReqDev *reqs = nullptr) {
     switch (key.W * 2 + key.rule) {
-        case 8: record_launch<4, 0>(E, key, st, pipe, hdr, jobsd); break;
-        case 9: record_launch<4, 1>(E, key, st, pipe, hdr, jobsd); break;
-        case 10: record_launch<5, 0>(E, key, st, pipe, hdr, jobsd); break;
-        case 11: record_launch<5, 1>(E, key, st, pipe, hdr, jobsd); break;
-        case 12: record_launch<6, 0>(E, key, st, pipe, hdr, jobsd); break;
-        default: record_launch<6, 1>(E, key, st, pipe, hdr, jobsd); break;
+        case 8: record_launch<4, 0>(E, key, st, pipe, hdr, jobsd, reqs); break;
+        case 9: record_launch<4, 1>(E, key, st, pipe, hdr, jobsd, reqs); break;
+        case 10: record_launch<5, 0>(E, key, st, pipe, hdr, jobsd, reqs); break;
+        case 11: record_launch<5, 1>(E, key, st, pipe, hdr, jobsd, reqs); break;
+        case 12: record_launch<6, 0>(E, key, st, pipe, hdr, jobsd, reqs); break;
+        default: record_launch<6, 1>(E, key, st, pipe, hdr, jobsd, reqs); break;
     }
 }
 
@@ -846,6 +872,12 @@ int alloc_slot(Engine &E) {
     return s;
 }
 
+// Deals in whole blocks of 24, lane runs too: a batch for the flow kernel's block-stepping variant.
+bool deal_blocks(const war_request &r, int W, u64 lo, u64 hi, const war_tuning &t) {
+    return W == 4 && r.mode == WAR_MODE_FULL && r.n >= 5 && lo % 24 == 0 && hi % 24 == 0 &&
+           (t.ranks_per_refill / 64) % 24 == 0;
+}
+
 // One persistent batch [lo, hi) of a single request.
 int run_range_batch(Engine &E, ReqHost &h, int W, u64 lo, u64 hi, const war_tuning &t, double *dev) {
     const war_request &r = h.rq;
@@ -853,14 +885,62 @@ int run_range_batch(Engine &E, ReqHost &h, int W, u64 lo, u64 hi, const war_tuni
     jobs[0].lo = lo; jobs[0].hi = hi; jobs[0].req = (u32)h.slot;
     float ms = 0;
     const int det = r.detector;
-    // deals in whole blocks of 24, lane runs too: the flow kernel's block-stepping variant
-    const bool blk = W == 4 && r.mode == WAR_MODE_FULL && r.n >= 5 && lo % 24 == 0 && hi % 24 == 0 &&
-                     (t.ranks_per_refill / 64) % 24 == 0;
+    const bool blk = deal_blocks(r, W, lo, hi, t);
     int rc;
     if (W == 4)      rc = run_batch_dispatch<4>(E, r.rule, jobs, t, r.hbins, r.lbins, h.dev.walk != 0, false, &ms, det, blk);
     else if (W == 6) rc = run_batch_dispatch<6>(E, r.rule, jobs, t, r.hbins, r.lbins, false, false, &ms, det);
     else             rc = run_batch_dispatch<5>(E, r.rule, jobs, t, r.hbins, r.lbins, h.dev.walk != 0, false, &ms, det);
     *dev += ms;
+    return rc;
+}
+
+// The persistent batches of a request's range [r.lo, r.hi), in order: run_one(lo, hi, tuning) for
+// each. On the flow kernel, batches of FLOW_BATCH positions (the cycle records bound them to `batch`
+// with walks); canonical lane runs of whole orders starting at split 0; deals in whole blocks of 24
+// with the parts before the first and after the last block boundary apart.
+template <class F>
+int for_each_persistent_batch(const war_request &r, const war_tuning &t, int W, bool walk, u64 batch, F &&run_one) {
+    int rc = 0;
+    const bool flow = t.stage_kind == 3;
+    war_tuning bt = t;
+    u64 bstep = batch;
+    u64 lo = r.lo, hi = r.hi;    // batches cover [lo, hi); a tail [hi, r.hi) runs after them
+    if (flow) {
+        // the flow kernel needs no queues, so its batches are larger (fewer batch ends); with
+        // cycle walks the cycle records bound them as in the staged pipeline
+        if (!walk) bstep = FLOW_BATCH;
+        if (r.mode == WAR_MODE_CANON && r.n >= 3) {
+            // canonical positions: lane runs of whole orders (a multiple of n-1 positions) that
+            // start at split 0, so every lane of a wave steps to its next order (next_perm) in
+            // the same pool step instead of a few lanes in almost every step
+            const u64 m = (u64)r.n - 1u;
+            const u64 per = std::max<u64>(1, ((u64)bt.ranks_per_refill / 64 + m / 2) / m) * m;
+            if (64 * per <= MAX_TILE) bt.ranks_per_refill = (int)(64 * per);
+            const u64 tp = (u64)bt.ranks_per_refill;
+            bstep = bstep / tp * tp;
+            const u64 a = (lo + m - 1) / m * m;          // the first order boundary
+            if (a > lo && a < r.hi) {                    // the part of an order before it
+                rc = run_one(lo, a, bt);
+                lo = a;
+            }
+        } else if (r.mode == WAR_MODE_FULL && W == 4 && r.n >= 5 && !walk) {
+            // deals: lane runs of whole blocks of 24 (the orders that share all but the last
+            // four cards), which the kernel steps through with one swap of two dealt cards
+            // each; the parts before the first and after the last block boundary run apart
+            const u64 per = std::max<u64>(1, ((u64)bt.ranks_per_refill / 64 + 12) / 24) * 24;
+            if (64 * per <= MAX_TILE) bt.ranks_per_refill = (int)(64 * per);
+            const u64 tp = (u64)bt.ranks_per_refill;
+            bstep = bstep / tp * tp;
+            const u64 a = (lo + 23) / 24 * 24, b = r.hi / 24 * 24;
+            if (a < b) {
+                if (a > lo) rc = run_one(lo, a, bt);
+                lo = a;
+                hi = b;
+            }
+        }
+    }
+    for (; lo < hi && !rc; lo += bstep) rc = run_one(lo, std::min<u64>(hi, lo + bstep), bt);
+    if (hi < r.hi && !rc) rc = run_one(hi, r.hi, bt);
     return rc;
 }
 
@@ -891,46 +971,8 @@ int run_request_locked(Engine &E, ReqHost &h) {
             dev += ms;
         }
     } else {
-        const bool flow = t.stage_kind == 3;
-        war_tuning bt = t;
-        u64 bstep = E.batch;
-        u64 lo = r.lo, hi = r.hi;    // batches cover [lo, hi); a tail [hi, r.hi) runs after them
-        if (flow) {
-            // the flow kernel needs no queues, so its batches are larger (fewer batch ends); with
-            // cycle walks the cycle records bound them as in the staged pipeline
-            if (!h.dev.walk) bstep = FLOW_BATCH;
-            if (r.mode == WAR_MODE_CANON && r.n >= 3) {
-                // canonical positions: lane runs of whole orders (a multiple of n-1 positions) that
-                // start at split 0, so every lane of a wave steps to its next order (next_perm) in
-                // the same pool step instead of a few lanes in almost every step
-                const u64 m = (u64)r.n - 1u;
-                const u64 per = std::max<u64>(1, ((u64)bt.ranks_per_refill / 64 + m / 2) / m) * m;
-                if (64 * per <= MAX_TILE) bt.ranks_per_refill = (int)(64 * per);
-                const u64 tp = (u64)bt.ranks_per_refill;
-                bstep = bstep / tp * tp;
-                const u64 a = (lo + m - 1) / m * m;          // the first order boundary
-                if (a > lo && a < r.hi) {                    // the part of an order before it
-                    rc = run_range_batch(E, h, W, lo, a, bt, &dev);
-                    lo = a;
-                }
-            } else if (r.mode == WAR_MODE_FULL && W == 4 && r.n >= 5 && !h.dev.walk) {
-                // deals: lane runs of whole blocks of 24 (the orders that share all but the last
-                // four cards), which the kernel steps through with one swap of two dealt cards
-                // each; the parts before the first and after the last block boundary run apart
-                const u64 per = std::max<u64>(1, ((u64)bt.ranks_per_refill / 64 + 12) / 24) * 24;
-                if (64 * per <= MAX_TILE) bt.ranks_per_refill = (int)(64 * per);
-                const u64 tp = (u64)bt.ranks_per_refill;
-                bstep = bstep / tp * tp;
-                const u64 a = (lo + 23) / 24 * 24, b = r.hi / 24 * 24;
-                if (a < b) {
-                    if (a > lo) rc = run_range_batch(E, h, W, lo, a, bt, &dev);
-                    lo = a;
-                    hi = b;
-                }
-            }
-        }
-        for (; lo < hi && !rc; lo += bstep) rc = run_range_batch(E, h, W, lo, std::min<u64>(hi, lo + bstep), bt, &dev);
-        if (hi < r.hi && !rc) rc = run_range_batch(E, h, W, hi, r.hi, bt, &dev);
+        rc = for_each_persistent_batch(r, t, W, h.dev.walk != 0, E.batch,
+                                       [&](u64 a, u64 b, const war_tuning &bt) { return run_range_batch(E, h, W, a, b, bt, &dev); });
     }
     h.dev_ms = dev;
     if (!rc) rc = req_collect(E, h);
@@ -1244,6 +1286,8 @@ int war_init(int32_t device, uint32_t flags) {
         HIPCHK(hipEventCreateWithFlags(&E->bjoin[b], hipEventDisableTiming));
     }
     HIPCHK(hipEventCreateWithFlags(&E->bfork, hipEventDisableTiming));
+    HIPCHK(hipGetSymbolAddress((void **)&E->na, HIP_SYMBOL(scratch)));
+    HIPCHK(hipHostMalloc((void **)&E->na_host, NA_WORDS * sizeof(u64), hipHostMallocDefault));
     HIPCHK(hipMemsetAsync(E->reqs, 0, MAXSLOTS * sizeof(ReqDev), E->st));
     HIPCHK(hipMemsetAsync(E->pipe, 0, sizeof(Pipe), E->st));
     HIPCHK(hipStreamSynchronize(E->st));
@@ -1268,6 +1312,7 @@ int war_shutdown(void) {
         (void)hipFree(E->bpipe);
         (void)hipFree(E->bhdr);
         (void)hipHostFree(E->bstage);
+        (void)hipHostFree(E->na_host);
         for (QueueBufs *Q : {&E->qb4, &E->qb5}) {
             for (int i = 0; i < 2; i++) {
                 (void)hipFree(Q->qa[i]); (void)hipFree(Q->qb[i]); (void)hipFree(Q->qreq[i]); (void)hipFree(Q->qpos[i]);
@@ -1355,6 +1400,138 @@ int war_run_range(const war_request *req, war_result *res) {
     }
     res->latency_ms = now_ms() - t0;
     return rc;
+}
+
+// The same query as war_run_range with no device allocation and no graph: the counters, histograms,
+// pipe counters, job list and request descriptor live in `scratch` (resolved at war_init), cleared
+// by a fill kernel at every call; kernels are launched directly on the engine's stream; the job list
+// and the results pass through pinned staging allocated at init; nothing on the host heap. Requests
+// without cycle walks, on the plain kernel or the flow schedule, histograms of at most NA_BINS bins.
+// Every call on a GPU uses that GPU's one array, and keeps its contents until the next call: the
+// engine mutex serialises the calls of a process, so overlapping calls on one GPU would race only
+// if something else wrote the array.
+int war_run_range_noalloc(const war_request *req, war_result *res) {
+    t_err.clear();
+    if (!g_eng) return fail(60, "war_init has not been called");
+    int rc = check_request(req);
+    if (rc) return rc;
+    if (!res) return fail(1, "null result");
+    const war_request &r = *req;
+    if (r.distinct || r.dbins || r.shape_blk)
+        return fail(90, "war_run_range_noalloc: no cycle walks (distinct, dbins, shape need war_run_range)");
+    if ((u32)r.hbins > NA_BINS || (u32)r.lbins > NA_BINS)
+        return fail(90, "war_run_range_noalloc: at most %u histogram bins (hbins %d, lbins %d)", NA_BINS, r.hbins, r.lbins);
+    Engine &E = *g_eng;
+    const double t0 = now_ms();
+    std::lock_guard<std::mutex> g(E.mu);
+    war_tuning t;
+    resolve_tuning(E, r, t);
+    rc = check_tuning(t);
+    if (!rc) rc = check_cap(r, t);
+    int W = 4;
+    if (!rc) rc = layout_of(r, t, &W);
+    if (rc) return rc;
+    const bool flow = t.stage_kind == 3 && (r.rule == 0 || r.detector == WAR_DET_BRENT);
+    if (r.kernel == WAR_KERNEL_PERSISTENT && !flow)
+        return fail(90, "war_run_range_noalloc: the persistent kernel runs the flow schedule only (stage_kind 3)");
+    u64 *na = E.na;
+    Pipe *pipe = (Pipe *)(na + NA_PIPE);
+    JobHdr *hdr = (JobHdr *)(na + NA_HDR);
+    Job *job = (Job *)(na + NA_JOB);
+    ReqDev *rd = (ReqDev *)(na + NA_REQ);
+    // counters and histograms start at zero (the array keeps the last call's contents)
+    const u32 ncnt = C_N + (u32)r.hbins + (u32)r.lbins;
+    hipLaunchKernelGGL(k_fill, dim3(std::min(64u, (ncnt + 255u) / 256u)), dim3(256), 0, E.st, na + NA_CNT, 0ull, ncnt);
+    ReqDev d{};
+    d.n = (u32)r.n; d.mode = (u32)r.mode; d.rule = (u32)r.rule;
+    d.hbins = (u32)r.hbins; d.lbins = (u32)r.lbins;
+    d.cap = r.step_cap;
+    d.cnt = na + NA_CNT; d.hist = na + NA_HIST; d.lenhist = na + NA_HIST + r.hbins;
+    u64 *hs = E.na_host;                                  // pinned staging: [0, 16) request, [16, ..) job list
+    memcpy(hs, &d, sizeof d);
+    HIPCHK(hipMemcpyAsync(rd, hs, sizeof d, hipMemcpyHostToDevice, E.st));
+    double dev = 0;
+    int batches = 0;
+    auto one = [&](u64 lo, u64 hi, const war_tuning &bt) -> int {
+        // the previous batch's staging copy is complete: every batch ends with a synchronisation
+        char *js = (char *)(hs + 16);
+        JobHdr h{};
+        const u64 tp = (u64)bt.ranks_per_refill;
+        h.njobs = 1; h.req0 = 0; h.ntiles = (hi - lo + tp - 1) / tp; h.capck = r.step_cap ? 1u : 0u;
+        Job j{};
+        j.lo = lo; j.hi = hi; j.tile_base = 0; j.req = 0;
+        memcpy(js, &h, sizeof h);
+        memcpy(js + JOBS_OFF, &j, sizeof j);
+        HIPCHK(hipMemcpyAsync(hdr, js, JOBS_OFF + sizeof j, hipMemcpyHostToDevice, E.st));
+        const bool blk = deal_blocks(r, W, lo, hi, bt);
+        GraphKey key{W, r.rule, 1, bt.waves_per_cu, bt.ranks_per_refill / 64, FLOW_MIN_FREE, 0, 0, round_pow2(r.hbins),
+                     round_pow2(r.lbins), 3, r.rule ? WAR_DET_BRENT : r.detector, (blk && W == 4) ? 1 : 0};
+        HIPCHK(hipEventRecord(E.ev0, E.st));
+        record_launch_any(E, key, E.st, pipe, hdr, job, rd);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(E.ev1, E.st));
+        HIPCHK(hipStreamSynchronize(E.st));
+        float m = 0;
+        HIPCHK(hipEventElapsedTime(&m, E.ev0, E.ev1));
+        dev += m;
+        batches++;
+        return 0;
+    };
+    if (r.kernel == WAR_KERNEL_PERSISTENT) {
+        rc = for_each_persistent_batch(r, t, W, false, E.batch, one);
+    } else {
+        // the plain kernel, chunks of 2^26 positions as war_run_range runs them
+        const u32 per = (u32)t.plain_per;
+        for (u64 lo = r.lo; lo < r.hi && !rc; lo += 1ull << 26) {
+            const u64 hi = std::min<u64>(r.hi, lo + (1ull << 26));
+            HIPCHK(hipEventRecord(E.ev0, E.st));
+            const u64 threads = (hi - lo + per - 1) / per, blocks = (threads + 255) / 256;
+            const size_t lds = (size_t)(r.hbins + r.lbins) * sizeof(u32);
+            auto launch = [&](auto W_, auto R_) {
+                constexpr int WW = decltype(W_)::value, RR = decltype(R_)::value;
+                PlainArgs<WW> P{};
+                P.lo = lo; P.hi = hi; P.per = per; P.hb = (u32)r.hbins; P.lb = (u32)r.lbins; P.tile_pos = PLAIN_CYC_TILE;
+                P.R = d; P.req = 0; P.pipe = pipe;
+                if (RR == 0 && r.detector == WAR_DET_STREAK)
+                    hipLaunchKernelGGL((k_plain<WW, 0, 1>), dim3((unsigned)blocks), dim3(256), lds, E.st, P);
+                else if (RR == 0 && r.detector == WAR_DET_ALT)
+                    hipLaunchKernelGGL((k_plain<WW, 0, 2>), dim3((unsigned)blocks), dim3(256), lds, E.st, P);
+                else
+                    hipLaunchKernelGGL((k_plain<WW, RR, 0>), dim3((unsigned)blocks), dim3(256), lds, E.st, P);
+            };
+            using I4 = std::integral_constant<int, 4>;
+            using I5 = std::integral_constant<int, 5>;
+            using R0 = std::integral_constant<int, 0>;
+            using R1 = std::integral_constant<int, 1>;
+            if (W == 4) { if (r.rule) launch(I4{}, R1{}); else launch(I4{}, R0{}); }
+            else        { if (r.rule) launch(I5{}, R1{}); else launch(I5{}, R0{}); }
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipEventRecord(E.ev1, E.st));
+            HIPCHK(hipStreamSynchronize(E.st));
+            float m = 0;
+            HIPCHK(hipEventElapsedTime(&m, E.ev0, E.ev1));
+            dev += m;
+        }
+    }
+    if (rc) return rc;
+    // results: counters and histograms through the pinned staging
+    HIPCHK(hipMemcpyAsync(hs, na + NA_CNT, (size_t)ncnt * sizeof(u64), hipMemcpyDeviceToHost, E.st));
+    HIPCHK(hipStreamSynchronize(E.st));
+    const u64 *b = hs;
+    res->cyc = b[C_CYC]; res->done = b[C_DONE]; res->maxfin = b[C_MAXFIN]; res->turns = b[C_TURNS];
+    res->oncycle = 0; res->distinct_minkey = 0; res->distinct_hashset = 0;
+    res->shape_nonstd = 0; res->shape_unchk = 0; res->undetermined = b[C_UNDET];
+    if (res->hist) for (int i = 0; i < std::min(res->hbins, r.hbins); i++) res->hist[i] = b[C_N + i];
+    if (res->lenhist) for (int i = 0; i < std::min(res->lbins, r.lbins); i++) res->lenhist[i] = b[C_N + r.hbins + i];
+    res->device_ms = dev;
+    res->latency_ms = now_ms() - t0;
+    if (b[C_UNDET] && ((r.flags & WAR_FLAG_UNDET_ERROR) || (!r.step_cap && r.detector != WAR_DET_STREAK)))
+        return fail(21, "%llu games not resolved within %u turns", b[C_UNDET],
+                    r.step_cap ? r.step_cap : (r.detector == WAR_DET_STREAK ? WAR_STREAK_CAP : WAR_SAFETY_CAP));
+    if (r.mode != WAR_MODE_HALF && b[C_DONE] != r.hi - r.lo)
+        return fail(22, "played %llu positions, expected %llu", b[C_DONE], r.hi - r.lo);
+    (void)batches;
+    return 0;
 }
 
 int war_run_batch(const war_request *reqs, int32_t count, war_result *res, int32_t *launches) {

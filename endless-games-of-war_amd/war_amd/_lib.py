@@ -95,6 +95,7 @@ _SIGS = {
     "war_last_error": ([], ctypes.c_char_p),
     "war_device_info": ([ctypes.POINTER(WarDevinfo)], ctypes.c_int),
     "war_run_range": ([ctypes.POINTER(WarRequest), ctypes.POINTER(WarResult)], ctypes.c_int),
+    "war_run_range_noalloc": ([ctypes.POINTER(WarRequest), ctypes.POINTER(WarResult)], ctypes.c_int),
     "war_run_batch": ([ctypes.POINTER(WarRequest), ctypes.c_int32, ctypes.POINTER(WarResult),
                        ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
     "war_set_tuning": ([ctypes.POINTER(WarTuning)], ctypes.c_int),

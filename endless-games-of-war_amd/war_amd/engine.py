@@ -160,6 +160,14 @@ class Engine:
         check(self.lib.war_run_range(ctypes.byref(req), ctypes.byref(buf.res)))
         return buf.as_dict()
 
+    def run_range_noalloc(self, n, mode="full", lo=0, hi=None, **kw):
+        """run_range through war_run_range_noalloc: the same counts with no device allocation (the
+        engine's static scratch array); no cycle walks, histograms of at most 4096 bins."""
+        req = make_request(n, mode, lo, hi, **kw)
+        buf = _ResultBuf(req)
+        check(self.lib.war_run_range_noalloc(ctypes.byref(req), ctypes.byref(buf.res)))
+        return buf.as_dict()
+
     def run_batch(self, queries):
         """Several requests in shared persistent launches (war_run_batch). queries: [(n, mode, lo,
         hi, kw), ...] with run_range's keywords. Returns (results, launches): one result dict per

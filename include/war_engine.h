@@ -180,6 +180,16 @@ int war_device_info(war_devinfo *out);
 /* Synchronous range query: plays every position of [lo, hi) on the GPU and fills *res. */
 int war_run_range(const war_request *req, war_result *res);
 
+/* war_run_range with no device allocation: no counterpart in the reference. For callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * same counts as war_run_range, from a static device array of the library (about 67 KB per GPU,
+ * held until the process ends; its address is resolved at war_init) instead of the per-request
+ * allocation, with kernels launched directly (no graph) and results through pinned staging set up at
+ * war_init. Limits: no cycle walks (distinct, dbins, shape), at most 4096 bins per histogram, the
+ * plain kernel or the flow schedule. Calls are serialised per process; calls that overlapped on one
+ * GPU would share the array and race. */
+int war_run_range_noalloc(const war_request *req, war_result *res);
+
 /* Several requests of mixed n, mode and rule in shared persistent launches, synchronously: the
  * reference launches war_count / war_canon once per range (gpu/war_gpu.py:309-328); here requests
  * of one pile layout, rule, detector and cycle-walk need share each kernel launch (one, when their
