@@ -7,23 +7,27 @@ bit-exactly against the published a(13) = 4,880,546,113, longest 57 and cycle-le
 every step. One step = one whole enumeration. Beside it, on the same GPU in the same run:
   brent_flow     the same enumeration with Brent's detector on the flow kernel (its turn count is
                  the reference's Sum of rounds), and with Brent on the staged pipeline;
-  plain_kernel   the one-deal-per-lane kernel on the same n = 13 enumeration;
+  plain_kernel   the one-deal-per-lane kernel on the same n = 13 enumeration (Brent, its fastest turn
+                 form; the flow kernel's turn measured 18 % slower in it);
   canon14        the n = 14 canonical scan (13! * 13 positions, Brent; flow kernel and staged
                  pipeline) against the published counts, and the plain kernel on sampled chunks;
   n17_range      the n = 17 rank range [0, 8992587776) the reference logged: held-card layout
                  (4-bit piles, card 16 apart) and 5-bit piles, Brent and the alternation detector;
-  serving        a continuous-batching run of mixed queries with p50/p99 latency, beside the same
-                 queries one by one through the plain and the persistent kernels;
-  mixed_batch    240 requests of n = 5..15 (full, canonical, distinct) in shared persistent launches
-                 (war_run_batch), beside the same requests one by one;
+  serving        512 mixed queries (n = 7..17, lengths log-uniform to 10^10) through the
+                 continuous-batching loop: saturated throughput, then Poisson arrivals at 0.25, 0.5
+                 and 0.9 of the saturated rate with p50/p99 latency per rate, beside the same queries
+                 one by one through the plain and the persistent kernels;
+  mixed_batch    256 requests of n = 5..15 (full, canonical, distinct) autotuned, then run as one
+                 war_run_batch call (one graph launch), beside the same requests one by one;
   roofline       integer VALU: useful lane-ops/s (turns x VALU instructions per turn of the flow
                  kernel's loop x 64 lanes) against the VALU's dual-issue peak, CUs x 128 lane-ops
                  per clock, with the fraction of the single-issue rate and of the turn's own
                  instruction-mix ceiling beside it;
-  cpu_baseline   the CPU oracle (OpenMP) on a bounded sample of the same n = 13 workload.
+  cpu_baseline   the CPU oracle (OpenMP, every CPU the process may use) on bounded samples of the
+                 n = 13 deals, all n = 9 deals and sampled n = 14 canonical chunks.
 
   python bench.py [--gpus 1] [--steps 3] [--warmup 1]
-                  [--only main|brent13|staged13|canon14|canon14walk|plain13|copy]
+                  [--only main|brent13|staged13|canon14|canon14walk|plain13|plaincanon14|serving|mixed|copy]
                   [--no-extras]
 """
 import argparse
@@ -88,6 +92,11 @@ def chunks_spread(total, count, size, seed):
     return out
 
 
+def canon14_chunks(tot14):
+    """8 chunks of 13 * 2^22 n = 14 canonical positions (whole q-ranks), seed 14."""
+    return [(lo - lo % 13, lo - lo % 13 + 13 * (1 << 22)) for lo, _ in chunks_spread(tot14, 8, 13 * (1 << 22), 14)]
+
+
 def cpu_baseline(n, mode, chunks, threads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
@@ -105,7 +114,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--only", choices=["main", "brent13", "staged13", "canon14", "canon14walk", "plain13", "copy"], default=None,
+    ap.add_argument("--only", choices=["main", "brent13", "staged13", "canon14", "canon14walk", "plain13",
+                                       "plaincanon14", "serving", "mixed", "copy"], default=None,
                     help="run one workload only (for profiling); copy: a 1 GiB device-to-device copy that "
                          "calibrates the HBM byte counters")
     ap.add_argument("--no-extras", action="store_true")
@@ -171,6 +181,18 @@ def main():
         r = eng.run_range(13, "full", kernel="plain", hbins=256)
         check_n13(r)
         log("n13 plain device_ms %.1f" % r["device_ms"])
+        return
+    if a.only == "plaincanon14":
+        tot14 = math.factorial(13) * 13
+        for lo, hi in canon14_chunks(tot14):
+            r = eng.run_range(14, "canon", lo, hi, kernel="plain")
+        log("canon14 plain (last chunk) device_ms %.1f" % r["device_ms"])
+        return
+    if a.only == "serving":
+        print(json.dumps(serving(eng)), flush=True)
+        return
+    if a.only == "mixed":
+        print(json.dumps(mixed_batch(eng)), flush=True)
         return
 
     # ---- headline: n = 13 full enumeration as the drop-in runs it (flow kernel, alternation detector)
@@ -242,7 +264,11 @@ def main():
         pw = time.perf_counter() - t0
         check_n13(r)
         out["plain_kernel"] = {"workload": "n13_full", "value": n13 / pw, "unit": "deals/s",
-                               "device_ms": r["device_ms"], "persistent_over_plain": value / (n13 / pw)}
+                               "device_ms": r["device_ms"], "persistent_over_plain": value / (n13 / pw),
+                               "brent_flow_over_plain": r["device_ms"] / bf["flow"]["device_ms"],
+                               "note": "the plain kernel runs Brent with its own fastest turn (the flow kernel's "
+                                       "turn_top measured 18 % slower in it); brent_flow_over_plain compares the two "
+                                       "kernels on the same detector, device time against device time"}
         log("n13 plain: %.3e deals/s" % (n13 / pw))
 
         # ---- n = 14 canonical scan, persistent, whole range: flow kernel, then the staged pipeline
@@ -259,7 +285,7 @@ def main():
                  "staged_device_ms": rs["device_ms"], "turns": r["turns"],
                  "parity": "cycling 2872129173, longest 465, hist {96,112} equal the published values"}
         # both kernels on the same sampled chunks (seed 14, whole q-ranks): batched / plain ratio
-        ch = [(lo - lo % 13, lo - lo % 13 + 13 * (1 << 22)) for lo, _ in chunks_spread(tot14, 8, 13 * (1 << 22), 14)]
+        ch = canon14_chunks(tot14)
         tp = tq = 0.0
         for lo, hi in ch:
             rp = eng.run_range(14, "canon", lo, hi, kernel="plain")
@@ -294,90 +320,130 @@ def main():
         # ---- serving: continuous batching of mixed queries
         out["serving"] = serving(eng)
 
-        # ---- mixed-n batch: many requests of n = 5..15 in shared persistent launches
+        # ---- mixed-n batch: 256 requests of n = 5..15 as one graph launch, autotuned
         out["mixed_batch"] = mixed_batch(eng)
-        log("mixed batch: %.3e positions/s in %d launches; one by one %.3e"
-            % (out["mixed_batch"]["value"], out["mixed_batch"]["launches"],
+        log("mixed batch: %.3e positions/s in %d graph launch(es); one by one %.3e"
+            % (out["mixed_batch"]["value"], out["mixed_batch"]["graph_launches"],
                out["mixed_batch"]["one_by_one_persistent_positions_per_s"]))
 
-        # ---- CPU baseline: the oracle on sampled n = 13 chunks, compared chunk by chunk
-        threads = min(16, os.cpu_count() or 1)
-        ch = chunks_spread(n13, a.cpu_chunks, 1 << 22, 13)
-        cres, cel = cpu_baseline(13, "full", ch, threads)
-        for (lo, hi), c in zip(ch, cres):
-            g = eng.run_range(13, "full", lo, hi, hbins=256)
-            assert (g["cyc"], g["maxfin"], g["turns"], g["hist"]) == (c["cyc"], c["maxfin"], c["turns"], c["hist"])
-        out["cpu_baseline"] = {"value": sum(hi - lo for lo, hi in ch) / cel, "unit": "deals/s", "cores": threads,
-                               "kind": "port",
-                               "sample": "%d chunks of 2^22 n=13 deal ranks spread over 13! (seed 13), "
-                                         "each equal to the GPU's counts" % len(ch)}
-        log("cpu baseline: %.3e deals/s on %d threads" % (out["cpu_baseline"]["value"], threads))
+        # ---- CPU baseline: the oracle on sampled chunks of the same workloads, compared chunk by chunk
+        out["cpu_baseline"] = cpu_legs(eng, a.cpu_chunks)
+        out["cpu_baseline"]["c3_n14_canon"]["gpu_over_cpu"] = (out["canon14"]["persistent_value"] /
+                                                                out["cpu_baseline"]["c3_n14_canon"]["value"])
     print(json.dumps(out), flush=True)
 
 
-def serving(eng, nreq=64, seed=20261203, rate=100.0):
-    """Mixed (n, range, mode) queries. (1) Poisson arrivals at `rate`/s through the continuous-batching
-    loop: p50/p99 latency. (2) All submitted at once: aggregate positions/s. (3) The same queries one
-    by one, synchronously, through the plain kernel and through the persistent kernel. Every count
-    must be identical across (1), (2) and (3)."""
-    rng = random.Random(seed)
+def served_equal(a, b, keys):
+    for k in keys:
+        if a[k] != b[k]:
+            return k
+    return None
+
+
+def pct(lat, p):
+    lat = sorted(lat)
+    return lat[min(len(lat) - 1, max(0, int(math.ceil(p * len(lat))) - 1))]
+
+
+def serving(eng, nreq=512, seed=20261205, fractions=(0.25, 0.5, 0.9), slots=64):
+    """The serving workload: `nreq` queries, n uniform over 7..17, mode full or canonical (canonical
+    n <= 13 with distinct), length floor(10^U(3, 10)) clipped to the total, lo uniform (numpy PCG64).
+    (1) The queries one by one, synchronously, through the persistent kernel (the parity reference of
+    every later run). (2) All submitted at once (saturation): aggregate positions/s and the saturated
+    request rate. (3) Poisson arrivals at fractions of that rate: p50 / p99 latency (submit to
+    result) per rate. (4) The queries one by one through the plain kernel. Every result must equal
+    its one-by-one persistent result (tolerance 0); the first difference stops the bench."""
+    import numpy as np
+    rng = np.random.Generator(np.random.PCG64(seed))
     qs = []
     for _ in range(nreq):
-        n = rng.randint(7, 17)
-        mode = rng.choice(["full", "canon"])
+        n = int(rng.integers(7, 18))
+        mode = "canon" if rng.random() < 0.5 else "full"
         total = math.factorial(n - 1) * (n - 1) if mode == "canon" else math.factorial(n)
-        ln = min(total, int(10 ** rng.uniform(3, 8.5)))
-        lo = rng.randrange(0, total - ln + 1)
-        distinct = mode == "canon" and n <= 13 and rng.random() < 0.3
-        qs.append((n, mode, lo, lo + ln, distinct))
-    keys = ("cyc", "done", "maxfin", "turns", "hist", "distinct_minkey")
+        ln = min(total, int(10 ** rng.uniform(3, 10)))
+        lo = int(rng.integers(0, total - ln + 1))
+        kw = {"hbins": 128}
+        if mode == "canon" and n <= 13:
+            kw.update(distinct=True)
+        qs.append((n, mode, lo, lo + ln, kw))
+    npos = sum(hi - lo for _, _, lo, hi, _ in qs)
+    keys = ("cyc", "done", "maxfin", "turns", "hist", "distinct_minkey", "undetermined")
+    alone, alone_lat = [], []
+    t0 = time.perf_counter()
+    for n, mode, lo, hi, kw in qs:
+        t1 = time.perf_counter()
+        alone.append(eng.run_range(n, mode, lo, hi, **kw))
+        alone_lat.append(1e3 * (time.perf_counter() - t1))
+    t_alone = time.perf_counter() - t0
+    log("serving: %d queries, %d positions; one by one (persistent) %.1f s" % (nreq, npos, t_alone))
 
     def run_served(gaps):
-        eng.serve_start(max_slots=64, step_positions=1 << 26)
+        eng.serve_start(max_slots=slots, step_positions=1 << 26)
         ids = []
         t0 = time.perf_counter()
-        for q, g in zip(qs, gaps):
-            if g:
-                time.sleep(g)
-            n, mode, lo, hi, d = q
-            ids.append(eng.submit(n, mode, lo, hi, hbins=128, distinct=d))
-        res = [eng.wait(i) for i in ids]
+        try:
+            for q, g in zip(qs, gaps):
+                if g:
+                    time.sleep(g)
+                n, mode, lo, hi, kw = q
+                ids.append(eng.submit(n, mode, lo, hi, **kw))
+            res = [eng.wait(i) for i in ids]
+        finally:
+            eng.serve_stop()
         wall = time.perf_counter() - t0
-        eng.serve_stop()
+        for q, r, a in zip(qs, res, alone):
+            bad = served_equal(r, a, keys)
+            assert bad is None, ("served result differs from the same query alone", q, bad)
         return res, wall
 
-    arrivals, wall_a = run_served([rng.expovariate(rate) for _ in qs])
-    burst, wall_b = run_served([0.0] * len(qs))
-    lat = sorted(r["latency_ms"] for r in arrivals)
-    npos = sum(hi - lo for _, _, lo, hi, _ in qs)
-    alone = {}
-    for kernel in ("plain", "persistent"):
-        t = 0.0
-        for q, r, b in zip(qs, arrivals, burst):
-            n, mode, lo, hi, d = q
-            t1 = time.perf_counter()
-            p = eng.run_range(n, mode, lo, hi, kernel=kernel, hbins=128, distinct=d)
-            t += time.perf_counter() - t1
-            for k in keys:
-                assert p[k] == r[k] == b[k], (q, kernel, k)
-        alone[kernel] = npos / t
-    return {"requests": nreq, "arrival_rate_per_s": rate, "p50_ms": lat[len(lat) // 2],
-            "p99_ms": lat[min(len(lat) - 1, int(0.99 * len(lat)))], "arrivals_wall_s": wall_a,
-            "saturated_positions_per_s": npos / wall_b,
-            "one_by_one_plain_positions_per_s": alone["plain"],
-            "one_by_one_persistent_positions_per_s": alone["persistent"],
-            "queries": "n uniform 7..17, full or canon (distinct for 30%% of canon n<=13), length log-uniform "
-                       "[1e3, 10^8.5] (%d positions in all); counts identical alone, in the mix and in the "
-                       "burst" % npos}
+    burst, wall_b = run_served([0.0] * nreq)
+    sat_rate = nreq / wall_b
+    lb = [r["latency_ms"] for r in burst]
+    out = {"workload": "%d queries, n uniform 7..17, full or canonical (canonical n <= 13 with distinct), length "
+                       "floor(10^U(3, 10)) clipped to the total, lo uniform, numpy PCG64 seed %d: %d positions"
+                       % (nreq, seed, npos),
+           "slots": slots, "step_positions": 1 << 26,
+           "saturated": {"positions_per_s": npos / wall_b, "requests_per_s": sat_rate, "wall_s": wall_b,
+                         "p50_ms": pct(lb, 0.5), "p99_ms": pct(lb, 0.99)},
+           "rates": []}
+    log("serving saturated: %.3e positions/s, %.1f req/s" % (npos / wall_b, sat_rate))
+    for f in fractions:
+        rate = f * sat_rate
+        gaps = list(rng.exponential(1.0 / rate, nreq))
+        res, wall = run_served(gaps)
+        lat = [r["latency_ms"] for r in res]
+        out["rates"].append({"fraction_of_saturation": f, "arrival_rate_per_s": rate,
+                             "offered_positions_per_s": rate * npos / nreq, "p50_ms": pct(lat, 0.5),
+                             "p99_ms": pct(lat, 0.99), "max_ms": max(lat), "wall_s": wall,
+                             "achieved_positions_per_s": npos / wall})
+        log("serving at %.1f req/s: p50 %.2f ms, p99 %.2f ms" % (rate, pct(lat, 0.5), pct(lat, 0.99)))
+    t0 = time.perf_counter()
+    plain_lat = []
+    for q, a in zip(qs, alone):
+        n, mode, lo, hi, kw = q
+        t1 = time.perf_counter()
+        p = eng.run_range(n, mode, lo, hi, kernel="plain", **kw)
+        plain_lat.append(1e3 * (time.perf_counter() - t1))
+        bad = served_equal(p, a, keys)
+        assert bad is None, ("plain kernel differs", q, bad)
+    t_plain = time.perf_counter() - t0
+    out["one_by_one_persistent"] = {"positions_per_s": npos / t_alone, "p50_ms": pct(alone_lat, 0.5),
+                                    "p99_ms": pct(alone_lat, 0.99)}
+    out["one_by_one_plain"] = {"positions_per_s": npos / t_plain, "p50_ms": pct(plain_lat, 0.5),
+                               "p99_ms": pct(plain_lat, 0.99)}
+    out["parity"] = ("every served result (saturated and at each rate) and every plain-kernel result equals the "
+                     "same query run alone on the persistent kernel (cyc, done, maxfin, turns, hist, minkey)")
+    log("serving one by one: plain %.3e, persistent %.3e positions/s" % (npos / t_plain, npos / t_alone))
+    return out
 
 
-def mixed_batch(eng, nreq=240, seed=20261203):
-    """The mixed-n single-launch workload: seeded numpy PCG64, `nreq` requests (the engine has 248
-    request slots), n uniform over 5..15, full or canonical, distinct + dhist for a third of the
-    canonical requests of n <= 13, length floor(10^U(3, 9)) clipped to the total, lo uniform. Run
-    as one war_run_batch call (requests of one layout, rule and detector share each launch; with
-    cycle walks a launch holds 2^28 positions), then one by one through war_run_range: every count
-    must be identical."""
+def mixed_batch(eng, nreq=256, seed=20261203):
+    """The mixed-n single-launch workload: seeded numpy PCG64, `nreq` requests, n uniform over 5..15,
+    full or canonical (canonical n <= 13 with distinct + dhist), length floor(10^U(3, 9)) clipped to the
+    total, lo uniform. The schedule is autotuned first on the same requests clipped to 2^20 positions
+    (every candidate must reproduce the plain kernel's counts), then the whole batch runs with it as
+    one war_run_batch call (every kernel launch a branch of one hipGraph), then one by one through
+    war_run_range: every count must be identical."""
     import numpy as np
     rng = np.random.Generator(np.random.PCG64(seed))
     qs = []
@@ -388,27 +454,113 @@ def mixed_batch(eng, nreq=240, seed=20261203):
         ln = min(total, int(10 ** rng.uniform(3, 9)))
         lo = int(rng.integers(0, total - ln + 1))
         kw = {"hbins": 128, "lbins": 128}
-        if mode == "canon" and n <= 13 and rng.random() < 1 / 3:
+        if mode == "canon" and n <= 13:
             kw.update(distinct=True, dbins=256)
         qs.append((n, mode, lo, lo + ln, kw))
     npos = sum(hi - lo for _, _, lo, hi, _ in qs)
-    eng.run_batch(qs[:8])                                   # graphs of the batch shapes captured
+    sample = [(n, m, lo, min(hi, lo + (1 << 20)), kw) for n, m, lo, hi, kw in qs]
     t0 = time.perf_counter()
-    got, launches = eng.run_batch(qs)
-    wall = time.perf_counter() - t0
+    tun = eng.autotune_batch(sample, budget_ms=30000)
+    t_tune = time.perf_counter() - t0
+    log("mixed batch: autotuned in %.1f s: %s" % (t_tune, tun))
+    tuned = [(n, m, lo, hi, dict(kw, tuning=tun)) for n, m, lo, hi, kw in qs]
+    res = {}
+    for name, batch in (("default", qs), ("tuned", tuned)):
+        eng.run_batch(batch)                                # graphs of the batch's shape captured
+        t0 = time.perf_counter()
+        got, launches = eng.run_batch(batch)
+        res[name] = (got, launches, time.perf_counter() - t0)
     t = 0.0
-    for q, g in zip(qs, got):
+    for q, g, g2 in zip(qs, res["tuned"][0], res["default"][0]):
         n, mode, lo, hi, kw = q
         t1 = time.perf_counter()
         p = eng.run_range(n, mode, lo, hi, **kw)
         t += time.perf_counter() - t1
         for k in ("cyc", "done", "maxfin", "turns", "hist", "lenhist", "oncycle", "distinct_minkey", "dhist"):
-            assert p[k] == g[k], (q, k)
-    return {"workload": "%d requests, n uniform 5..15, full or canon (distinct for 1/3 of canon n<=13), length "
-                        "log-uniform [1e3, 1e9] clipped, numpy PCG64 seed %d: %d positions" % (nreq, seed, npos),
-            "value": npos / wall, "unit": "positions/s", "launches": launches, "wall_s": wall,
+            assert p[k] == g[k] == g2[k], (q, k)
+    got, launches, wall = res["tuned"]
+    return {"workload": "%d requests, n uniform 5..15, full or canon (canonical n <= 13 with distinct + dhist), "
+                        "length floor(10^U(3, 9)) clipped, numpy PCG64 seed %d: %d positions" % (nreq, seed, npos),
+            "value": npos / wall, "unit": "positions/s", "graph_launches": launches, "wall_s": wall,
+            "tuning": tun, "autotune_s": t_tune,
+            "autotune_sample": "the same requests clipped to 2^20 positions each; candidates checked against the "
+                               "plain kernel's counts",
+            "default_tuning_value": npos / res["default"][2], "default_graph_launches": res["default"][1],
             "one_by_one_persistent_positions_per_s": npos / t,
-            "parity": "every count of every request identical in the batch and alone"}
+            "parity": "every count of every request identical in the tuned batch, the default batch and alone"}
+
+
+def host_cpus():
+    """The host's CPUs as this process may use them: nproc, the affinity set, the cgroup CPU quota
+    and the model name."""
+    info = {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)),
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()
+            if q != "max":
+                quota = int(q) / int(per)
+    except (OSError, ValueError):
+        pass
+    info["cgroup_cpus"] = quota
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    info["cpu_model"] = model
+    usable = info["affinity"]
+    if quota:
+        usable = min(usable, max(1, int(quota)))
+    info["usable"] = usable
+    return info
+
+
+def cpu_legs(eng, n13_chunks=64):
+    """The CPU baseline: the oracle (C++/OpenMP restatement of cpu/war.cpp, cpu/war_all.cpp) on every
+    CPU this process may use (the affinity set within the cgroup quota), on bounded samples of the
+    same workloads: n = 13 deals (64 chunks of 2^22 ranks spread over 13!, seed 13), all n = 9 deals,
+    and n = 14 canonical positions (64 chunks of 2^20 q-ranks spread over 13!, seed 14); every chunk
+    is compared with the GPU's counts."""
+    hc = host_cpus()
+    threads = hc["usable"]
+    n13 = math.factorial(13)
+    ch = chunks_spread(n13, n13_chunks, 1 << 22, 13)
+    cres, cel = cpu_baseline(13, "full", ch, threads)
+    for (lo, hi), c in zip(ch, cres):
+        g = eng.run_range(13, "full", lo, hi, hbins=256)
+        assert (g["cyc"], g["maxfin"], g["turns"], g["hist"]) == (c["cyc"], c["maxfin"], c["turns"], c["hist"])
+    c2 = sum(hi - lo for lo, hi in ch) / cel
+    reps = 20
+    r9, el9 = cpu_baseline(9, "full", [(0, math.factorial(9))] * reps, threads)
+    assert r9[0]["cyc"] == 218680 and r9[0]["maxfin"] == 28
+    q14 = math.factorial(13)
+    qch = chunks_spread(q14, 64, 1 << 20, 14)
+    pch = [(13 * lo, 13 * hi) for lo, hi in qch]
+    cres, el14 = cpu_baseline(14, "canon", pch, threads)
+    for (lo, hi), c in zip(pch, cres):
+        g = eng.run_range(14, "canon", lo, hi, hbins=256)
+        assert (g["cyc"], g["maxfin"], g["turns"], g["hist"]) == (c["cyc"], c["maxfin"], c["turns"], c["hist"])
+    c3 = sum(hi - lo for lo, hi in pch) / el14
+    out = {"value": c2, "unit": "deals/s", "cores": threads, "kind": "port",
+           "sample": "%d chunks of 2^22 n=13 deal ranks spread over 13! (seed 13), %d deals, each chunk equal to "
+                     "the GPU's counts" % (len(ch), len(ch) << 22),
+           "host": hc,
+           "threads_note": "OpenMP threads = the CPUs this process may use (affinity set within the cgroup quota)",
+           "c1_n9_full": {"value": reps * math.factorial(9) / el9, "unit": "deals/s",
+                          "sample": "all 9! deals, %d times; a(9) = 218680, longest 28" % reps},
+           "c3_n14_canon": {"value": c3, "unit": "positions/s",
+                            "sample": "64 chunks of 2^20 q-ranks (13 * 2^20 positions each) spread over 13! (seed "
+                                      "14), each chunk equal to the GPU's counts",
+                            "gpu_over_cpu": None}}
+    log("cpu baseline (%d threads of %s): n13 %.3e deals/s, n9 %.3e, n14 canon %.3e positions/s"
+        % (threads, hc["cpu_model"], c2, out["c1_n9_full"]["value"], c3))
+    return out
 
 
 if __name__ == "__main__":

@@ -84,6 +84,7 @@ struct ReqHost {
     std::string err;
     size_t bytes = 0;
     int layout = 4;                 // pile layout the request runs in (4, 5 or 6)
+    u64 share = 0;                  // serving: positions given to the request in the current step
     bool reach = false;             // war_reach: search each distinct cycle's predecessors for a deal
     u64 reach_cap = 0;
     war_reach_result *rres = nullptr;
@@ -107,7 +108,7 @@ struct QueueBufs {
 };
 
 constexpr u64 MIN_TILE = 64, MAX_TILE = 8192;
-constexpr int MAXB = 16;                 // launches one batch graph holds (one branch each at most)
+constexpr int MAXB = 32;                 // launches one batch graph holds (one branch each at most)
 constexpr size_t SLOT_BYTES = JOBS_OFF + MAXJOBS * 32;   // one launch's job header + jobs
 constexpr int FLOW_MIN_FREE = 1;        // flow kernel: refill at a checkpoint once this many lanes are free
 constexpr u64 FLOW_BATCH = 1ull << 32;  // flow kernel: positions per batch
@@ -980,17 +981,19 @@ int run_request_locked(Engine &E, ReqHost &h) {
     return rc;
 }
 
-// One group's ranges (one pile layout, rule, detector and walk flag) as multi-request persistent
-// launches. With `blocks`, 4-bit deal ranges without walks are split at blocks of 24 orders and
-// their whole blocks run on the kernel's deal-block variant (lane runs of 24 deals), a launch of
-// their own: worth it for whole requests (war_run_batch), not for the serving loop's small steps,
-// where the second launch per step cost more than it saved (measured). Each set of ranges is
-// packed, in order, into as few launches as the batch capacity allows (FLOW_BATCH positions on
-// the flow kernel, the queue capacity with cycle walks or the staged pipeline; MAXJOBS ranges).
+// One group's ranges (one pile layout, rule, detector and walk flag) planned as multi-request
+// persistent launches, all later branches of one graph (exec_plan). With `blocks_apart`, 4-bit deal
+// ranges without walks are split at blocks of 24 orders and their whole blocks run on the kernel's
+// deal-block variant (lane runs of 24 deals), a launch of their own. On the flow kernel, canonical
+// ranges of one n (when they hold 2^26 positions or more) take a launch of their own whose lane runs
+// are whole orders from split 0, as a single request's do. Each set of ranges is packed, in order,
+// into as few launches as the batch capacity allows (FLOW_BATCH positions on the flow kernel, the
+// queue capacity with cycle walks or the staged pipeline; MAXJOBS ranges).
 struct GroupRange { u32 slot, n, mode; u64 lo, hi; };
 int plan_group(Engine &E, int W, int rule, int det, bool walk, const war_tuning &gt, int hb, int lb,
                const std::vector<GroupRange> &rs, bool blocks_apart, std::vector<Launch> *plan) {
     std::vector<GroupRange> rest, blocks;
+    std::map<u32, std::vector<GroupRange>> canon;   // canonical ranges by n, from an order boundary on
     war_tuning bt = gt;
     {
         const u64 per = std::max<u64>(1, ((u64)gt.ranks_per_refill / 64 + 12) / 24) * 24;
@@ -999,6 +1002,17 @@ int plan_group(Engine &E, int W, int rule, int det, bool walk, const war_tuning 
     const bool blk_ok =
         blocks_apart && W == 4 && !walk && gt.stage_kind == 3 && (bt.ranks_per_refill / 64) % 24 == 0;
     for (const GroupRange &x : rs) {
+        if (gt.stage_kind == 3 && x.mode == WAR_MODE_CANON && x.n >= 3) {
+            // canonical positions as a single request runs them: a launch per n whose lane runs are
+            // whole orders starting at split 0, so the lanes of a wave step to their next order
+            // together; the part of an order before the first boundary runs with the rest
+            const u64 m = x.n - 1u, a = (x.lo + m - 1) / m * m;
+            if (a < x.hi) {
+                if (x.lo < a) rest.push_back({x.slot, x.n, x.mode, x.lo, a});
+                canon[x.n].push_back({x.slot, x.n, x.mode, a, x.hi});
+                continue;
+            }
+        }
         if (blk_ok && x.mode == WAR_MODE_FULL && x.n >= 5) {
             const u64 a = (x.lo + 23) / 24 * 24, b = x.hi / 24 * 24;
             if (a < b) {
@@ -1041,8 +1055,27 @@ int plan_group(Engine &E, int W, int rule, int det, bool walk, const war_tuning 
         if (!rc) rc = flush();
         return rc;
     };
+    // a bucket too small to fill the GPU a few times over joins the rest (its own launch would cost
+    // more than the aligned runs save)
+    for (auto it = canon.begin(); it != canon.end();) {
+        u64 tot = 0;
+        for (const GroupRange &x : it->second) tot += x.hi - x.lo;
+        if (tot < (1ull << 26)) {
+            rest.insert(rest.end(), it->second.begin(), it->second.end());
+            it = canon.erase(it);
+        } else {
+            ++it;
+        }
+    }
     int rc = pack(blocks, bt, true);
     if (!rc) rc = pack(rest, gt, false);
+    for (auto it = canon.begin(); it != canon.end() && !rc; ++it) {
+        const u64 m = it->first - 1u;
+        war_tuning ct = gt;
+        const u64 per = std::max<u64>(1, ((u64)gt.ranks_per_refill / 64 + m / 2) / m) * m;
+        if (64 * per <= MAX_TILE) ct.ranks_per_refill = (int)(64 * per);
+        rc = pack(it->second, ct, false);
+    }
     return rc;
 }
 
@@ -1163,6 +1196,33 @@ void serve_loop(Engine *Ep) {
             active.push_back(h);
         }
         if (active.empty()) continue;
+        // the step's positions: a quarter to the oldest request with work left (none starves), the
+        // rest shortest remaining first, so a request whose rest fits finishes in this step and a
+        // step's launches hold a few large ranges rather than a sliver of every request
+        {
+            std::vector<std::pair<u64, ReqHost *>> order;
+            ReqHost *oldest = nullptr;
+            for (auto &h : active) {
+                h->share = 0;
+                if (h->next >= h->rq.hi) continue;
+                order.push_back({h->rq.hi - h->next, h.get()});
+                if (!oldest || h->id < oldest->id) oldest = h.get();
+            }
+            u64 budget = E.step_positions;
+            if (oldest) {
+                oldest->share = std::min<u64>(oldest->rq.hi - oldest->next, budget / 4);
+                budget -= oldest->share;
+            }
+            std::sort(order.begin(), order.end(), [](const std::pair<u64, ReqHost *> &a, const std::pair<u64, ReqHost *> &b) {
+                return a.first != b.first ? a.first < b.first : a.second->id < b.second->id;
+            });
+            for (auto &o : order) {
+                if (!budget) break;
+                const u64 more = std::min<u64>(o.first - o.second->share, budget);
+                o.second->share += more;
+                budget -= more;
+            }
+        }
         // one launch per (layout, rule, detector, cycle walks) group: requests without walks keep
         // the kernel without them
         std::vector<Launch> plan;
@@ -1175,19 +1235,16 @@ void serve_loop(Engine *Ep) {
                 std::vector<std::shared_ptr<ReqHost>> grp;
                 for (auto &h : active)
                     if (h->layout == W && h->rq.rule == rule && (h->rq.detector == WAR_DET_ALT) == (alt != 0) &&
-                        (h->dev.walk != 0) == (wk != 0) && h->next < h->rq.hi)
+                        (h->dev.walk != 0) == (wk != 0) && h->next < h->rq.hi && h->share)
                         grp.push_back(h);
                 if (grp.empty()) continue;
                 war_tuning gt;
                 resolve_tuning(E, grp[0]->rq, gt);
-                const u64 tile = (u64)gt.ranks_per_refill;
-                u64 share = std::max<u64>(tile, E.step_positions / grp.size());
-                share = (share + tile - 1) / tile * tile;
                 std::vector<GroupRange> rs;
                 int hb = 0, lb = 0;
                 bool walk = false;
                 for (auto &h : grp) {
-                    const u64 hi = std::min<u64>(h->rq.hi, h->next + share);
+                    const u64 hi = std::min<u64>(h->rq.hi, h->next + h->share);
                     rs.push_back({(u32)h->slot, (u32)h->rq.n, (u32)h->rq.mode, h->next, hi});
                     h->next = hi;
                     hb = std::max(hb, h->rq.hbins); lb = std::max(lb, h->rq.lbins);
@@ -1195,7 +1252,7 @@ void serve_loop(Engine *Ep) {
                 }
                 const int det = alt ? WAR_DET_ALT : WAR_DET_BRENT;
                 const size_t np = plan.size();
-                rc = plan_group(E, W, rule, det, walk, gt, hb, lb, rs, false, &plan);
+                rc = plan_group(E, W, rule, det, walk, gt, hb, lb, rs, true, &plan);
                 if (rc) {
                     plan.resize(np);
                     for (auto &h : grp) { h->status = rc; h->err = t_err; h->next = h->rq.hi; }
@@ -1674,7 +1731,7 @@ int war_autotune_batch(const war_request *reqs, int32_t count, double budget_ms,
                        war_tuning *best) {
     t_err.clear();
     if (!g_eng) return fail(60, "war_init has not been called");
-    if (count < 1 || count > MAXSLOTS - 8 || !reqs) return fail(1, "autotune: 1 to %d requests", MAXSLOTS - 8);
+    if (count < 1 || count > MAXSLOTS || !reqs) return fail(1, "autotune: 1 to %d requests", MAXSLOTS);
     Engine &E = *g_eng;
     for (int i = 0; i < count; i++) {
         int rc = check_request(&reqs[i]);

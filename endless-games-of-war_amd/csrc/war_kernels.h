@@ -1150,6 +1150,30 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
     for (int b = 0; b < FB; b++) fb[b] = 0;
     u64 lturns = 0;            // per lane: sum of its finished games' lengths / detection ages
     u32 lmax = 0;              // per lane: longest ending game
+    // (MULTI: both belong to the request the wave's accumulator follows, acc.req)
+    // Fold the per-lane sums into the wave's accumulator (wave sums; every lane calls this).
+    auto fold_lanes = [&]() {
+        u64 tsum = lturns;
+        u32 tmax = lmax;
+        for (int off = 32; off > 0; off >>= 1) {
+            tsum += __shfl_xor(tsum, off);
+            const u32 o = __shfl_xor(tmax, off);
+            tmax = o > tmax ? o : tmax;
+        }
+        acc.turns += tsum;
+        acc.maxfin = tmax > acc.maxfin ? tmax : acc.maxfin;
+        acc.dirty = true;
+        lturns = 0;
+        lmax = 0;
+    };
+    // MULTI: make the accumulator follow request r (folding the per-lane sums of the previous one
+    // first); every lane calls this
+    auto switch_to = [&](u32 r) {
+        if (r != acc.req) {
+            fold_lanes();
+            acc.select(r, reqs);
+        }
+    };
     Zh<W> z = make_zh<W>(A.zero);
     Zt zt = make_zt(A.zero);
 
@@ -1186,7 +1210,7 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
                 // a position that is over before it starts (only with fewer than 2 cards): length 0
                 if (n < 2u) {
                     e0 = gm;
-                    if constexpr (MULTI) acc.select(preq, reqs);
+                    if constexpr (MULTI) switch_to(preq);
                     acc.ended((u32)__popcll(e0), 0);
                 }
                 alive |= gm & ~e0;
@@ -1203,12 +1227,34 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
         for (int b = 0; b < FB; b++) { if (in_mask(fb[b])) fin += 1u << b; fb[b] = 0; }
         const u32 age = fin - birth, lam = fin - T_.R;
         if constexpr (MULTI) {
-            // by request: wave sums of the group's ages, one LDS add per lane for the histograms
-            for (u64 rem = endm | cycm; rem;) {
+            // the games of the request the accumulator follows go to the per-lane sums as in a
+            // single-request batch; only when some game belongs to another request are the sums
+            // folded and the other requests accounted group by group (wave sums, one LDS add per
+            // lane), the accumulator then following the last of them
+            const u64 ended = endm | cycm;
+            const u64 own = ballot(greq == acc.req) & ended;
+            const u64 rest = ended & ~own;
+            if (own) {
+                if (in_mask(own)) lturns += age;
+                if (in_mask(endm & own)) {
+                    lmax = age > lmax ? age : lmax;
+                    if (acc.lb) atomicAdd(&acc.ll[age < acc.lb - 1 ? age : acc.lb - 1], 1u);
+                }
+                if (acc.hb && in_mask(cycm & own)) atomicAdd(&acc.lh[lam < acc.hb - 1 ? lam : acc.hb - 1], 1u);
+                acc.done += __popcll(own); acc.cyc += __popcll(cycm & own); acc.dirty = true;
+                if constexpr (WALK) {
+                    if (reqs[acc.req].walk && (cycm & own)) {
+                        St<W> m = T_.t;
+                        if constexpr (W == 4) m = flip_piles(m);
+                        acc.pushed += push_cycles<W>(A.cq, TP, cycm & own, m, reqs[acc.req].n, acc.req, lam, gpos, ghome);
+                    }
+                }
+            }
+            for (u64 rem = rest; rem;) {
                 const u32 r = lanev(greq, (u32)(__ffsll((long long)rem) - 1));
                 const u64 mr = ballot(greq == r) & rem;
                 rem &= ~mr;
-                acc.select(r, reqs);
+                switch_to(r);
                 acc.lanes(endm & mr, cycm & mr, age, lam);
                 if constexpr (WALK) {
                     if (reqs[r].walk && (cycm & mr)) {
@@ -1331,7 +1377,7 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
                     if constexpr (MULTI) {
                         const u32 r = lanev(greq, (u32)(__ffsll((long long)rem) - 1));
                         mr = ballot(greq == r) & rem;
-                        acc.select(r, reqs);
+                        switch_to(r);
                     }
                     rem &= ~mr;
                     acc.undet += __popcll(mr); acc.done += __popcll(mr); acc.dirty = true;
@@ -1353,7 +1399,7 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
                     if constexpr (MULTI) {
                         const u32 r = lanev(greq, (u32)(__ffsll((long long)rem) - 1));
                         mr = ballot(greq == r) & rem;
-                        acc.select(r, reqs);
+                        switch_to(r);
                         cp = reqs[r].cap;
                     }
                     rem &= ~mr;
@@ -1365,18 +1411,7 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
         }
         if (more && (u32)__popcll(~alive) >= min_dead) refill();
     }
-    if constexpr (!MULTI) {    // MULTI batches accounted turns and longest games at each span end
-        u64 tsum = lturns;
-        u32 tmax = lmax;
-        for (int off = 32; off > 0; off >>= 1) {
-            tsum += __shfl_xor(tsum, off);
-            const u32 o = __shfl_xor(tmax, off);
-            tmax = o > tmax ? o : tmax;
-        }
-        acc.turns += tsum;
-        acc.maxfin = tmax > acc.maxfin ? tmax : acc.maxfin;
-        acc.dirty = true;
-    }
+    fold_lanes();
     acc.flush(reqs);
     if (WALK && acc.pushed && lane == 0) atomicAdd((unsigned long long *)&A.pipe->ctotal, (unsigned long long)acc.pushed);
 }
@@ -1436,6 +1471,9 @@ __global__ __launch_bounds__(256) void k_plain(PlainArgs<W> P) {
             St<W> t = s;
             Zh<W> z = make_zh<W>(P.zero);
             if constexpr (DET == 0) {
+                // the generic turn with its own end compare: measured faster here than turn_top, the
+                // flow kernel's form (n = 13: 4.64e9 against 3.81e9 deals/s), so the plain kernel,
+                // the baseline of the persistent/plain ratio, keeps it
                 u32 power = 1, l = 0;
                 for (;;) {
                     turn<W, RULE>(s, C, z);
