@@ -1304,7 +1304,16 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
             // scalar operations and the branch; the scalar unit is busy about 70 % of the time here
             u64 cand = ballot(s.a == T_.t.a);
             if constexpr (DET == FLOW_BRENT) {
-                if (tix == 0u) cand &= ~newm;               // a new game's first turn: no repeat
+                // Brent's windows of the tortoise at ages 1 and 3 see only cycles of length <= 2
+                // and <= 4, and no rule-0 War cycle is that short: over one cycle each player plays
+                // every card it holds (its pile would otherwise come back with another card on top),
+                // so a cycle is at least max(cards of player 1, of player 2) >= ceil(n/2) long,
+                // which is 5 or more from n = 9 on, and the rule-0 cycles of n <= 8 are 6 and 8
+                // long (every canonical position). So under rule 0 a new game's tortoise stays at
+                // its start until age 7 and its repeat test is masked until then: every detection,
+                // and so `turns`, is Brent's. Rule 1 has cycles of 2 and 4 turns (n = 3, 5) and
+                // keeps every window.
+                if (tix < (RULE == 0 ? 7u : 1u)) cand &= ~newm;
             }
             if constexpr (DET == FLOW_ALT) {
                 // a repeated winner on a turn of odd age (tix even: births fall on span starts)
@@ -1333,9 +1342,9 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
                 alive &= ~ev;
             }
             if constexpr (DET == FLOW_BRENT) {
-                // Brent's tortoise moves at ages 2^k - 1: turn 2^k - 2 of a lane's first span; turn
-                // SPAN - 2 of a later span when age + 1 is a power of 2
-                if (((tix + 2u) & (tix + 1u)) == 0u) {           // folded: the loop is unrolled
+                // Brent's tortoise moves at ages 2^k - 1 (from 7 on under rule 0, above): turn 2^k - 2 of a
+                // lane's first span; turn SPAN - 2 of a later span when age + 1 is a power of 2
+                if (((tix + 2u) & (tix + 1u)) == 0u && tix >= (RULE == 0 ? 6u : 0u)) {   // unrolled: folded
                     u64 jm = newm & alive;
                     if (tix == SPAN - 2u) {
                         const u32 age1 = clk0 + tix + 2u - birth;    // age + 1
