@@ -27,7 +27,8 @@ every step. One step = one whole enumeration. Beside it, on the same GPU in the 
                  n = 13 deals, all n = 9 deals and sampled n = 14 canonical chunks.
 
   python bench.py [--gpus 1] [--steps 3] [--warmup 1]
-                  [--only main|brent13|staged13|canon14|canon14walk|plain13|plaincanon14|serving|mixed|copy]
+                  [--only main|brent13|staged13|canon14|canon14walk|plain13|plaincanon14|serving|serveburst|
+                          mixed|mixedbatch|copy]
                   [--no-extras]
 """
 import argparse
@@ -115,7 +116,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--only", choices=["main", "brent13", "staged13", "canon14", "canon14walk", "plain13",
-                                       "plaincanon14", "serving", "mixed", "copy"], default=None,
+                                       "plaincanon14", "serving", "serveburst", "mixed", "mixedbatch", "copy"],
+                    default=None,
                     help="run one workload only (for profiling); copy: a 1 GiB device-to-device copy that "
                          "calibrates the HBM byte counters")
     ap.add_argument("--no-extras", action="store_true")
@@ -191,8 +193,21 @@ def main():
     if a.only == "serving":
         print(json.dumps(serving(eng)), flush=True)
         return
+    if a.only == "serveburst":
+        qs, _ = serving_queries()
+        res, wall = serve_burst(eng, qs)
+        log("serving burst: %d queries, %.3e positions/s" % (len(qs), sum(hi - lo for _, _, lo, hi, _ in qs) / wall))
+        return
     if a.only == "mixed":
         print(json.dumps(mixed_batch(eng)), flush=True)
+        return
+    if a.only == "mixedbatch":
+        qs = mixed_queries()
+        eng.run_batch(qs)
+        t0 = time.perf_counter()
+        got, launches = eng.run_batch(qs)
+        log("mixed batch: %.3e positions/s, %d graph launch(es)"
+            % (sum(hi - lo for _, _, lo, hi, _ in qs) / (time.perf_counter() - t0), launches))
         return
 
     # ---- headline: n = 13 full enumeration as the drop-in runs it (flow kernel, alternation detector)
@@ -345,14 +360,10 @@ def pct(lat, p):
     return lat[min(len(lat) - 1, max(0, int(math.ceil(p * len(lat))) - 1))]
 
 
-def serving(eng, nreq=512, seed=20261205, fractions=(0.25, 0.5, 0.9), slots=64):
-    """The serving workload: `nreq` queries, n uniform over 7..17, mode full or canonical (canonical
-    n <= 13 with distinct), length floor(10^U(3, 10)) clipped to the total, lo uniform (numpy PCG64).
-    (1) The queries one by one, synchronously, through the persistent kernel (the parity reference of
-    every later run). (2) All submitted at once (saturation): aggregate positions/s and the saturated
-    request rate. (3) Poisson arrivals at fractions of that rate: p50 / p99 latency (submit to
-    result) per rate. (4) The queries one by one through the plain kernel. Every result must equal
-    its one-by-one persistent result (tolerance 0); the first difference stops the bench."""
+def serving_queries(nreq=512, seed=20261205):
+    """The serving workload's queries (numpy PCG64): n uniform over 7..17, full or canonical
+    (canonical n <= 13 with distinct), length floor(10^U(3, 10)) clipped to the total, lo uniform.
+    Returns them and the generator (its later draws time the arrivals)."""
     import numpy as np
     rng = np.random.Generator(np.random.PCG64(seed))
     qs = []
@@ -366,6 +377,30 @@ def serving(eng, nreq=512, seed=20261205, fractions=(0.25, 0.5, 0.9), slots=64):
         if mode == "canon" and n <= 13:
             kw.update(distinct=True)
         qs.append((n, mode, lo, lo + ln, kw))
+    return qs, rng
+
+
+def serve_burst(eng, qs, slots=64, step=1 << 26):
+    """All queries submitted at once to the serving loop; returns (results, wall seconds)."""
+    eng.serve_start(max_slots=slots, step_positions=step)
+    t0 = time.perf_counter()
+    try:
+        ids = [eng.submit(n, mode, lo, hi, **kw) for n, mode, lo, hi, kw in qs]
+        res = [eng.wait(i) for i in ids]
+    finally:
+        eng.serve_stop()
+    return res, time.perf_counter() - t0
+
+
+def serving(eng, nreq=512, seed=20261205, fractions=(0.25, 0.5, 0.9), slots=64):
+    """The serving workload: `nreq` queries, n uniform over 7..17, mode full or canonical (canonical
+    n <= 13 with distinct), length floor(10^U(3, 10)) clipped to the total, lo uniform (numpy PCG64).
+    (1) The queries one by one, synchronously, through the persistent kernel (the parity reference of
+    every later run). (2) All submitted at once (saturation): aggregate positions/s and the saturated
+    request rate. (3) Poisson arrivals at fractions of that rate: p50 / p99 latency (submit to
+    result) per rate. (4) The queries one by one through the plain kernel. Every result must equal
+    its one-by-one persistent result (tolerance 0); the first difference stops the bench."""
+    qs, rng = serving_queries(nreq, seed)
     npos = sum(hi - lo for _, _, lo, hi, _ in qs)
     keys = ("cyc", "done", "maxfin", "turns", "hist", "distinct_minkey", "undetermined")
     alone, alone_lat = [], []
@@ -437,13 +472,9 @@ def serving(eng, nreq=512, seed=20261205, fractions=(0.25, 0.5, 0.9), slots=64):
     return out
 
 
-def mixed_batch(eng, nreq=256, seed=20261203):
-    """The mixed-n single-launch workload: seeded numpy PCG64, `nreq` requests, n uniform over 5..15,
-    full or canonical (canonical n <= 13 with distinct + dhist), length floor(10^U(3, 9)) clipped to the
-    total, lo uniform. The schedule is autotuned first on the same requests clipped to 2^20 positions
-    (every candidate must reproduce the plain kernel's counts), then the whole batch runs with it as
-    one war_run_batch call (every kernel launch a branch of one hipGraph), then one by one through
-    war_run_range: every count must be identical."""
+def mixed_queries(nreq=256, seed=20261203):
+    """The mixed-n batch's requests (numpy PCG64): n uniform over 5..15, full or canonical (canonical
+    n <= 13 with distinct + dhist), length floor(10^U(3, 9)) clipped to the total, lo uniform."""
     import numpy as np
     rng = np.random.Generator(np.random.PCG64(seed))
     qs = []
@@ -457,6 +488,17 @@ def mixed_batch(eng, nreq=256, seed=20261203):
         if mode == "canon" and n <= 13:
             kw.update(distinct=True, dbins=256)
         qs.append((n, mode, lo, lo + ln, kw))
+    return qs
+
+
+def mixed_batch(eng, nreq=256, seed=20261203):
+    """The mixed-n single-launch workload: seeded numpy PCG64, `nreq` requests, n uniform over 5..15,
+    full or canonical (canonical n <= 13 with distinct + dhist), length floor(10^U(3, 9)) clipped to the
+    total, lo uniform. The schedule is autotuned first on the same requests clipped to 2^20 positions
+    (every candidate must reproduce the plain kernel's counts), then the whole batch runs with it as
+    one war_run_batch call (every kernel launch a branch of one hipGraph), then one by one through
+    war_run_range: every count must be identical."""
+    qs = mixed_queries(nreq, seed)
     npos = sum(hi - lo for _, _, lo, hi, _ in qs)
     sample = [(n, m, lo, min(hi, lo + (1 << 20)), kw) for n, m, lo, hi, kw in qs]
     t0 = time.perf_counter()

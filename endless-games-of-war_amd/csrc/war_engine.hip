@@ -815,6 +815,21 @@ int plan_dispatch(Engine &E, int rule, std::vector<Job> &jobs, const war_tuning 
                 : plan_launch<W, 0>(E, jobs, t, hb, lb, walk, multi, det, blk, out);
 }
 
+// The plain kernel for a request's detector, with the explicit-cap variant where the request has one.
+template <int W, int RULE>
+void launch_plain(int det, bool capped, unsigned blocks, size_t lds, hipStream_t st, const PlainArgs<W> &P) {
+    if (RULE == 0 && det == WAR_DET_STREAK) {
+        if (capped) hipLaunchKernelGGL((k_plain<W, 0, 1, true>), dim3(blocks), dim3(256), lds, st, P);
+        else hipLaunchKernelGGL((k_plain<W, 0, 1>), dim3(blocks), dim3(256), lds, st, P);
+    } else if (RULE == 0 && det == WAR_DET_ALT) {
+        if (capped) hipLaunchKernelGGL((k_plain<W, 0, 2, true>), dim3(blocks), dim3(256), lds, st, P);
+        else hipLaunchKernelGGL((k_plain<W, 0, 2>), dim3(blocks), dim3(256), lds, st, P);
+    } else {
+        if (capped) hipLaunchKernelGGL((k_plain<W, RULE, 0, true>), dim3(blocks), dim3(256), lds, st, P);
+        else hipLaunchKernelGGL((k_plain<W, RULE, 0>), dim3(blocks), dim3(256), lds, st, P);
+    }
+}
+
 template <int W, int RULE>
 int run_plain_batch(Engine &E, ReqHost &h, u64 lo, u64 hi, const war_tuning &t, float *ms) {
     const bool walk = h.dev.walk != 0;
@@ -837,12 +852,7 @@ int run_plain_batch(Engine &E, ReqHost &h, u64 lo, u64 hi, const war_tuning &t, 
     HIPCHK(hipEventRecord(E.ev0, E.st));
     hipLaunchKernelGGL(k_setup, dim3(256), dim3(256), 0, E.st, E.pipe, (const JobHdr *)E.hdr,
                        walk ? qbufs<W>(E).ccnt : (u32 *)nullptr);
-    if (RULE == 0 && h.rq.detector == WAR_DET_STREAK)
-        hipLaunchKernelGGL((k_plain<W, 0, 1>), dim3((unsigned)blocks), dim3(256), lds, E.st, P);
-    else if (RULE == 0 && h.rq.detector == WAR_DET_ALT)
-        hipLaunchKernelGGL((k_plain<W, 0, 2>), dim3((unsigned)blocks), dim3(256), lds, E.st, P);
-    else
-        hipLaunchKernelGGL((k_plain<W, RULE, 0>), dim3((unsigned)blocks), dim3(256), lds, E.st, P);
+    launch_plain<W, RULE>(h.rq.detector, h.rq.step_cap != 0, (unsigned)blocks, lds, E.st, P);
     HIPCHK(hipGetLastError());
     if (walk) {
         WalkArgs<W> WA{};
@@ -1549,12 +1559,7 @@ int war_run_range_noalloc(const war_request *req, war_result *res) {
                 PlainArgs<WW> P{};
                 P.lo = lo; P.hi = hi; P.per = per; P.hb = (u32)r.hbins; P.lb = (u32)r.lbins; P.tile_pos = PLAIN_CYC_TILE;
                 P.R = d; P.req = 0; P.pipe = pipe;
-                if (RR == 0 && r.detector == WAR_DET_STREAK)
-                    hipLaunchKernelGGL((k_plain<WW, 0, 1>), dim3((unsigned)blocks), dim3(256), lds, E.st, P);
-                else if (RR == 0 && r.detector == WAR_DET_ALT)
-                    hipLaunchKernelGGL((k_plain<WW, 0, 2>), dim3((unsigned)blocks), dim3(256), lds, E.st, P);
-                else
-                    hipLaunchKernelGGL((k_plain<WW, RR, 0>), dim3((unsigned)blocks), dim3(256), lds, E.st, P);
+                launch_plain<WW, RR>(r.detector, r.step_cap != 0, (unsigned)blocks, lds, E.st, P);
             };
             using I4 = std::integral_constant<int, 4>;
             using I5 = std::integral_constant<int, 5>;

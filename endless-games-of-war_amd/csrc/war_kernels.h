@@ -1448,8 +1448,10 @@ __device__ __forceinline__ u32 gcd_u(u32 a, u32 b) { while (b) { const u32 t = a
 __device__ __forceinline__ u32 lcm_u(u32 a, u32 b) { return a / gcd_u(a, b) * b; }
 
 // DET: 0 = Brent, 1 = the streak test (RULE 0 only; no cycle lengths, so no histogram or walks),
-// 2 = the alternation detector (RULE 0 only; every output).
-template <int W, int RULE, int DET>
+// 2 = the alternation detector (RULE 0 only; every output). CAPPED: the request has an explicit step
+// cap (P.R.cap); without one the cap is the compile-time safety stop (a run-time cap in the loop
+// measured 10 % slower).
+template <int W, int RULE, int DET, bool CAPPED = false>
 __global__ __launch_bounds__(256) void k_plain(PlainArgs<W> P) {
     typedef typename Lay<W>::T T;
     extern __shared__ u32 lds[];
@@ -1468,7 +1470,7 @@ __global__ __launch_bounds__(256) void k_plain(PlainArgs<W> P) {
     u64 cyc = 0, done = 0, maxfin = 0, turns = 0, undet = 0, pushed = 0;
     // step caps: Brent and the alternation detector stop a game unresolved after `cap` turns (the
     // request's, else the 2^24-turn safety stop); the streak test after more than `scap`
-    const u32 cap = P.R.cap ? P.R.cap : AGE_CAP, scap = P.R.cap ? P.R.cap : STREAK_CAP;
+    const u32 cap = CAPPED ? P.R.cap : AGE_CAP, scap = CAPPED ? P.R.cap : STREAK_CAP;
     for (u32 j = 0; j < cnt; j++) {
         St<W> s = (mode == 2) ? canon<W>(q, (int)n, k) : deal<W>(q, (int)n);
         const bool take = (mode == 1) ? top_card_to_p1<W>(q, (int)n) : true;

@@ -2,6 +2,14 @@
 
     python -m war_amd.profsum trace  <results.db>      per-kernel totals and per-batch stage times
     python -m war_amd.profsum pmc    <results.db>      counters per kernel (summed over dispatches)
+    python -m war_amd.profsum issue  <results.db> [turns]
+                                                       VALU issue read from the SQ counters (the
+                                                       pass of SQ_INSTS_VALU, SQ_ACTIVE_INST_VALU,
+                                                       SQ_ACTIVE_INST_VALU2, SQ_THREAD_CYCLES_VALU,
+                                                       SQ_INSTS_SALU, GRBM_GUI_ACTIVE): per SIMD
+                                                       quad-cycle, lanes active, scalar issue; with
+                                                       turns, the useful share (turns x 17 / 64 /
+                                                       VALU instructions)
 
 A batch of the persistent pipeline starts with k_setup, so the trace view also lists, for the
 first and last batches, the duration of each of its kernels (the flow kernel, or the stage
@@ -65,7 +73,41 @@ def pmc(db):
     return "\n".join(out)
 
 
+def issue(db, turns=None, simds=1024, cus=256, xcds=8, valu_per_turn=17):
+    """VALU issue of each kernel of the run, from the SQ counters summed over its dispatches.
+    GRBM_GUI_ACTIVE is summed over the XCDs (divided by `xcds` for the kernel's clocks);
+    SQ_ACTIVE_INST_VALU counts the VALU instructions issued and SQ_ACTIVE_INST_VALU2 the SIMD
+    quad-cycles in which two issued, so the quad-cycles with an issue are their difference."""
+    c = sqlite3.connect(db)
+    d = collections.OrderedDict()
+    for kn, cn, v in c.execute("select kernel_name, counter_name, value from counters_collection"):
+        k = _short(kn)
+        d.setdefault(k, collections.defaultdict(float))[cn] += v
+    out = ["| kernel | clocks | VALU per SIMD quad-cycle | quad-cycles: two / one / none | lanes active per "
+           "VALU instruction | busy x lanes | SALU per CU per clock | useful share |",
+           "|---|---:|---:|---:|---:|---:|---:|---:|"]
+    top = max((v.get("GRBM_GUI_ACTIVE", 0) for v in d.values()), default=0)
+    for k, v in d.items():
+        # the kernels that hold the run (not the runtime's copies and fills)
+        if not v.get("SQ_INSTS_VALU") or v.get("GRBM_GUI_ACTIVE", 0) < 0.01 * top:
+            continue
+        clocks = v["GRBM_GUI_ACTIVE"] / xcds
+        qc = simds * clocks / 4
+        iv = v["SQ_INSTS_VALU"]
+        two = v.get("SQ_ACTIVE_INST_VALU2", 0) / qc
+        busy = (v.get("SQ_ACTIVE_INST_VALU", iv) - v.get("SQ_ACTIVE_INST_VALU2", 0)) / qc
+        lanes = v.get("SQ_THREAD_CYCLES_VALU", 0) / 64 / iv
+        useful = "%.3f" % (turns * valu_per_turn / 64 / iv) if turns else "—"
+        out.append("| `%s` | %.4g | %.3f | %.3f / %.3f / %.3f | %.3f | %.3f | %.3f | %s |" % (
+            k, clocks, iv / qc, two, busy - two, 1 - busy, lanes, busy * lanes, v.get("SQ_INSTS_SALU", 0) / cus / clocks,
+            useful))
+    return "\n".join(out)
+
+
 if __name__ == "__main__":
-    if len(sys.argv) != 3 or sys.argv[1] not in ("trace", "pmc"):
+    if len(sys.argv) < 3 or sys.argv[1] not in ("trace", "pmc", "issue"):
         sys.exit(__doc__)
-    print(trace(sys.argv[2]) if sys.argv[1] == "trace" else pmc(sys.argv[2]))
+    if sys.argv[1] == "issue":
+        print(issue(sys.argv[2], int(sys.argv[3]) if len(sys.argv) > 3 else None))
+    else:
+        print(trace(sys.argv[2]) if sys.argv[1] == "trace" else pmc(sys.argv[2]))
