@@ -356,7 +356,7 @@ int req_collect(Engine &E, ReqHost &h) {
         if (h.rres) {
             war_reach_result *q = h.rres;
             q->classes = buf[C_MINKEY]; q->reached = buf[C_RYES]; q->not_reached = buf[C_RNO];
-            q->undetermined = buf[C_RUND]; q->device_ms = h.dev_ms;
+            q->undetermined = buf[C_RUND]; q->device_ms = h.dev_ms; q->max_depth = buf[C_RDEPTH];
             if (q->nolen && q->nbins > 0) {
                 std::vector<u64> nl(4096);
                 HIPCHK(hipMemcpy(nl.data(), h.dev.nolen, 4096 * sizeof(u64), hipMemcpyDeviceToHost));

@@ -27,7 +27,7 @@ namespace war {
 
 enum {
     C_CYC = 0, C_DONE, C_MAXFIN, C_TURNS, C_ONCYCLE, C_MINKEY, C_NONSTD, C_UNCHK, C_UNDET,
-    C_HASH, C_HOVF, C_QOVF, C_RYES, C_RNO, C_RUND, C_N = 16
+    C_HASH, C_HOVF, C_QOVF, C_RYES, C_RNO, C_RUND, C_RDEPTH, C_N = 16
 };
 
 constexpr u32 AGE_CAP = 1u << 24;      // safety stop of the last stage (never reached by a valid game)
@@ -1635,7 +1635,7 @@ __device__ __forceinline__ bool undo_win4(const St<4> &s, u32 w, u32 n, St<4> &o
     return true;
 }
 template <int RULE>
-__device__ int reach_search(const St<4> &s0, u32 lam, u32 n, u32 C, u64 cap, u64 *stk, Zh<4> &z) {
+__device__ int reach_search(const St<4> &s0, u32 lam, u32 n, u32 C, u64 cap, u64 *stk, Zh<4> &z, u32 &depth) {
     St<4> prev = s0;
     for (u32 i = 0; i + 1u < lam; i++) turn<4, RULE>(prev, C, z);
     St<4> t = s0;
@@ -1647,6 +1647,7 @@ __device__ int reach_search(const St<4> &s0, u32 lam, u32 n, u32 C, u64 cap, u64
             if (!undo_win4(t, w, n, q) || same<4>(q, prev)) continue;
             u32 sp = 0;
             stk[0] = q.a | ((u64)(q.sa >> 2) << 60); stk[1] = q.b; sp = 1;
+            depth = depth > 1u ? depth : 1u;
             while (sp) {
                 sp--;
                 St<4> u;
@@ -1658,6 +1659,7 @@ __device__ int reach_search(const St<4> &s0, u32 lam, u32 n, u32 C, u64 cap, u64
                     if (!undo_win4(u, w2, n, r)) continue;
                     if (sp == REACH_DEPTH) return -1;
                     stk[2 * sp] = r.a | ((u64)(r.sa >> 2) << 60); stk[2 * sp + 1] = r.b; sp++;
+                    depth = sp > depth ? sp : depth;
                 }
             }
         }
@@ -1805,8 +1807,10 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs<W> A) {
                             if (hash_insert(R, K)) atomicAdd((unsigned long long *)&R.cnt[C_HASH], 1ull);
                             if (rep && R.reach && RULE == 0) {
                                 const u64 gtid = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+                                u32 depth = 0;
                                 const int res = reach_search<RULE>(s0, lam, n, C, R.reach_cap,
-                                                                   A.stack + gtid * (2u * REACH_DEPTH), z);
+                                                                   A.stack + gtid * (2u * REACH_DEPTH), z, depth);
+                                atomicMax((unsigned long long *)&R.cnt[C_RDEPTH], (unsigned long long)depth);
                                 atomicAdd((unsigned long long *)&R.cnt[res == 1 ? C_RYES : res == 0 ? C_RNO : C_RUND], 1ull);
                                 if (res == 0) atomicAdd((unsigned long long *)&R.nolen[lam < 4095u ? lam : 4095u], 1ull);
                             }

@@ -51,7 +51,7 @@ class WarResult(ctypes.Structure):
 class WarReachResult(ctypes.Structure):
     _fields_ = [("classes", ctypes.c_uint64), ("reached", ctypes.c_uint64), ("not_reached", ctypes.c_uint64),
                 ("undetermined", ctypes.c_uint64), ("nolen", ctypes.POINTER(ctypes.c_uint64)), ("nbins", ctypes.c_int32),
-                ("device_ms", ctypes.c_double)]
+                ("device_ms", ctypes.c_double), ("max_depth", ctypes.c_uint64)]
 
 
 class WarPilePosition(ctypes.Structure):

@@ -195,7 +195,7 @@ class Engine:
         check(self.lib.war_reach(int(n), int(lo), int(hi), int(cap), ctypes.byref(r)))
         return {"classes": r.classes, "reached": r.reached, "not_reached": r.not_reached,
                 "undetermined": r.undetermined, "nolen": [int(nl[i]) for i in range(nbins)],
-                "device_ms": r.device_ms}
+                "device_ms": r.device_ms, "max_depth": r.max_depth}
 
     def cycle_info(self, positions, rule=0, cap=10**12):
         """positions: [(pile1, pile2), ...] of card values, top card first. Returns per position

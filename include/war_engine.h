@@ -144,6 +144,8 @@ typedef struct war_reach_result {
     uint64_t *nolen; int32_t nbins;  /* caller-owned: cycle lengths of the cycles not reached
                                         (last bin = overflow), or NULL                           */
     double   device_ms;
+    uint64_t max_depth;        /* the most entries any search's stack held (at most 64; a full
+                                  stack is counted in undetermined)                              */
 } war_reach_result;
 
 /* war_cycle_info: one position of up to 255 cards (cpu/cycle_length.c). */

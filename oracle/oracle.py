@@ -133,17 +133,22 @@ def cycle_info(p1, p2, rule=0, cap=10**12):
             "lam": lam.value if rc == 1 else 0, "mu": mu.value if rc == 1 else 0}
 
 
-def reach(n, threads=0, cap=200000000):
-    """Distinct cycles of n cards and whether ordinary deals enter them (cpu/reach_cycles.c)."""
+def reach(n, threads=0, cap=200000000, lo=0, hi=None):
+    """Distinct cycles met by the canonical positions [lo, hi) of n cards (each at its minimum-key
+    position) and whether ordinary deals enter them (cpu/reach_cycles.c); max_depth: the deepest
+    backward-search stack."""
     L = lib()
-    f = L.orc_reach
-    f.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, _U64P, _U64P, _U64P, _U64P, _U64P]
+    f = L.orc_reach_range
+    f.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, ctypes.c_uint64, _U64P, _U64P, _U64P,
+                  _U64P, _U64P, _U64P]
     f.restype = ctypes.c_int
-    c, y, no, u = (ctypes.c_uint64(0) for _ in range(4))
+    if hi is None:
+        hi = total(n, CANON)
+    c, y, no, u, d = (ctypes.c_uint64(0) for _ in range(5))
     nl = (ctypes.c_uint64 * 4096)()
-    rc = f(int(n), int(threads), int(cap), ctypes.byref(c), ctypes.byref(y), ctypes.byref(no), ctypes.byref(u),
-           ctypes.cast(nl, _U64P))
+    rc = f(int(n), int(lo), int(hi), int(threads), int(cap), ctypes.byref(c), ctypes.byref(y), ctypes.byref(no),
+           ctypes.byref(u), ctypes.cast(nl, _U64P), ctypes.byref(d))
     if rc:
         raise ValueError("oracle: %s" % L.orc_last_error().decode())
     return {"classes": c.value, "reached": y.value, "not_reached": no.value, "undetermined": u.value,
-            "nolen": [int(nl[i]) for i in range(4096)]}
+            "nolen": [int(nl[i]) for i in range(4096)], "max_depth": d.value}

@@ -596,8 +596,10 @@ static inline bool deal_sized(const St<4>& s, int n) {
     const int hi = (n + 1) / 2, lo = n / 2;
     return (s.la == hi && s.lb == lo) || (s.la == lo && s.lb == hi);
 }
-// 1 = some deal-size position leads into the cycle through s0, 0 = none, -1 = capped
-static int reached_by_deal(const St<4>& s0, u64 lam, u64 cap, int n) {
+// 1 = some deal-size position leads into the cycle through s0, 0 = none, -1 = capped. *depth: the
+// most positions the search's stack held at once (no limit here, as the reference's 2^18 is never
+// approached).
+static int reached_by_deal(const St<4>& s0, u64 lam, u64 cap, int n, u64* depth) {
     St<4> prev = s0;                              // the loop predecessor of t
     for (u64 i = 0; i + 1 < lam; i++) step<4>(prev, 0);
     St<4> t = s0;
@@ -610,11 +612,13 @@ static int reached_by_deal(const St<4>& s0, u64 lam, u64 cap, int n) {
             if (!undo_win(t, w, q) || same<4>(q, prev)) continue;
             todo.clear();
             todo.push_back(q);
+            if (*depth < 1) *depth = 1;
             while (!todo.empty()) {
                 St<4> u = todo.back(); todo.pop_back();
                 if (deal_sized(u, n)) return 1;
                 if (++visited > cap) return -1;
                 for (int w2 = 0; w2 < 2; w2++) { St<4> r; if (undo_win(u, w2, r)) todo.push_back(r); }
+                if (todo.size() > *depth) *depth = todo.size();
             }
         }
         prev = t;
@@ -622,21 +626,28 @@ static int reached_by_deal(const St<4>& s0, u64 lam, u64 cap, int n) {
     }
     return 0;
 }
-extern "C" int orc_reach(int n, int threads, u64 cap, u64* classes, u64* yes, u64* no, u64* und, u64* nolen) {
+// Reach over canonical positions [lo, hi) (classes taken at their minimum-key position, as the
+// engine's war_reach does); *maxdepth: the deepest search stack.
+extern "C" int orc_reach_range(int n, u64 lo, u64 hi, int threads, u64 cap, u64* classes, u64* yes, u64* no,
+                               u64* und, u64* nolen, u64* maxdepth) {
     if (n < 2 || n > 15) { g_err = "reach needs 2 <= n <= 15"; return 1; }
+    if (lo > hi || hi > range_total(n, ORC_CANON)) { g_err = "range outside the canonical positions"; return 2; }
     if (threads > 0) omp_set_num_threads(threads);
     const int m = n - 1;
-    const u64 F = fact_u64(m);
-    u64 C = 0, Y = 0, N = 0, U = 0;
+    u64 C = 0, Y = 0, N = 0, U = 0, D = 0;
     std::vector<u64> NL(4096, 0);
+    const u64 r0 = lo / (u64)m, r1 = (hi + (u64)m - 1) / (u64)m;
     #pragma omp parallel
     {
         std::vector<u64> nl(4096, 0);
+        u64 d = 0;
         #pragma omp for schedule(dynamic, 64) reduction(+ : C, Y, N, U)
-        for (long long r = 0; r < (long long)F; r++) {
+        for (long long r = (long long)r0; r < (long long)r1; r++) {
             int p[16];
             unrank((u64)r, m, p);
             for (int k = 0; k < m; k++) {
+                const u64 pos = (u64)r * (u64)m + (u64)k;
+                if (pos < lo || pos >= hi) continue;
                 St<4> s = canon_pos<4>(p, n, k);
                 u64 rounds = 0, lam = 0;
                 if (!brent<4>(s, 0, rounds, lam, nullptr)) continue;
@@ -652,18 +663,27 @@ extern "C" int orc_reach(int n, int threads, u64 cap, u64* classes, u64* yes, u6
                 }
                 if (key4(s) != K) continue;                       // not the class's representative
                 C++;
-                const int res = reached_by_deal(s, lam, cap, n);
+                const int res = reached_by_deal(s, lam, cap, n, &d);
                 if (res == 1) Y++;
                 else if (res == 0) { N++; nl[lam < 4095 ? lam : 4095]++; }
                 else U++;
             }
         }
         #pragma omp critical
-        for (int i = 0; i < 4096; i++) NL[(size_t)i] += nl[(size_t)i];
+        {
+            for (int i = 0; i < 4096; i++) NL[(size_t)i] += nl[(size_t)i];
+            if (d > D) D = d;
+        }
     }
     *classes = C; *yes = Y; *no = N; *und = U;
     if (nolen) for (int i = 0; i < 4096; i++) nolen[i] = NL[(size_t)i];
+    if (maxdepth) *maxdepth = D;
     return 0;
+}
+
+extern "C" int orc_reach(int n, int threads, u64 cap, u64* classes, u64* yes, u64* no, u64* und, u64* nolen) {
+    if (n < 2 || n > 15) { g_err = "reach needs 2 <= n <= 15"; return 1; }
+    return orc_reach_range(n, 0, range_total(n, ORC_CANON), threads, cap, classes, yes, no, und, nolen, nullptr);
 }
 
 // Lexicographic unrank, exposed for the permutation tests.

@@ -33,7 +33,7 @@ STRUCTS = {
     "war_request": (_lib.WarRequest, ["n", "mode", "rule", "kernel", "lo", "hi", "hbins", "lbins", "dbins",
                                       "distinct", "shape_blk", "detector", "tuning", "step_cap", "flags"]),
     "war_reach_result": (_lib.WarReachResult, ["classes", "reached", "not_reached", "undetermined", "nolen",
-                                               "nbins", "device_ms"]),
+                                               "nbins", "device_ms", "max_depth"]),
     "war_pile_position": (_lib.WarPilePosition, ["n1", "n2", "p1", "p2"]),
     "war_cycle_info": (_lib.WarCycleInfo, ["status", "turns", "lam", "mu"]),
     "war_result": (_lib.WarResult, ["cyc", "done", "maxfin", "turns", "oncycle", "distinct_minkey",

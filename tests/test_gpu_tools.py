@@ -5,6 +5,8 @@ the published counts for n = 5..14 and the oracle."""
 import random
 import re
 
+import os
+
 import pytest
 
 import cycle_length as CL
@@ -72,3 +74,37 @@ def test_reach_ranges_add_up(eng, orc):
         assert sum(p[k] for p in parts) == whole[k]
     with pytest.raises(WarError):
         eng.reach(16)
+
+
+@pytest.mark.parametrize("n,lo,length", [
+    (11, 0, None),
+    (13, 344_881_152, 1 << 24),          # 30240 classes, stack depth 4 (the oracle's)
+    (15, 57_973_563_648, 1 << 23),       # 14652 classes
+    (15, 64_076_044_032, 1 << 23),       # 24288 classes, stack depth 9
+])
+def test_reach_slices_against_oracle(eng, orc, n, lo, length):
+    """war_reach on canonical slices against the oracle's unbounded-stack search (n = 15 has no
+    published line): classes, reached, not reached, the cycle lengths not reached, and the deepest
+    search stack, which stays far below the kernel's 64 entries (a full stack would be counted
+    undetermined)."""
+    hi = orc.total(n, "canon") if length is None else lo + length
+    print("case reach slice", n, lo, hi, flush=True)
+    got = eng.reach(n, lo, hi)
+    want = orc.reach(n, lo=lo, hi=hi)
+    for k in ("classes", "reached", "not_reached", "undetermined", "nolen", "max_depth"):
+        assert got[k] == want[k], (n, lo, hi, k, got[k], want[k])
+    assert got["classes"] > 0 and got["undetermined"] == 0 and got["max_depth"] <= 16
+
+
+@pytest.mark.parametrize("n", [14, 15])
+def test_reach_stack_depth_whole_range(eng, n):
+    """Over the whole canonical range of n = 14 and 15 the deepest backward-search stack stays below
+    the kernel's 64 entries, so no search is cut short by it (a full stack would count as
+    undetermined). n = 15 (about two minutes: 48,304,896 classes, deepest stack 9) runs with
+    WAR_SLOW_TESTS=1."""
+    if n == 15 and not os.environ.get("WAR_SLOW_TESTS"):
+        pytest.skip("the whole n = 15 reach takes about two minutes; set WAR_SLOW_TESTS=1")
+    print("case reach depth", n, flush=True)
+    r = eng.reach(n)
+    print("reach", n, "classes", r["classes"], "max_depth", r["max_depth"], flush=True)
+    assert r["undetermined"] == 0 and r["max_depth"] < 64, r["max_depth"]
