@@ -110,7 +110,6 @@ struct QueueBufs {
 constexpr u64 MIN_TILE = 64, MAX_TILE = 8192;
 constexpr int MAXB = 32;                 // launches one batch graph holds (one branch each at most)
 constexpr size_t SLOT_BYTES = JOBS_OFF + MAXJOBS * 32;   // one launch's job header + jobs
-constexpr int FLOW_MIN_FREE = 1;        // flow kernel: refill at a checkpoint once this many lanes are free
 constexpr u64 FLOW_BATCH = 1ull << 32;  // flow kernel: positions per batch
 constexpr u32 PLAIN_CYC_TILE = 1024;    // home-tile size of the plain kernel's cycle records
 
@@ -171,7 +170,7 @@ struct Engine {
     u64 *na = nullptr;
     u64 *na_host = nullptr;
     std::vector<int> free_slots;
-    uint32_t slot_cap[MAXSLOTS] = {};  // each slot's explicit step cap (ReqDev::cap), for JobHdr::capck
+    uint32_t slot_cap[MAXSLOTS] = {};  // each slot's explicit step cap (ReqDev::cap), for the kernel variant
     std::mutex mu;                   // serialises all GPU work of the engine
 
     // serving
@@ -511,17 +510,22 @@ void launch_refill(int flags, int blocks, size_t lds, hipStream_t st, const Stag
 
 // The flow kernel for a batch's detector (WAR_DET_*): Brent for either rule, the alternation and
 // pair-run detectors for rule 0 (pair-run: single requests, layouts 4 and 5). BLK: the variant for
-// 4-bit deals whose jobs and lane runs are whole blocks of 24 deals.
-template <int W, int RULE, bool WALK, bool MULTI, bool BLK = false>
-void launch_flow(int det, int blocks, size_t lds, hipStream_t st, const StageArgs<W> &A) {
+// 4-bit deals whose jobs and lane runs are whole blocks of 24 deals; CAP: with the step-cap test.
+template <int W, int RULE, bool WALK, bool MULTI, bool BLK, bool CAP>
+void launch_flow_c(int det, int blocks, size_t lds, hipStream_t st, const StageArgs<W> &A) {
     if (det == WAR_DET_BRENT) {
-        hipLaunchKernelGGL((k_flow<W, RULE, FLOW_BRENT, WALK, MULTI, BLK>), dim3(blocks), dim3(256), lds, st, A);
+        hipLaunchKernelGGL((k_flow<W, RULE, FLOW_BRENT, WALK, MULTI, BLK, CAP>), dim3(blocks), dim3(256), lds, st, A);
     } else if constexpr (RULE == 0) {
         if (det == WAR_DET_ALT)
-            hipLaunchKernelGGL((k_flow<W, 0, FLOW_ALT, WALK, MULTI, BLK>), dim3(blocks), dim3(256), lds, st, A);
+            hipLaunchKernelGGL((k_flow<W, 0, FLOW_ALT, WALK, MULTI, BLK, CAP>), dim3(blocks), dim3(256), lds, st, A);
         else if constexpr (!MULTI && W != 6)
-            hipLaunchKernelGGL((k_flow<W, 0, FLOW_PAIR, WALK, false, BLK>), dim3(blocks), dim3(256), lds, st, A);
+            hipLaunchKernelGGL((k_flow<W, 0, FLOW_PAIR, WALK, false, BLK, CAP>), dim3(blocks), dim3(256), lds, st, A);
     }
+}
+template <int W, int RULE, bool WALK, bool MULTI, bool BLK = false>
+void launch_flow(int det, bool cap, int blocks, size_t lds, hipStream_t st, const StageArgs<W> &A) {
+    if (cap) launch_flow_c<W, RULE, WALK, MULTI, BLK, true>(det, blocks, lds, st, A);
+    else launch_flow_c<W, RULE, WALK, MULTI, BLK, false>(det, blocks, lds, st, A);
 }
 
 // Enqueue one batch's kernels on stream st (inside a capture): k_setup, then the flow kernel or the
@@ -543,22 +547,21 @@ void record_launch(Engine &E, const GraphKey &key, hipStream_t st, Pipe *pipe, J
         A.pipe = pipe; A.hdr = hdr; A.jobs = jobsd; A.reqs = reqs;
         A.per = (u32)key.per; A.tile_pos = 64u * (u32)key.per;
         A.hb = (u32)key.hb; A.lb = (u32)key.lb;
-        A.flow_min = (u32)key.cap0;
         constexpr bool CW = W != 6;     // cycle walks exist for layouts 4 and 5
         if (key.walk) {
             A.cq = cyc_queue<W>(E);
             if constexpr (CW) {
-                if (key.multi) launch_flow<W, RULE, true, true>(key.det, blocks, lds, st, A);
-                else launch_flow<W, RULE, true, false>(key.det, blocks, lds, st, A);
+                if (key.multi) launch_flow<W, RULE, true, true>(key.det, key.cap0 != 0, blocks, lds, st, A);
+                else launch_flow<W, RULE, true, false>(key.det, key.cap0 != 0, blocks, lds, st, A);
             }
         } else if (key.blk) {
             if constexpr (W == 4) {
-                if (key.multi) launch_flow<4, RULE, false, true, true>(key.det, blocks, lds, st, A);
-                else launch_flow<4, RULE, false, false, true>(key.det, blocks, lds, st, A);
+                if (key.multi) launch_flow<4, RULE, false, true, true>(key.det, key.cap0 != 0, blocks, lds, st, A);
+                else launch_flow<4, RULE, false, false, true>(key.det, key.cap0 != 0, blocks, lds, st, A);
             }
         } else {
-            if (key.multi) launch_flow<W, RULE, false, true>(key.det, blocks, lds, st, A);
-            else launch_flow<W, RULE, false, false>(key.det, blocks, lds, st, A);
+            if (key.multi) launch_flow<W, RULE, false, true>(key.det, key.cap0 != 0, blocks, lds, st, A);
+            else launch_flow<W, RULE, false, false>(key.det, key.cap0 != 0, blocks, lds, st, A);
         }
     }
     u32 a0 = 0;
@@ -648,8 +651,11 @@ int plan_launch(Engine &E, std::vector<Job> &jobs, const war_tuning &t, int hb, 
     u64 tiles = 0;
     for (Job &j : jobs) { j.tile_base = tiles; tiles += (j.hi - j.lo + tile_pos - 1) / tile_pos; }
     const int kind = flow ? 3 : (t.stage_kind == 3 ? 2 : t.stage_kind);
-    // flow: cap0 holds the refill threshold (free lanes of 64), brent_cap's stage windows unused
-    const int cap0 = flow ? FLOW_MIN_FREE : t.brent_cap;
+    // flow: cap0 = 1 when some request of the launch has an explicit step cap (the kernel variant
+    // with the cap test); brent_cap's stage windows are unused
+    bool capped = false;
+    for (const Job &j : jobs) capped |= E.slot_cap[j.req] != 0u;
+    const int cap0 = flow ? (capped ? 1 : 0) : t.brent_cap;
     out->key = GraphKey{W, RULE, 1, t.waves_per_cu, t.ranks_per_refill / 64, cap0, walk ? 1 : 0, multi ? 1 : 0,
                         round_pow2(hb), round_pow2(lb), kind, det, (blk && flow && W == 4 && !walk) ? 1 : 0};
     if ((!flow || walk) && (tiles > qbufs<W>(E).tmax || tiles * tile_pos > qbufs<W>(E).cap))
@@ -666,7 +672,6 @@ void stage_jobs(Engine &E, const Launch &L, char *sp) {
     const u64 tp = 64u * (u64)L.key.per;
     for (const Job &j : L.jobs) tiles += (j.hi - j.lo + tp - 1) / tp;
     h.njobs = (u32)L.jobs.size(); h.req0 = L.jobs[0].req; h.ntiles = tiles;
-    for (const Job &j : L.jobs) h.capck |= E.slot_cap[j.req] ? 1u : 0u;
     memcpy(sp, &h, sizeof h);
     memcpy(sp + JOBS_OFF, L.jobs.data(), L.jobs.size() * sizeof(Job));
 }
@@ -1524,14 +1529,14 @@ int war_run_range_noalloc(const war_request *req, war_result *res) {
         char *js = (char *)(hs + 16);
         JobHdr h{};
         const u64 tp = (u64)bt.ranks_per_refill;
-        h.njobs = 1; h.req0 = 0; h.ntiles = (hi - lo + tp - 1) / tp; h.capck = r.step_cap ? 1u : 0u;
+        h.njobs = 1; h.req0 = 0; h.ntiles = (hi - lo + tp - 1) / tp;
         Job j{};
         j.lo = lo; j.hi = hi; j.tile_base = 0; j.req = 0;
         memcpy(js, &h, sizeof h);
         memcpy(js + JOBS_OFF, &j, sizeof j);
         HIPCHK(hipMemcpyAsync(hdr, js, JOBS_OFF + sizeof j, hipMemcpyHostToDevice, E.st));
         const bool blk = deal_blocks(r, W, lo, hi, bt);
-        GraphKey key{W, r.rule, 1, bt.waves_per_cu, bt.ranks_per_refill / 64, FLOW_MIN_FREE, 0, 0, round_pow2(r.hbins),
+        GraphKey key{W, r.rule, 1, bt.waves_per_cu, bt.ranks_per_refill / 64, r.step_cap ? 1 : 0, 0, 0, round_pow2(r.hbins),
                      round_pow2(r.lbins), 3, r.rule ? WAR_DET_BRENT : r.detector, (blk && W == 4) ? 1 : 0};
         HIPCHK(hipEventRecord(E.ev0, E.st));
         record_launch_any(E, key, E.st, pipe, hdr, job, rd);

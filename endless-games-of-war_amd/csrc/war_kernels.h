@@ -48,7 +48,7 @@ struct Job { u64 lo, hi, tile_base; u32 req, pad; };
 struct JobHdr {
     u32 njobs, req0;        // req0: the request of job 0
     u64 ntiles;
-    u32 capck, pad;         // some request of the batch has an explicit step cap (ReqDev::cap)
+    u64 pad;
 };
 
 struct Pipe {
@@ -138,7 +138,6 @@ template <int W> struct StageArgs {
     u64 max_regions;       // regions of tile_pos slots each queue buffer holds
     u32 in_refill;         // the input regions were written by a refill stage (count: nreg[stage-1])
     u32 zero;              // always 0 (see Zh)
-    u32 flow_min;          // flow kernel: refill once this many lanes of 64 are free
 };
 
 // ------------------------------------------------------------------------------------------
@@ -994,6 +993,22 @@ __device__ __forceinline__ St<W> shfl_state(const St<W> &x, u32 src) {
     return r;
 }
 
+// The turn of the span each lane's game ended on, from the bit planes of the end turns: bit b of the
+// turn is this lane's bit of fb[b]. Horner's rule in add-with-carry steps (e = 2e + bit, the bit as
+// the carry in), one VALU instruction per plane instead of a select and an add.
+template <int FB>
+__device__ __forceinline__ u32 end_turn(const u64 (&fb)[FB]) {
+    u32 e = in_mask(fb[FB - 1]) ? 1u : 0u;
+    for (int b = FB - 2; b >= 0; b--) {
+        u32 r;
+        u64 co;                // the carry out, unused
+        asm volatile("v_addc_co_u32_e64 %0, %1, %2, %2, %3" : "=v"(r), "=s"(co) : "v"(e), "s"(fb[b]));
+        (void)co;
+        e = r;
+    }
+    return e;
+}
+
 constexpr int FLOW_BRENT = 0, FLOW_ALT = 2, FLOW_PAIR = 3;
 // The pair-run detector (n = 2 mod 4, rule 0): the reference's loop-shape test
 // (gpu/war_gpu.py:51-73, Theorem 3) finds the cycles of such n made of winner runs of exactly two
@@ -1010,7 +1025,10 @@ constexpr u32 FLOW_SPAN = ALT_K * FLOW_BLOCKS;
 // to one job, so the pool's positions share one request (its n, mode and C change at a tile grab);
 // a game keeps its request and C per lane, and accounting groups the finished lanes by request.
 // BLK: every job is 4-bit deals in whole blocks of 24 orders, and so is every lane run.
-template <int W, int RULE, int DET, bool WALK, bool MULTI, bool BLK = false>
+// CAP: some request of the batch has an explicit step cap (war_request.step_cap), tested at span
+// ends (a variant, not a run-time flag: the flag and the cap held across the loop cost the kernel
+// scalar registers it spills into vector lanes, 1 % of its time at n = 14 canonical).
+template <int W, int RULE, int DET, bool WALK, bool MULTI, bool BLK = false, bool CAP = false>
 __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
     static_assert(!BLK || (W == 4 && !WALK), "deal blocks: 4-bit deals without walks");
     static_assert(DET == FLOW_BRENT || ((DET == FLOW_ALT || DET == FLOW_PAIR) && RULE == 0),
@@ -1033,9 +1051,7 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
     u32 C = c_of<W>(n);
     const u32 per = A.per, TP = A.tile_pos;
     const u64 ntiles = A.hdr->ntiles;
-    const u32 min_dead = A.flow_min;       // refill once this many lanes are free (or none is alive)
-    const bool capck = A.hdr->capck != 0u; // some request of the batch has an explicit step cap
-    const u32 rcap = reqs[req].cap;
+    const u32 rcap = CAP ? reqs[req].cap : 0u;
     u64 *grab = &A.pipe->grab[0];
     u64 nxt = 0;
     if (lane == 0) nxt = atomicAdd((unsigned long long *)grab, 1ull);
@@ -1223,8 +1239,8 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
     };
     // account the games that finished in the span that began at clock clk0 (every lane calls this)
     auto settle = [&](u32 clk0) {
-        u32 fin = clk0 + 1u;
-        for (int b = 0; b < FB; b++) { if (in_mask(fb[b])) fin += 1u << b; fb[b] = 0; }
+        const u32 fin = clk0 + 1u + end_turn<FB>(fb);
+        for (int b = 0; b < FB; b++) fb[b] = 0;
         const u32 age = fin - birth, lam = fin - T_.R;
         if constexpr (MULTI) {
             // the games of the request the accumulator follows go to the per-lane sums as in a
@@ -1397,7 +1413,7 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
         // an explicit step cap (war_request.step_cap), tested at every span end when a request of
         // the batch has one: ages at span ends are multiples of the span, which divides the cap, so
         // a game stops exactly when it has played the cap's turns unresolved, charged those turns
-        if (capck) {
+        if constexpr (CAP) {
             u32 gcap = rcap;
             if constexpr (MULTI) gcap = reqs[greq].cap;
             const u64 um = ballot(gcap != 0u && clk - birth >= gcap) & alive;
@@ -1418,7 +1434,9 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
                 alive &= ~um;
             }
         }
-        if (more && (u32)__popcll(~alive) >= min_dead) refill();
+        // refill once a lane is free (a higher threshold measured within 0.2 %: almost every span
+        // frees several lanes)
+        if (more && ~alive) refill();
     }
     fold_lanes();
     acc.flush(reqs);
