@@ -110,7 +110,8 @@ struct QueueBufs {
 constexpr u64 MIN_TILE = 64, MAX_TILE = 8192;
 constexpr int MAXB = 32;                 // launches one batch graph holds (one branch each at most)
 constexpr size_t SLOT_BYTES = JOBS_OFF + MAXJOBS * 32;   // one launch's job header + jobs
-constexpr u64 FLOW_BATCH = 1ull << 32;  // flow kernel: positions per batch
+constexpr u64 FLOW_BATCH = (1ull << 32) - (1ull << 20);  // flow kernel: positions per batch (below 2^32: the
+                                                        // kernel's per-wave counts are 32-bit)
 constexpr u32 PLAIN_CYC_TILE = 1024;    // home-tile size of the plain kernel's cycle records
 
 // ---- the entry point that allocates nothing (war_run_range_noalloc): its device memory is this

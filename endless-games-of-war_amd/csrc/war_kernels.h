@@ -209,11 +209,14 @@ __device__ __forceinline__ void group_locate(u32 excl, u32 ng, u32 k, u32 &tile,
 }
 
 // ------------------------------------------------------------------------------------------
-// Scalar (wave-uniform) accounting of one request, with wave-private LDS histograms.
+// Scalar (wave-uniform) accounting of one request, with wave-private LDS histograms. The counts
+// are 32-bit (one scalar register each: the flow kernel's span loop has none to spare), as no
+// launch holds 2^32 positions (a staged batch at most 2^28, a flow launch less than 2^32).
 struct WaveAcc {
     u32 req;                   // request slot these totals belong to (UNBOUNDED = none)
     u32 hb, lb;                // that request's bins
-    u64 cyc, done, turns, maxfin, undet;
+    u32 cyc, done, maxfin, undet;
+    u64 turns;
     u64 pushed;                // cycle records this wave wrote (all requests)
     u32 *lh, *ll;              // LDS: hist[hbmax], lenhist[lbmax] of this wave
     bool dirty;
@@ -272,7 +275,7 @@ struct WaveAcc {
         }
         const u32 ne = (u32)__popcll(em), nc = (u32)__popcll(cm);
         done += ne + nc; cyc += nc; turns += sum; dirty = true;
-        maxfin = (u64)mx > maxfin ? (u64)mx : maxfin;
+        maxfin = mx > maxfin ? mx : maxfin;
         if (lb && ie) atomicAdd(&ll[fin < lb - 1 ? fin : lb - 1], 1u);
         if (hb && ic) atomicAdd(&lh[lam < hb - 1 ? lam : hb - 1], 1u);
     }
