@@ -243,8 +243,8 @@ def main():
                      "frac_single_issue": 2.0 * achieved / peak_lane_ops,
                      "turn_mix_ceiling": TURN_MIX_CEILING,
                      "frac_of_turn_mix_ceiling": achieved / peak_lane_ops / TURN_MIX_CEILING,
-                     "kernel": "k_flow<4, 0, alt, deal blocks>: one launch per batch of at most 2^32 deals "
-                               "(whole tiles of 1536: 4294966272 and 1932054528, 2 launches per step)",
+                     "kernel": "k_flow<4, 0, alt, deal blocks>: one launch per batch of fewer than 2^32 deals "
+                               "(whole tiles of 1536: 4293918720 and 1933102080, 2 launches per step)",
                      "traffic_note": "HBM bytes per k_flow launch at this workload from rocprofv3 FETCH_SIZE and "
                                      "WRITE_SIZE (scaled by a 1 GiB copy: x2.0 and x1.0), profiles/"
                                      "k_flow_n13_full_pmc.md; not measured inside this run",
