@@ -1,7 +1,8 @@
 // Host engine and C ABI of libwar_engine.so (declared in include/war_engine.h).
 //
-// One engine per process, bound to one GPU. All GPU work of the engine runs on one HIP stream,
-// guarded by one mutex; the serving loop is a host thread that takes the same mutex per step.
+// One engine per process, bound to one GPU. GPU work runs on the engine's stream, guarded by one
+// mutex; the serving loop is a host thread that takes the same mutex per step and alternates its
+// steps between that stream and a second one, keeping one step in flight while it plans the next.
 //
 // A persistent batch is one hipGraph: k_setup, the FRESH stage, the queue stages and (if any
 // request of the batch needs them) the cycle walks. Graphs are captured once per launch shape and
@@ -85,6 +86,8 @@ struct ReqHost {
     size_t bytes = 0;
     int layout = 4;                 // pile layout the request runs in (4, 5 or 6)
     u64 share = 0;                  // serving: positions given to the request in the current step
+    int inflight = 0;               // serving: launched steps with ranges of this request, not yet waited for
+    bool pooled = false;            // serving: dmem came from (and returns to) the engine's pool
     bool reach = false;             // war_reach: search each distinct cycle's predecessors for a deal
     u64 reach_cap = 0;
     war_reach_result *rres = nullptr;
@@ -141,6 +144,23 @@ struct GraphKey {
     }
 };
 
+// One step context: MAXB slots of pipe counters and job lists on the device, their pinned host
+// staging, the pipe counters read back, a stream, timing events and the batch graphs captured on
+// these buffers (keyed by the shape of their branches). While a step is in flight, `stepped` holds
+// the requests with ranges in it.
+struct StepCtx {
+    Pipe *pipe = nullptr;
+    char *hdr = nullptr;
+    void *stage = nullptr;
+    Pipe *pipe_host = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::map<std::vector<int>, hipGraphExec_t> graphs;
+    size_t nl = 0;              // launches of the step in flight (0: none)
+    bool walk = false;          // ... and whether any of them has cycle walks
+    std::vector<std::shared_ptr<ReqHost>> stepped;
+};
+
 struct Engine {
     int device = 0;
     bool verbose = false;
@@ -159,13 +179,15 @@ struct Engine {
     QueueBufs qb4, qb5;
     u64 *reach_stack = nullptr;     // reach searches: REACH_DEPTH entries per thread of a walk grid
     std::map<GraphKey, hipGraphExec_t> graphs;
-    // batch graphs: MAXB slots of pipe counters and job lists, a stream and a join event per branch
-    Pipe *bpipe = nullptr;
-    char *bhdr = nullptr;
-    void *bstage = nullptr;
+    // batch graphs: two step contexts (the serving loop launches a step on one while the step on
+    // the other runs; everything else uses context 0, whose stream is `st`), and the streams and
+    // events that capture a graph's branches (used only while capturing)
+    StepCtx sc[2];
     hipStream_t bst[MAXB] = {};
     hipEvent_t bfork = nullptr, bjoin[MAXB] = {};
-    std::map<std::vector<int>, hipGraphExec_t> bgraphs;
+    // the serving loop's request memory, kept for reuse by size class (hipFree waits for the whole
+    // device, which would stall the step in flight); released when the loop goes idle
+    std::multimap<size_t, void *> pool;
     // war_run_range_noalloc: the device address of `scratch` on this device, and pinned staging
     // (allocated at init) for its job header, request descriptor and results
     u64 *na = nullptr;
@@ -306,7 +328,23 @@ int layout_of(const war_request &r, const war_tuning &t, int *out) {
 }
 
 // ---- per-request device memory
-int req_alloc(Engine &E, ReqHost &h) {
+// Pool size class of a request's memory: a power of two of at least 64 KiB.
+size_t pool_class(size_t bytes) {
+    size_t c = (size_t)1 << 16;
+    while (c < bytes) c <<= 1;
+    return c;
+}
+
+void pool_release(Engine &E) {
+    for (auto &kv : E.pool) (void)hipFree(kv.second);
+    E.pool.clear();
+}
+
+// The request's device memory (counters | hist | lenhist | dhist | htab), zeroed, and its descriptor
+// in the device table, both on stream st. pooled: memory from the serving loop's pool (a block of the
+// request's size class when one is free), returned there by req_free.
+int req_alloc(Engine &E, ReqHost &h, hipStream_t st = nullptr, bool pooled = false) {
+    if (!st) st = E.st;
     const war_request &r = h.rq;
     const size_t nc = C_N, nh = (size_t)r.hbins, nl = (size_t)r.lbins, nd = (size_t)r.dbins;
     u32 hlog2 = 0;
@@ -319,9 +357,17 @@ int req_alloc(Engine &E, ReqHost &h) {
     const size_t nr = h.reach ? 4096 : 0;
     h.bytes = (nc + nh + nl + nd + nt + nr) * sizeof(u64);
     void *p = nullptr;
-    HIPCHK(hipMalloc(&p, h.bytes));
-    HIPCHK(hipMemsetAsync(p, 0, h.bytes, E.st));
+    h.pooled = pooled;
+    if (pooled) {
+        const size_t c = pool_class(h.bytes);
+        auto it = E.pool.find(c);
+        if (it != E.pool.end()) { p = it->second; E.pool.erase(it); }
+        else HIPCHK(hipMalloc(&p, c));
+    } else {
+        HIPCHK(hipMalloc(&p, h.bytes));
+    }
     h.dmem = (u64 *)p;
+    HIPCHK(hipMemsetAsync(p, 0, h.bytes, st));
     ReqDev &d = h.dev;
     d = ReqDev{};
     d.n = (u32)r.n; d.mode = (u32)r.mode; d.rule = (u32)r.rule;
@@ -339,15 +385,21 @@ int req_alloc(Engine &E, ReqHost &h) {
     d.nolen = nr ? d.dhist + nd + nt : nullptr;
     d.cap = r.step_cap;
     E.slot_cap[h.slot] = r.step_cap;
-    HIPCHK(hipMemcpyAsync(&E.reqs[h.slot], &d, sizeof(ReqDev), hipMemcpyHostToDevice, E.st));
+    HIPCHK(hipMemcpyAsync(&E.reqs[h.slot], &d, sizeof(ReqDev), hipMemcpyHostToDevice, st));
     return 0;
 }
 
-int req_collect(Engine &E, ReqHost &h) {
+// Copy a request's counters and histograms into buf on stream st (the caller synchronises).
+int req_fetch(ReqHost &h, hipStream_t st, std::vector<u64> &buf) {
     const war_request &r = h.rq;
-    std::vector<u64> buf(C_N + (size_t)r.hbins + (size_t)r.lbins + (size_t)r.dbins);
-    HIPCHK(hipMemcpyAsync(buf.data(), h.dmem, buf.size() * sizeof(u64), hipMemcpyDeviceToHost, E.st));
-    HIPCHK(hipStreamSynchronize(E.st));
+    buf.assign(C_N + (size_t)r.hbins + (size_t)r.lbins + (size_t)r.dbins, 0);
+    HIPCHK(hipMemcpyAsync(buf.data(), h.dmem, buf.size() * sizeof(u64), hipMemcpyDeviceToHost, st));
+    return 0;
+}
+
+// The result of a request from its fetched counters, with the checks of a finished request.
+int req_finish(ReqHost &h, const std::vector<u64> &buf) {
+    const war_request &r = h.rq;
     war_result *o = h.res;
     if (o) {
         o->cyc = buf[C_CYC]; o->done = buf[C_DONE]; o->maxfin = buf[C_MAXFIN]; o->turns = buf[C_TURNS];
@@ -382,8 +434,20 @@ int req_collect(Engine &E, ReqHost &h) {
     return 0;
 }
 
+int req_collect(Engine &E, ReqHost &h) {
+    std::vector<u64> buf;
+    int rc = req_fetch(h, E.st, buf);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(E.st));
+    return req_finish(h, buf);
+}
+
 void req_free(Engine &E, ReqHost &h) {
-    if (h.dmem) { (void)hipFree(h.dmem); h.dmem = nullptr; }
+    if (h.dmem) {
+        if (h.pooled) E.pool.insert({pool_class(h.bytes), (void *)h.dmem});
+        else (void)hipFree(h.dmem);
+        h.dmem = nullptr;
+    }
     if (h.slot >= 0) { E.slot_cap[h.slot] = 0; E.free_slots.push_back(h.slot); h.slot = -1; }
 }
 
@@ -612,9 +676,9 @@ void record_launch_any(Engine &E, const GraphKey &key, hipStream_t st, Pipe *pip
     }
 }
 
-int end_capture(Engine &E, hipGraphExec_t *out) {
+int end_capture(Engine &E, hipGraphExec_t *out, hipStream_t st = nullptr) {
     hipGraph_t g = nullptr;
-    hipError_t ce = hipStreamEndCapture(E.st, &g);
+    hipError_t ce = hipStreamEndCapture(st ? st : E.st, &g);
     if (ce != hipSuccess) return fail(30, "graph capture failed: %s", hipGetErrorString(ce));
     HIPCHK(hipGraphInstantiate(out, g, nullptr, nullptr, 0));
     HIPCHK(hipGraphDestroy(g));
@@ -716,36 +780,36 @@ int exec_single(Engine &E, const Launch &L, float *ms) {
     return 0;
 }
 
-// Several planned launches. Flow launches (up to MAXB of them) run as ONE graph: each launch on a
-// branch of its own with its own pipe counters and job list, forked from and joined to the engine's
-// stream; launches with cycle walks of one pile layout share that layout's cycle records and
-// follow each other on one branch. Anything else (staged launches, more than MAXB) runs launch by
-// launch. *ms: device time; *graphs: graph launches made.
-int exec_plan(Engine &E, std::vector<Launch> &plan, double *ms, int *graphs) {
-    size_t nl = 0;
-    for (const Launch &L : plan) nl += L.jobs.empty() ? 0 : 1;
-    if (nl == 0) return 0;
+// Whether a plan runs as ONE graph: every launch a flow launch, at most MAXB of them. *nl: launches.
+bool one_graph(const std::vector<Launch> &plan, size_t *nl) {
+    size_t n = 0;
     bool all_flow = true;
-    for (const Launch &L : plan) all_flow = all_flow && (L.jobs.empty() || L.flow);
-    if (nl == 1 || !all_flow || nl > (size_t)MAXB) {
-        for (const Launch &L : plan) {
-            if (L.jobs.empty()) continue;
-            float m = 0;
-            const int rc = exec_single(E, L, &m);
-            if (ms) *ms += m;
-            if (graphs) ++*graphs;
-            if (rc) return rc;
-        }
-        return 0;
+    for (const Launch &L : plan) {
+        if (L.jobs.empty()) continue;
+        n++;
+        all_flow = all_flow && L.flow;
     }
+    *nl = n;
+    return n > 0 && all_flow && n <= (size_t)MAXB;
+}
+
+// Launch a plan as ONE graph on step context c, without waiting: each launch on a branch of its own
+// with its own pipe counters and job list, forked from and joined to the context's stream; launches
+// with cycle walks of one pile layout share that layout's cycle records and follow each other on one
+// branch. The job lists are copied in first and the pipe counters read back after (into pinned
+// memory), all on the context's stream, between its timing events. The caller has checked
+// one_graph(plan) and that the context has no step in flight.
+int launch_plan(Engine &E, StepCtx &c, std::vector<Launch> &plan) {
     // branches: each launch without walks alone; the launches with walks, one branch per layout
     std::vector<const Launch *> ls;
     for (const Launch &L : plan) if (!L.jobs.empty()) ls.push_back(&L);
     std::vector<std::vector<int>> br;
     int walk_branch[7] = {-1, -1, -1, -1, -1, -1, -1};
+    bool walk = false;
     for (int i = 0; i < (int)ls.size(); i++) {
         const GraphKey &k = ls[(size_t)i]->key;
         if (k.walk) {
+            walk = true;
             int &b = walk_branch[k.W];
             if (b < 0) { b = (int)br.size(); br.emplace_back(); }
             br[(size_t)b].push_back(i);
@@ -764,43 +828,78 @@ int exec_plan(Engine &E, std::vector<Launch> &plan, double *ms, int *graphs) {
         }
     }
     // job lists of every slot, one copy
-    char *sp = (char *)E.bstage;
+    char *sp = (char *)c.stage;
     for (size_t i = 0; i < ls.size(); i++) stage_jobs(E, *ls[i], sp + i * SLOT_BYTES);
-    HIPCHK(hipMemcpyAsync(E.bhdr, sp, ls.size() * SLOT_BYTES, hipMemcpyHostToDevice, E.st));
+    HIPCHK(hipMemcpyAsync(c.hdr, sp, ls.size() * SLOT_BYTES, hipMemcpyHostToDevice, c.st));
     hipGraphExec_t ge = nullptr;
-    auto it = E.bgraphs.find(bkey);
-    if (it != E.bgraphs.end()) {
+    auto it = c.graphs.find(bkey);
+    if (it != c.graphs.end()) {
         ge = it->second;
     } else {
-        HIPCHK(hipStreamBeginCapture(E.st, hipStreamCaptureModeThreadLocal));
-        HIPCHK(hipEventRecord(E.bfork, E.st));
+        HIPCHK(hipStreamBeginCapture(c.st, hipStreamCaptureModeThreadLocal));
+        HIPCHK(hipEventRecord(E.bfork, c.st));
         for (size_t b = 0; b < br.size(); b++) {
             hipStream_t st = E.bst[b];
             HIPCHK(hipStreamWaitEvent(st, E.bfork, 0));
             for (int i : br[b]) {
-                char *hd = E.bhdr + (size_t)i * SLOT_BYTES;
-                record_launch_any(E, ls[(size_t)i]->key, st, E.bpipe + i, (JobHdr *)hd, (Job *)(hd + JOBS_OFF));
+                char *hd = c.hdr + (size_t)i * SLOT_BYTES;
+                record_launch_any(E, ls[(size_t)i]->key, st, c.pipe + i, (JobHdr *)hd, (Job *)(hd + JOBS_OFF));
             }
             HIPCHK(hipEventRecord(E.bjoin[b], st));
-            HIPCHK(hipStreamWaitEvent(E.st, E.bjoin[b], 0));
+            HIPCHK(hipStreamWaitEvent(c.st, E.bjoin[b], 0));
         }
-        const int rc = end_capture(E, &ge);
+        const int rc = end_capture(E, &ge, c.st);
         if (rc) return rc;
-        E.bgraphs[bkey] = ge;
+        c.graphs[bkey] = ge;
     }
-    HIPCHK(hipEventRecord(E.ev0, E.st));
-    HIPCHK(hipGraphLaunch(ge, E.st));
-    HIPCHK(hipEventRecord(E.ev1, E.st));
-    std::vector<Pipe> pp(ls.size());
-    HIPCHK(hipMemcpyAsync(pp.data(), E.bpipe, ls.size() * sizeof(Pipe), hipMemcpyDeviceToHost, E.st));
-    HIPCHK(hipStreamSynchronize(E.st));
-    float m = 0;
-    HIPCHK(hipEventElapsedTime(&m, E.ev0, E.ev1));
-    if (ms) *ms += m;
-    if (graphs) ++*graphs;
-    for (const Pipe &p : pp)
-        if (p.ovf) return fail(33, "persistent pipeline ran out of queue regions (%llu refused)", p.ovf);
+    HIPCHK(hipEventRecord(c.ev0, c.st));
+    HIPCHK(hipGraphLaunch(ge, c.st));
+    HIPCHK(hipEventRecord(c.ev1, c.st));
+    HIPCHK(hipMemcpyAsync(c.pipe_host, c.pipe, ls.size() * sizeof(Pipe), hipMemcpyDeviceToHost, c.st));
+    c.nl = ls.size();
+    c.walk = walk;
     return 0;
+}
+
+// Wait for the step in flight on context c: its device time into *ms (added), and its pipe counters
+// checked. The context is free again afterwards.
+int wait_plan(StepCtx &c, double *ms) {
+    const size_t nl = c.nl;
+    c.nl = 0;
+    c.walk = false;
+    if (!nl) return 0;
+    HIPCHK(hipStreamSynchronize(c.st));
+    float m = 0;
+    HIPCHK(hipEventElapsedTime(&m, c.ev0, c.ev1));
+    if (ms) *ms += m;
+    for (size_t i = 0; i < nl; i++)
+        if (c.pipe_host[i].ovf)
+            return fail(33, "persistent pipeline ran out of queue regions (%llu refused)", c.pipe_host[i].ovf);
+    return 0;
+}
+
+// Several planned launches, synchronously, on context 0. Flow launches (up to MAXB of them) run as
+// ONE graph (launch_plan); anything else (staged launches, more than MAXB) runs launch by launch.
+// *ms: device time; *graphs: graph launches made.
+int exec_plan(Engine &E, std::vector<Launch> &plan, double *ms, int *graphs) {
+    size_t nl = 0;
+    const bool one = one_graph(plan, &nl);
+    if (nl == 0) return 0;
+    if (nl == 1 || !one) {
+        for (const Launch &L : plan) {
+            if (L.jobs.empty()) continue;
+            float m = 0;
+            const int rc = exec_single(E, L, &m);
+            if (ms) *ms += m;
+            if (graphs) ++*graphs;
+            if (rc) return rc;
+        }
+        return 0;
+    }
+    int rc = launch_plan(E, E.sc[0], plan);
+    if (rc) { E.sc[0].nl = 0; (void)hipStreamSynchronize(E.sc[0].st); return rc; }
+    if (graphs) ++*graphs;
+    return wait_plan(E.sc[0], ms);
 }
 
 // One batch of jobs (all of layout W and rule RULE) through the persistent pipeline, synchronously.
@@ -1156,14 +1255,78 @@ int run_batch_locked(Engine &E, std::vector<ReqHost> &hs, int *launches, double 
 }
 
 // ---------------------------------------------------------------------------------------------
-// Serving loop: continuous batching. Each step takes, from every admitted request, its next
-// share of positions (step_positions split evenly, whole tiles), runs each group of one layout,
-// rule, detector and walk flag as planned launches, runs every group's launches as one graph
-// (exec_plan), and retires finished requests. Requests that arrive during a step
-// are admitted at the next one, while slots and device memory allow.
+// Serving loop: continuous batching. Each step takes, from every admitted request, its next share
+// of positions, plans each group of one layout, rule, detector and walk flag as launches, and runs
+// every group's launches as one graph. Steps alternate between the two step contexts and are
+// pipelined: the worker launches step k while step k - 1 still runs, then waits for step k - 1 and
+// retires the requests it completed, so the GPU does not wait for the host between steps (and the
+// two steps' kernels may overlap on the device: a request's counters only take atomic adds, and the
+// ranges of two steps are disjoint). Requests that arrive during a step are admitted at the next
+// one, while slots and device memory allow.
+
+// Retire the active requests with no positions left and no step in flight: their counters are
+// fetched on stream st (one synchronisation), their memory returns to the pool, waiters are woken.
+void retire(Engine &E, std::vector<std::shared_ptr<ReqHost>> &active, hipStream_t st) {
+    std::vector<std::shared_ptr<ReqHost>> done;
+    for (size_t i = 0; i < active.size();) {
+        if (active[i]->next >= active[i]->rq.hi && active[i]->inflight == 0) {
+            done.push_back(active[i]);
+            active.erase(active.begin() + (long)i);
+        } else {
+            i++;
+        }
+    }
+    if (done.empty()) return;
+    std::vector<std::vector<u64>> bufs(done.size());
+    for (size_t i = 0; i < done.size(); i++) {
+        ReqHost &h = *done[i];
+        if (!h.status) {
+            const int c = req_fetch(h, st, bufs[i]);
+            if (c) { h.status = c; h.err = t_err; }
+        }
+    }
+    const hipError_t se = hipStreamSynchronize(st);
+    for (auto &hp : done) {
+        ReqHost &h = *hp;
+        if (!h.status && se != hipSuccess) {
+            h.status = fail(100 + (int)se, "hipStreamSynchronize: %s", hipGetErrorString(se));
+            h.err = t_err;
+        }
+        if (!h.status) {
+            const int c = req_finish(h, bufs[(size_t)(&hp - &done[0])]);
+            if (c) { h.status = c; h.err = t_err; }
+        }
+        req_free(E, h);
+    }
+    std::lock_guard<std::mutex> lk(E.smu);
+    const double t = now_ms();
+    for (auto &h : done) {
+        h->finished = true;
+        h->t_done = t;
+        if (h->res) h->res->latency_ms = h->t_done - h->t_submit;
+    }
+    E.scv.notify_all();
+}
+
+// Wait for the step in flight on context c; the requests it held take its device time (or its
+// failure).
+void finish_step(StepCtx &c) {
+    double dms = 0;
+    const int rc = wait_plan(c, &dms);
+    const std::string err = rc ? t_err : std::string();
+    for (auto &h : c.stepped) {
+        h->dev_ms += dms;        // device time of the steps the request took part in
+        h->inflight--;
+        if (rc && !h->status) { h->status = rc; h->err = err; h->next = h->rq.hi; }
+    }
+    c.stepped.clear();
+}
+
 void serve_loop(Engine *Ep) {
     Engine &E = *Ep;
     std::vector<std::shared_ptr<ReqHost>> active;
+    int cur = 0;          // the context of the next step
+    int prev = -1;        // the context of the step in flight (-1: none)
     for (;;) {
         {
             std::unique_lock<std::mutex> lk(E.smu);
@@ -1171,7 +1334,9 @@ void serve_loop(Engine *Ep) {
             if (!E.serving && E.pending.empty() && active.empty()) break;
         }
         std::lock_guard<std::mutex> g(E.mu);
-        // admit
+        StepCtx &C = E.sc[cur];
+        // admit: a new request's counters are cleared and its descriptor written on the stream of
+        // the step that first runs it
         for (;;) {
             std::shared_ptr<ReqHost> h;
             {
@@ -1200,18 +1365,11 @@ void serve_loop(Engine *Ep) {
                 h->slot = alloc_slot(E);
                 if (h->slot < 0) rc = fail(40, "no free request slot");
             }
-            if (!rc) rc = req_alloc(E, *h);
-            if (rc) {
-                h->status = rc; h->err = t_err; req_free(E, *h);
-                std::lock_guard<std::mutex> lk(E.smu);
-                h->finished = true; h->t_done = now_ms();
-                E.scv.notify_all();
-                continue;
-            }
-            h->next = h->rq.lo;
+            if (!rc) rc = req_alloc(E, *h, C.st, true);
+            if (rc) { h->status = rc; h->err = t_err; }
+            h->next = rc ? h->rq.hi : h->rq.lo;     // a refused request retires at the end of this pass
             active.push_back(h);
         }
-        if (active.empty()) continue;
         // the step's positions: a quarter to the oldest request with work left (none starves), the
         // rest shortest remaining first, so a request whose rest fits finishes in this step and a
         // step's launches hold a few large ranges rather than a sliver of every request
@@ -1243,7 +1401,6 @@ void serve_loop(Engine *Ep) {
         // the kernel without them
         std::vector<Launch> plan;
         std::vector<std::shared_ptr<ReqHost>> stepped;
-        int rc = 0;
         for (int W : {4, 5, 6}) {
             for (int rule : {0, 1})
             for (int alt : {0, 1})
@@ -1268,48 +1425,60 @@ void serve_loop(Engine *Ep) {
                 }
                 const int det = alt ? WAR_DET_ALT : WAR_DET_BRENT;
                 const size_t np = plan.size();
-                rc = plan_group(E, W, rule, det, walk, gt, hb, lb, rs, true, &plan);
+                const int rc = plan_group(E, W, rule, det, walk, gt, hb, lb, rs, true, &plan);
                 if (rc) {
                     plan.resize(np);
                     for (auto &h : grp) { h->status = rc; h->err = t_err; h->next = h->rq.hi; }
-                    rc = 0;
                 } else {
                     stepped.insert(stepped.end(), grp.begin(), grp.end());
                 }
             }
         }
-        // the step's launches, every group's, as one graph
-        {
+        // launch the step: one graph on this context, after the step in flight; staged launches (or
+        // more than one graph holds) run synchronously once the step in flight is done
+        bool launched = false;
+        size_t nl = 0;
+        if (one_graph(plan, &nl)) {
+            bool walk = false;
+            for (const Launch &L : plan) walk = walk || (!L.jobs.empty() && L.key.walk);
+            int rc = 0;
+            // the cycle records of a layout are shared: a step with cycle walks waits for the one in flight
+            if (walk && prev >= 0 && E.sc[prev].walk) {
+                const hipError_t we = hipStreamWaitEvent(C.st, E.sc[prev].ev1, 0);
+                if (we != hipSuccess) rc = fail(100 + (int)we, "hipStreamWaitEvent: %s", hipGetErrorString(we));
+            }
+            if (!rc) rc = launch_plan(E, C, plan);
+            if (rc) {
+                (void)hipStreamSynchronize(C.st);
+                for (auto &h : stepped) { h->status = rc; h->err = t_err; h->next = h->rq.hi; }
+            } else {
+                launched = true;
+                C.stepped = stepped;
+                for (auto &h : stepped) h->inflight++;
+            }
+        } else if (nl) {
+            if (prev >= 0) { finish_step(E.sc[prev]); prev = -1; }
             double dms = 0;
-            rc = exec_plan(E, plan, &dms, nullptr);
+            const int rc = exec_plan(E, plan, &dms, nullptr);
             for (auto &h : stepped) {
-                h->dev_ms += dms;   // device time of the steps the request took part in
+                h->dev_ms += dms;
                 if (rc) { h->status = rc; h->err = t_err; h->next = h->rq.hi; }
             }
-            rc = 0;
         }
-        // retire
-        for (size_t i = 0; i < active.size();) {
-            auto h = active[i];
-            if (h->next >= h->rq.hi) {
-                if (!h->status) {
-                    int c = req_collect(E, *h);
-                    if (c) { h->status = c; h->err = t_err; }
-                }
-                req_free(E, *h);
-                {
-                    std::lock_guard<std::mutex> lk(E.smu);
-                    h->finished = true;
-                    h->t_done = now_ms();
-                    if (h->res) h->res->latency_ms = h->t_done - h->t_submit;
-                    E.scv.notify_all();
-                }
-                active.erase(active.begin() + (long)i);
-            } else {
-                i++;
-            }
-        }
+        // the step before this one: wait for it, then retire what is complete (fetching on its now
+        // idle stream)
+        if (prev >= 0) { finish_step(E.sc[prev]); prev = -1; }
+        retire(E, active, E.sc[cur ^ 1].st);
+        if (launched) { prev = cur; cur ^= 1; }
+        if (active.empty()) pool_release(E);   // nothing in flight either (a step holds active requests)
     }
+}
+
+// Every GPU entry point other than the serving worker waits for the serving loop's step in flight
+// (if any) before it uses the device, which it then has to itself while it holds the engine's mutex.
+void drain(Engine &E) {
+    for (StepCtx &c : E.sc)
+        if (c.nl) (void)hipStreamSynchronize(c.st);
 }
 
 }  // namespace
@@ -1351,9 +1520,17 @@ int war_init(int32_t device, uint32_t flags) {
     HIPCHK(hipHostMalloc(&E->stage_host, JOBS_OFF + MAXJOBS * sizeof(Job), hipHostMallocDefault));
     HIPCHK(hipMalloc((void **)&E->reqs, MAXSLOTS * sizeof(ReqDev)));
     static_assert(sizeof(Job) == 32, "Job is 32 bytes (SLOT_BYTES)");
-    HIPCHK(hipMalloc((void **)&E->bpipe, MAXB * sizeof(Pipe)));
-    HIPCHK(hipMalloc((void **)&E->bhdr, MAXB * SLOT_BYTES));
-    HIPCHK(hipHostMalloc(&E->bstage, MAXB * SLOT_BYTES, hipHostMallocDefault));
+    for (int k = 0; k < 2; k++) {
+        StepCtx &c = E->sc[k];
+        HIPCHK(hipMalloc((void **)&c.pipe, MAXB * sizeof(Pipe)));
+        HIPCHK(hipMalloc((void **)&c.hdr, MAXB * SLOT_BYTES));
+        HIPCHK(hipHostMalloc(&c.stage, MAXB * SLOT_BYTES, hipHostMallocDefault));
+        HIPCHK(hipHostMalloc((void **)&c.pipe_host, MAXB * sizeof(Pipe), hipHostMallocDefault));
+        if (k == 0) c.st = E->st;
+        else HIPCHK(hipStreamCreateWithFlags(&c.st, hipStreamNonBlocking));   // default priority
+        HIPCHK(hipEventCreate(&c.ev0));
+        HIPCHK(hipEventCreate(&c.ev1));
+    }
     for (int b = 0; b < MAXB; b++) {
         HIPCHK(hipStreamCreateWithFlags(&E->bst[b], hipStreamNonBlocking));   // default priority
         HIPCHK(hipEventCreateWithFlags(&E->bjoin[b], hipEventDisableTiming));
@@ -1379,12 +1556,21 @@ int war_shutdown(void) {
         std::lock_guard<std::mutex> g(E->mu);
         (void)hipStreamSynchronize(E->st);
         for (auto &kv : E->graphs) (void)hipGraphExecDestroy(kv.second);
-        for (auto &kv : E->bgraphs) (void)hipGraphExecDestroy(kv.second);
+        pool_release(*E);
+        for (int k = 0; k < 2; k++) {
+            StepCtx &c = E->sc[k];
+            (void)hipStreamSynchronize(c.st);
+            for (auto &kv : c.graphs) (void)hipGraphExecDestroy(kv.second);
+            (void)hipFree(c.pipe);
+            (void)hipFree(c.hdr);
+            (void)hipHostFree(c.stage);
+            (void)hipHostFree(c.pipe_host);
+            (void)hipEventDestroy(c.ev0);
+            (void)hipEventDestroy(c.ev1);
+            if (k) (void)hipStreamDestroy(c.st);
+        }
         for (int b = 0; b < MAXB; b++) { (void)hipStreamDestroy(E->bst[b]); (void)hipEventDestroy(E->bjoin[b]); }
         (void)hipEventDestroy(E->bfork);
-        (void)hipFree(E->bpipe);
-        (void)hipFree(E->bhdr);
-        (void)hipHostFree(E->bstage);
         (void)hipHostFree(E->na_host);
         for (QueueBufs *Q : {&E->qb4, &E->qb5}) {
             for (int i = 0; i < 2; i++) {
@@ -1469,6 +1655,7 @@ int war_run_range(const war_request *req, war_result *res) {
     const double t0 = now_ms();
     {
         std::lock_guard<std::mutex> g(g_eng->mu);
+        drain(*g_eng);
         rc = run_request_locked(*g_eng, h);
     }
     res->latency_ms = now_ms() - t0;
@@ -1497,6 +1684,7 @@ int war_run_range_noalloc(const war_request *req, war_result *res) {
     Engine &E = *g_eng;
     const double t0 = now_ms();
     std::lock_guard<std::mutex> g(E.mu);
+    drain(E);
     war_tuning t;
     resolve_tuning(E, r, t);
     rc = check_tuning(t);
@@ -1621,6 +1809,7 @@ int war_run_batch(const war_request *reqs, int32_t count, war_result *res, int32
     const double t0 = now_ms();
     {
         std::lock_guard<std::mutex> g(g_eng->mu);
+        drain(*g_eng);
         if ((size_t)count > g_eng->free_slots.size())
             return fail(40, "%d requests, %zu free request slots", count, g_eng->free_slots.size());
         rc = run_batch_locked(*g_eng, hs, &nl);
@@ -1647,6 +1836,7 @@ int war_reach(int32_t n, uint64_t lo, uint64_t hi, uint64_t cap, war_reach_resul
     h.rq = r; h.res = &res; h.reach = true; h.reach_cap = cap; h.rres = out;
     {
         std::lock_guard<std::mutex> g(g_eng->mu);
+        drain(*g_eng);
         rc = run_request_locked(*g_eng, h);
     }
     return rc;
@@ -1689,6 +1879,7 @@ int war_cycle_info_batch(const war_pile_position *pos, int32_t count, int32_t ru
     }
     Engine &E = *g_eng;
     std::lock_guard<std::mutex> g(E.mu);
+    drain(E);
     CInfoDev *D = nullptr;
     HIPCHK(hipMalloc((void **)&D, H.size() * sizeof(CInfoDev)));
     int rc = 0;
@@ -1769,6 +1960,7 @@ int war_autotune_batch(const war_request *reqs, int32_t count, double budget_ms,
     // independent code path (one position per lane, the reference's structure)
     if (!expect) {
         std::lock_guard<std::mutex> g(E.mu);
+        drain(E);
         for (int i = 0; i < count; i++) {
             ReqHost h;
             h.rq = reqs[i]; h.rq.kernel = WAR_KERNEL_PLAIN;
@@ -1798,6 +1990,7 @@ int war_autotune_batch(const war_request *reqs, int32_t count, double budget_ms,
         double dev = 0;
         for (int rep = 0; rep < 2; rep++) {
             std::lock_guard<std::mutex> g(E.mu);
+            drain(E);
             int c = 0;
             if (count == 1) {
                 ReqHost h;
@@ -1878,6 +2071,7 @@ int war_autotune_batch(const war_request *reqs, int32_t count, double budget_ms,
     }
     if (count == 1) {
         std::lock_guard<std::mutex> g(E.mu);
+        drain(E);
         const war_request &q = reqs[0];
         E.tuned[std::make_tuple(q.n, q.mode, q.rule, q.detector)] = cur;
     }
