@@ -14,9 +14,9 @@ every step. One step = one whole enumeration. Beside it, on the same GPU in the 
   n17_range      the n = 17 rank range [0, 8992587776) the reference logged: held-card layout
                  (4-bit piles, card 16 apart) and 5-bit piles, Brent and the alternation detector;
   serving        512 mixed queries (n = 7..17, lengths log-uniform to 10^10) through the
-                 continuous-batching loop: saturated throughput, then Poisson arrivals at 0.25, 0.5
-                 and 0.9 of the saturated rate with p50/p99 latency per rate, beside the same queries
-                 one by one through the plain and the persistent kernels;
+                 continuous-batching loop: saturated throughput (2^26- and 2^28-position steps), then
+                 Poisson arrivals at 0.25, 0.5 and 0.9 of the saturated rate with p50/p99 latency per
+                 rate, beside the same queries one by one through the plain and the persistent kernels;
   mixed_batch    256 requests of n = 5..15 (full, canonical, distinct) autotuned, then run as one
                  war_run_batch call (one graph launch), beside the same requests one by one;
   roofline       integer VALU: useful lane-ops/s (turns x VALU instructions per turn of the flow
@@ -412,8 +412,8 @@ def serving(eng, nreq=512, seed=20261205, fractions=(0.25, 0.5, 0.9), slots=64):
     t_alone = time.perf_counter() - t0
     log("serving: %d queries, %d positions; one by one (persistent) %.1f s" % (nreq, npos, t_alone))
 
-    def run_served(gaps):
-        eng.serve_start(max_slots=slots, step_positions=1 << 26)
+    def run_served(gaps, step=1 << 26):
+        eng.serve_start(max_slots=slots, step_positions=step)
         ids = []
         t0 = time.perf_counter()
         try:
@@ -442,6 +442,13 @@ def serving(eng, nreq=512, seed=20261205, fractions=(0.25, 0.5, 0.9), slots=64):
                          "p50_ms": pct(lb, 0.5), "p99_ms": pct(lb, 0.99)},
            "rates": []}
     log("serving saturated: %.3e positions/s, %.1f req/s" % (npos / wall_b, sat_rate))
+    # the step size trades the latency of a newly arrived query (it waits for the steps in flight)
+    # against the per-step cost; the same burst with 2^28-position steps
+    burst28, wall_28 = run_served([0.0] * nreq, step=1 << 28)
+    l28 = [r["latency_ms"] for r in burst28]
+    out["saturated_step_2e28"] = {"step_positions": 1 << 28, "positions_per_s": npos / wall_28, "wall_s": wall_28,
+                                  "p50_ms": pct(l28, 0.5), "p99_ms": pct(l28, 0.99)}
+    log("serving saturated, 2^28-position steps: %.3e positions/s" % (npos / wall_28))
     for f in fractions:
         rate = f * sat_rate
         gaps = list(rng.exponential(1.0 / rate, nreq))

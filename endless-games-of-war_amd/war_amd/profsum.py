@@ -2,6 +2,8 @@
 
     python -m war_amd.profsum trace  <results.db>      per-kernel totals and per-batch stage times
     python -m war_amd.profsum pmc    <results.db>      counters per kernel (summed over dispatches)
+    python -m war_amd.profsum busy   <results.db>      device occupancy: share of the span with a kernel
+                                                       running, idle gaps, persistent kernels at once
     python -m war_amd.profsum issue  <results.db> [turns]
                                                        VALU issue read from the SQ counters (the
                                                        pass of SQ_INSTS_VALU, SQ_ACTIVE_INST_VALU,
@@ -104,10 +106,52 @@ def issue(db, turns=None, simds=1024, cus=256, xcds=8, valu_per_turn=17):
     return "\n".join(out)
 
 
+def busy(db):
+    """Device occupancy over a kernel trace: the share of the traced span in which some kernel runs,
+    the idle gaps between kernels, and how many persistent kernels (flow, walk) run at once."""
+    c = sqlite3.connect(db)
+    rows = sorted(c.execute("select name, start, end from kernels"), key=lambda r: r[1])
+    if not rows:
+        return "no kernels"
+    t0, t1 = rows[0][1], max(r[2] for r in rows)
+    on, gaps = 0, []
+    s0, e0 = rows[0][1], rows[0][2]
+    for _, s, e in rows[1:]:
+        if s > e0:
+            on += e0 - s0
+            gaps.append(s - e0)
+            s0, e0 = s, e
+        else:
+            e0 = max(e0, e)
+    on += e0 - s0
+    ev = []
+    for n, s, e in rows:
+        if "k_flow" in n or "k_walk" in n:
+            ev += [(s, 1), (e, -1)]
+    ev.sort()
+    share = collections.Counter()
+    k, last = 0, ev[0][0] if ev else 0
+    for t, d in ev:
+        share[k] += t - last
+        k += d
+        last = t
+    tot = sum(share.values()) or 1
+    gs = sorted(gaps) or [0]
+    out = ["| traced span | some kernel running | idle gaps | gap p50 / p90 / max |", "|---:|---:|---:|---:|",
+           "| %.1f ms | %.1f ms (%.1f %%) | %d, %.1f ms | %.3f / %.3f / %.3f ms |"
+           % ((t1 - t0) / 1e6, on / 1e6, 100.0 * on / (t1 - t0), len(gaps), sum(gaps) / 1e6,
+              gs[len(gs) // 2] / 1e6, gs[int(len(gs) * 0.9)] / 1e6, gs[-1] / 1e6),
+           "", "Persistent kernels (flow, walk) running at once, share of their time: " +
+           ", ".join("%d: %.3f" % (kk, v / tot) for kk, v in sorted(share.items()))]
+    return "\n".join(out)
+
+
 if __name__ == "__main__":
-    if len(sys.argv) < 3 or sys.argv[1] not in ("trace", "pmc", "issue"):
+    if len(sys.argv) < 3 or sys.argv[1] not in ("trace", "pmc", "issue", "busy"):
         sys.exit(__doc__)
     if sys.argv[1] == "issue":
         print(issue(sys.argv[2], int(sys.argv[3]) if len(sys.argv) > 3 else None))
+    elif sys.argv[1] == "busy":
+        print(busy(sys.argv[2]))
     else:
         print(trace(sys.argv[2]) if sys.argv[1] == "trace" else pmc(sys.argv[2]))
