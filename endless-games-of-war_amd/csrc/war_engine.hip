@@ -138,9 +138,10 @@ __global__ void k_fill(u64 *p, u64 v, u32 n) {
 struct GraphKey {
     int W, rule, kernel, wpc, per, cap0, walk, multi, hb, lb, kind, det;   // det: WAR_DET_* of the persistent path
     int blk;                                                                // flow kernel over whole blocks of 24 deals
+    int fw = 0;                                                             // flow kernel, Brent, rule 0: first window (fw_of)
     bool operator<(const GraphKey &o) const {
-        return std::tie(W, rule, kernel, wpc, per, cap0, walk, multi, hb, lb, kind, det, blk) <
-               std::tie(o.W, o.rule, o.kernel, o.wpc, o.per, o.cap0, o.walk, o.multi, o.hb, o.lb, o.kind, o.det, o.blk);
+        return std::tie(W, rule, kernel, wpc, per, cap0, walk, multi, hb, lb, kind, det, blk, fw) <
+               std::tie(o.W, o.rule, o.kernel, o.wpc, o.per, o.cap0, o.walk, o.multi, o.hb, o.lb, o.kind, o.det, o.blk, o.fw);
     }
 };
 
@@ -194,6 +195,7 @@ struct Engine {
     u64 *na_host = nullptr;
     std::vector<int> free_slots;
     uint32_t slot_cap[MAXSLOTS] = {};  // each slot's explicit step cap (ReqDev::cap), for the kernel variant
+    uint32_t slot_n[MAXSLOTS] = {};    // each slot's n (the first Brent window of its launches)
     std::mutex mu;                   // serialises all GPU work of the engine
 
     // serving
@@ -385,6 +387,7 @@ int req_alloc(Engine &E, ReqHost &h, hipStream_t st = nullptr, bool pooled = fal
     d.nolen = nr ? d.dhist + nd + nt : nullptr;
     d.cap = r.step_cap;
     E.slot_cap[h.slot] = r.step_cap;
+    E.slot_n[h.slot] = (u32)r.n;
     HIPCHK(hipMemcpyAsync(&E.reqs[h.slot], &d, sizeof(ReqDev), hipMemcpyHostToDevice, st));
     return 0;
 }
@@ -510,6 +513,34 @@ template <int W> CycQueue<W> cyc_queue(Engine &E) {
 
 int round_pow2(int x) { int p = 0; if (x <= 0) return 0; p = 1; while (p < x) p <<= 1; return p; }
 
+// Brent's first useful window under rule 0, as its power: a window of power p sees only cycles of
+// length <= p, so the windows below the shortest cycle of n can be left out without changing any
+// detection (or `turns`). The shortest cycles follow the reference's published cycle-length
+// histograms over every deal and every canonical position (data/checks.txt, data/long_runs.txt:
+// n = 9: 20, 30; 10: 60; 11: 12, 24; 13: 42, 56, 70; 14: 96, 112; no cycle at all for n = 12 and 16,
+// where 0 means no window is needed); from n = 17 on a cycle is at least ceil(n/2) >= 9 long (each
+// player plays every card it holds over a cycle). n <= 8 and 15 keep the window at age 7.
+u32 first_window(int n) {
+    switch (n) {
+        case 9: return 32;
+        case 10: return 64;
+        case 11: return 16;
+        case 12: return 0;
+        case 13: return 64;
+        case 14: return 128;
+        case 16: return 0;
+        default: return n >= 17 ? 16 : 8;
+    }
+}
+
+// The flow kernel's first-window variant (GraphKey::fw, template FWM) for a single request of n
+// (4-bit piles, Brent, rule 0): 1 = the first window at age 127 (n = 14), 2 = none (no cycles),
+// 0 = every window from age 7 (a window earlier than n needs is Brent's own, so still exact).
+int fw_of(int n) {
+    const u32 w = first_window(n);
+    return w >= 128 ? 1 : (w == 0 ? 2 : 0);
+}
+
 // Stage boundaries: cap0, 2 cap0 + 1, ... while below 1024, then the unbounded last stage.
 std::vector<u32> stage_caps(int cap0) {
     std::vector<u32> c;
@@ -577,8 +608,18 @@ void launch_refill(int flags, int blocks, size_t lds, hipStream_t st, const Stag
 // pair-run detectors for rule 0 (pair-run: single requests, layouts 4 and 5). BLK: the variant for
 // 4-bit deals whose jobs and lane runs are whole blocks of 24 deals; CAP: with the step-cap test.
 template <int W, int RULE, bool WALK, bool MULTI, bool BLK, bool CAP>
-void launch_flow_c(int det, int blocks, size_t lds, hipStream_t st, const StageArgs<W> &A) {
+void launch_flow_c(int det, int fw, int blocks, size_t lds, hipStream_t st, const StageArgs<W> &A) {
     if (det == WAR_DET_BRENT) {
+        if constexpr (W == 4 && RULE == 0 && !MULTI) {
+            if (fw == 1) {
+                hipLaunchKernelGGL((k_flow<4, 0, FLOW_BRENT, WALK, false, BLK, CAP, 127u>), dim3(blocks), dim3(256), lds, st, A);
+                return;
+            }
+            if (fw == 2) {
+                hipLaunchKernelGGL((k_flow<4, 0, FLOW_BRENT, WALK, false, BLK, CAP, 0xFFFFFFFFu>), dim3(blocks), dim3(256), lds, st, A);
+                return;
+            }
+        }
         hipLaunchKernelGGL((k_flow<W, RULE, FLOW_BRENT, WALK, MULTI, BLK, CAP>), dim3(blocks), dim3(256), lds, st, A);
     } else if constexpr (RULE == 0) {
         if (det == WAR_DET_ALT)
@@ -588,9 +629,9 @@ void launch_flow_c(int det, int blocks, size_t lds, hipStream_t st, const StageA
     }
 }
 template <int W, int RULE, bool WALK, bool MULTI, bool BLK = false>
-void launch_flow(int det, bool cap, int blocks, size_t lds, hipStream_t st, const StageArgs<W> &A) {
-    if (cap) launch_flow_c<W, RULE, WALK, MULTI, BLK, true>(det, blocks, lds, st, A);
-    else launch_flow_c<W, RULE, WALK, MULTI, BLK, false>(det, blocks, lds, st, A);
+void launch_flow(int det, int fw, bool cap, int blocks, size_t lds, hipStream_t st, const StageArgs<W> &A) {
+    if (cap) launch_flow_c<W, RULE, WALK, MULTI, BLK, true>(det, fw, blocks, lds, st, A);
+    else launch_flow_c<W, RULE, WALK, MULTI, BLK, false>(det, fw, blocks, lds, st, A);
 }
 
 // Enqueue one batch's kernels on stream st (inside a capture): k_setup, then the flow kernel or the
@@ -616,17 +657,17 @@ void record_launch(Engine &E, const GraphKey &key, hipStream_t st, Pipe *pipe, J
         if (key.walk) {
             A.cq = cyc_queue<W>(E);
             if constexpr (CW) {
-                if (key.multi) launch_flow<W, RULE, true, true>(key.det, key.cap0 != 0, blocks, lds, st, A);
-                else launch_flow<W, RULE, true, false>(key.det, key.cap0 != 0, blocks, lds, st, A);
+                if (key.multi) launch_flow<W, RULE, true, true>(key.det, key.fw, key.cap0 != 0, blocks, lds, st, A);
+                else launch_flow<W, RULE, true, false>(key.det, key.fw, key.cap0 != 0, blocks, lds, st, A);
             }
         } else if (key.blk) {
             if constexpr (W == 4) {
-                if (key.multi) launch_flow<4, RULE, false, true, true>(key.det, key.cap0 != 0, blocks, lds, st, A);
-                else launch_flow<4, RULE, false, false, true>(key.det, key.cap0 != 0, blocks, lds, st, A);
+                if (key.multi) launch_flow<4, RULE, false, true, true>(key.det, key.fw, key.cap0 != 0, blocks, lds, st, A);
+                else launch_flow<4, RULE, false, false, true>(key.det, key.fw, key.cap0 != 0, blocks, lds, st, A);
             }
         } else {
-            if (key.multi) launch_flow<W, RULE, false, true>(key.det, key.cap0 != 0, blocks, lds, st, A);
-            else launch_flow<W, RULE, false, false>(key.det, key.cap0 != 0, blocks, lds, st, A);
+            if (key.multi) launch_flow<W, RULE, false, true>(key.det, key.fw, key.cap0 != 0, blocks, lds, st, A);
+            else launch_flow<W, RULE, false, false>(key.det, key.fw, key.cap0 != 0, blocks, lds, st, A);
         }
     }
     u32 a0 = 0;
@@ -725,6 +766,8 @@ int plan_launch(Engine &E, std::vector<Job> &jobs, const war_tuning &t, int hb, 
                         round_pow2(hb), round_pow2(lb), kind, det, (blk && flow && W == 4 && !walk) ? 1 : 0};
     if ((!flow || walk) && (tiles > qbufs<W>(E).tmax || tiles * tile_pos > qbufs<W>(E).cap))
         return fail(32, "batch exceeds the queue capacity");
+    if (flow && W == 4 && RULE == 0 && det == WAR_DET_BRENT && !multi && !jobs.empty())
+        out->key.fw = fw_of((int)E.slot_n[jobs[0].req]);
     out->jobs = jobs;
     out->flow = flow;
     return 0;
@@ -823,7 +866,7 @@ int launch_plan(Engine &E, StepCtx &c, std::vector<Launch> &plan) {
         bkey.push_back(-1);
         for (int i : b) {
             const GraphKey &k = ls[(size_t)i]->key;
-            for (int v : {i, k.W, k.rule, k.kernel, k.wpc, k.per, k.cap0, k.walk, k.multi, k.hb, k.lb, k.kind, k.det, k.blk})
+            for (int v : {i, k.W, k.rule, k.kernel, k.wpc, k.per, k.cap0, k.walk, k.multi, k.hb, k.lb, k.kind, k.det, k.blk, k.fw})
                 bkey.push_back(v);
         }
     }
@@ -1727,6 +1770,7 @@ int war_run_range_noalloc(const war_request *req, war_result *res) {
         const bool blk = deal_blocks(r, W, lo, hi, bt);
         GraphKey key{W, r.rule, 1, bt.waves_per_cu, bt.ranks_per_refill / 64, r.step_cap ? 1 : 0, 0, 0, round_pow2(r.hbins),
                      round_pow2(r.lbins), 3, r.rule ? WAR_DET_BRENT : r.detector, (blk && W == 4) ? 1 : 0};
+        if (W == 4 && r.rule == 0 && key.det == WAR_DET_BRENT) key.fw = fw_of(r.n);
         HIPCHK(hipEventRecord(E.ev0, E.st));
         record_launch_any(E, key, E.st, pipe, hdr, job, rd);
         HIPCHK(hipGetLastError());

@@ -1031,7 +1031,9 @@ constexpr u32 FLOW_SPAN = ALT_K * FLOW_BLOCKS;
 // CAP: some request of the batch has an explicit step cap (war_request.step_cap), tested at span
 // ends (a variant, not a run-time flag: the flag and the cap held across the loop cost the kernel
 // scalar registers it spills into vector lanes, 1 % of its time at n = 14 canonical).
-template <int W, int RULE, int DET, bool WALK, bool MULTI, bool BLK = false, bool CAP = false>
+// FWM: Brent under rule 0, Brent's first window as its power minus 1 (0: every window, the tortoise
+// first at age 7; 127: first at age 127; 0xFFFFFFFF: none), a variant for the same reason.
+template <int W, int RULE, int DET, bool WALK, bool MULTI, bool BLK = false, bool CAP = false, u32 FWM = 0>
 __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
     static_assert(!BLK || (W == 4 && !WALK), "deal blocks: 4-bit deals without walks");
     static_assert(DET == FLOW_BRENT || ((DET == FLOW_ALT || DET == FLOW_PAIR) && RULE == 0),
@@ -1223,6 +1225,9 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
                     // turn (player 1's card count changes every turn), so the repeat test of a new
                     // game's first turn is masked off, and the tortoise takes the state after it
                     if constexpr (DET != FLOW_BRENT) { T_.t = s; T_.R = clk; T_.L = ALT_L0; }
+                    // rule 0: a tortoise no state can equal (every field the top card value) until
+                    // its first move, so the repeat tests before it need no mask
+                    else if constexpr (RULE == 0) { T_.t.a = ~decltype(T_.t.a)(0); }
                     if constexpr (WALK) { gpos = npos; ghome = tile; }
                 }
                 u64 e0 = 0;
@@ -1328,11 +1333,11 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
                 // every card it holds (its pile would otherwise come back with another card on top),
                 // so a cycle is at least max(cards of player 1, of player 2) >= ceil(n/2) long,
                 // which is 5 or more from n = 9 on, and the rule-0 cycles of n <= 8 are 6 and 8
-                // long (every canonical position). So under rule 0 a new game's tortoise stays at
-                // its start until age 7 and its repeat test is masked until then: every detection,
-                // and so `turns`, is Brent's. Rule 1 has cycles of 2 and 4 turns (n = 3, 5) and
-                // keeps every window.
-                if (tix < (RULE == 0 ? 7u : 1u)) cand &= ~newm;
+                // long (every canonical position). So under rule 0 a new game's tortoise first
+                // moves at age 7 (or later: the first window FWM) and no state
+                // equals it before: every detection, and so `turns`, is Brent's. Rule 1 has cycles
+                // of 2 and 4 turns (n = 3, 5) and keeps every window.
+                if (RULE != 0 && tix < 1u) cand &= ~newm;
             }
             if constexpr (DET == FLOW_ALT) {
                 // a repeated winner on a turn of odd age (tix even: births fall on span starts)
@@ -1362,12 +1367,20 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
             }
             if constexpr (DET == FLOW_BRENT) {
                 // Brent's tortoise moves at ages 2^k - 1 (from 7 on under rule 0, above): turn 2^k - 2 of a
-                // lane's first span; turn SPAN - 2 of a later span when age + 1 is a power of 2
+                // lane's first span; turn SPAN - 2 of a later span when age + 1 is a power of 2. Under
+                // rule 0 only from the first window FWM on: a window whose power is below every cycle
+                // length of the batch's n cannot see a repeat (the host derives the first window from
+                // the published cycle lengths), so leaving it out changes no detection; age + 1 (a
+                // power of 2) must then also have no bit of FWM set, for the lanes born
+                // at this span start (age + 1 = tix + 2) as for the older ones. (A move left out must
+                // not be replaced by an earlier one either: the tortoise would stay there, and its
+                // longer window could see a repeat Brent sees later.)
                 if (((tix + 2u) & (tix + 1u)) == 0u && tix >= (RULE == 0 ? 6u : 0u)) {   // unrolled: folded
-                    u64 jm = newm & alive;
+                    u64 jm = 0;
+                    if (RULE != 0 || ((tix + 2u) & FWM) == 0u) jm = newm & alive;
                     if (tix == SPAN - 2u) {
                         const u32 age1 = clk0 + tix + 2u - birth;    // age + 1
-                        jm |= ballot((age1 & (age1 - 1u)) == 0u) & alive;
+                        jm |= ballot((age1 & ((age1 - 1u) | (RULE == 0 ? FWM : 0u))) == 0u) & alive;
                     }
                     if (jm) {
                         if (in_mask(jm)) { T_.t = s; T_.R = clk0 + tix + 1u; }
