@@ -1329,14 +1329,14 @@ void retire(Engine &E, std::vector<std::shared_ptr<ReqHost>> &active, hipStream_
         }
     }
     const hipError_t se = hipStreamSynchronize(st);
-    for (auto &hp : done) {
-        ReqHost &h = *hp;
+    for (size_t i = 0; i < done.size(); i++) {
+        ReqHost &h = *done[i];
         if (!h.status && se != hipSuccess) {
             h.status = fail(100 + (int)se, "hipStreamSynchronize: %s", hipGetErrorString(se));
             h.err = t_err;
         }
         if (!h.status) {
-            const int c = req_finish(h, bufs[(size_t)(&hp - &done[0])]);
+            const int c = req_finish(h, bufs[i]);
             if (c) { h.status = c; h.err = t_err; }
         }
         req_free(E, h);
