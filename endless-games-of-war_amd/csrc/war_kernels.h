@@ -1071,9 +1071,29 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
     u32 tile = 0;              // the pool's tile: the home of its positions' cycle records
     u32 ptr = 64;              // next pool entry to hand out (64: all handed out)
     bool more = true;          // positions remain (in the pool or in tiles not yet taken)
+    // A canonical position with split n - 2 (player 2 holds one card, below player 1's top card
+    // n - 1) ends on its first turn: a game of length 1. Single-request batches (SKIP) count such
+    // positions per lane (ninst, added to the totals at the end) and step over them, so no lane idles
+    // a span for one; the run's position count `left` includes them. step is called with left >= 2
+    // (the next position is in the run); after an instant position it moves on to the next order
+    // only if that is in the run too (left >= 2 again): next_perm is never asked for the successor
+    // of the last order.
+    constexpr bool SKIP = !MULTI && !BLK;
+    u32 ninst = 0;
     auto step = [&]() {
         if (mode == 2) {
-            if (++k == n - 1u) {
+            ++k;
+            if constexpr (SKIP) {
+                if (k == n - 2u) {
+                    ninst++;
+                    left--;
+                    if (left > 1u) {
+                        ++k;
+                        if constexpr (WALK) pp++;
+                    }
+                }
+            }
+            if (k == n - 1u) {
                 k = 0;
                 q = next_perm<PW>(q, (int)n - 1);
                 if constexpr (W == 4) qr = rev_fields4(q);
@@ -1115,11 +1135,19 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
             }
             const u64 first = J.lo + (g0 - J.tile_base) * (u64)TP + (u64)lane * per;
             left = first < J.hi ? (u32)((J.hi - first) < per ? (J.hi - first) : per) : 0u;
+            u64 f = first;     // the run's first position to play
             if (left) {
                 if (mode == 2) {
                     const u32 m = n - 1u;
-                    const u64 r = first / m;
+                    u64 r = first / m;
                     k = (u32)(first - r * m);
+                    if constexpr (SKIP) {
+                        if (k == n - 2u) {                      // an instant position first
+                            ninst++;
+                            left--;
+                            if (left) { k = 0; r++; f++; }      // the next order is in the run
+                        }
+                    }
                     q = unrank<PW>(r, (int)m);
                     if constexpr (W == 4) qr = rev_fields4(q);
                 } else {
@@ -1127,7 +1155,7 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
                     q = unrank<PW>(first, (int)n);
                 }
             }
-            if constexpr (WALK) { pp = first; tile = (u32)g0; }
+            if constexpr (WALK) { pp = f; tile = (u32)g0; }
             hj = 0;
             fresh = true;
             skip_filtered();
@@ -1455,6 +1483,11 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
         if (more && ~alive) refill();
     }
     fold_lanes();
+    if constexpr (SKIP) {            // the length-1 games the pool stepped over
+        u32 t = ninst;
+        for (int off = 32; off > 0; off >>= 1) t += (u32)__shfl_xor((int)t, off);
+        if (t) acc.ended(t, 1u);
+    }
     acc.flush(reqs);
     if (WALK && acc.pushed && lane == 0) atomicAdd((unsigned long long *)&A.pipe->ctotal, (unsigned long long)acc.pushed);
 }
