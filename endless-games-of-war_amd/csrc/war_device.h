@@ -438,6 +438,19 @@ __device__ __forceinline__ St<4> canon_top(u64 qr, int n, int k) {
     return s;
 }
 
+// The same position after its first turn, which player 1's top card n - 1 always wins (rule 0: it goes
+// under player 1's pile first, then player 2's q(k)): player 1 = [q0..q(k-1), n-1, qk], player 2 =
+// [q(k+1)..q(n-2)]. k <= n - 3 (split n - 2 ends on that turn and is never dealt), so every shift
+// is below 64; the bits below each pile's bottom card are zero, as turn_top needs.
+__device__ __forceinline__ St<4> canon_top1(u64 qr, int n, int k) {
+    St<4> s;
+    const u32 sk = 4u * (u32)k;
+    s.a = (qr & ~(~0ull >> sk)) | ((u64)(u32)(n - 1) << (60u - sk)) | ((qr >> 4) & (0xFull << (56u - sk)));
+    s.b = qr << (sk + 4u);
+    s.sa = sk + 4u;
+    return s;
+}
+
 // The 24 deals whose orders share the first n-4 cards (a block of 24 consecutive ranks that starts
 // at a multiple of 24) differ in the last four deck positions only: Heap's algorithm visits them
 // with one swap of two of those positions per step. Step t (0..22) swaps positions i, j of the

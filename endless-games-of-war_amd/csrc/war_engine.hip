@@ -195,7 +195,8 @@ struct Engine {
     u64 *na_host = nullptr;
     std::vector<int> free_slots;
     uint32_t slot_cap[MAXSLOTS] = {};  // each slot's explicit step cap (ReqDev::cap), for the kernel variant
-    uint32_t slot_n[MAXSLOTS] = {};    // each slot's n (the first Brent window of its launches)
+    uint32_t slot_n[MAXSLOTS] = {};    // each slot's n and mode (the flow kernel's first-window variant)
+    uint32_t slot_mode[MAXSLOTS] = {};
     std::mutex mu;                   // serialises all GPU work of the engine
 
     // serving
@@ -388,6 +389,7 @@ int req_alloc(Engine &E, ReqHost &h, hipStream_t st = nullptr, bool pooled = fal
     d.cap = r.step_cap;
     E.slot_cap[h.slot] = r.step_cap;
     E.slot_n[h.slot] = (u32)r.n;
+    E.slot_mode[h.slot] = (u32)r.mode;
     HIPCHK(hipMemcpyAsync(&E.reqs[h.slot], &d, sizeof(ReqDev), hipMemcpyHostToDevice, st));
     return 0;
 }
@@ -533,12 +535,14 @@ u32 first_window(int n) {
     }
 }
 
-// The flow kernel's first-window variant (GraphKey::fw, template FWM) for a single request of n
-// (4-bit piles, Brent, rule 0): 1 = the first window at age 127 (n = 14), 2 = none (no cycles),
-// 0 = every window from age 7 (a window earlier than n needs is Brent's own, so still exact).
-int fw_of(int n) {
+// The flow kernel's first-window variant (GraphKey::fw, templates FWM and OFF) for a single request
+// of n (4-bit piles, Brent, rule 0): 1 = the first window at age 127 (n = 14), 2 = none (no cycles),
+// 0 = every window from age 7 (a window earlier than n needs is Brent's own, so still exact); 3 and
+// 4 = 1 and 2 for canonical positions dealt after their first turn (without an explicit step cap).
+int fw_of(int n, int mode, bool capped) {
     const u32 w = first_window(n);
-    return w >= 128 ? 1 : (w == 0 ? 2 : 0);
+    const int fw = w >= 128 ? 1 : (w == 0 ? 2 : 0);
+    return (fw && mode == WAR_MODE_CANON && !capped) ? fw + 2 : fw;
 }
 
 // Stage boundaries: cap0, 2 cap0 + 1, ... while below 1024, then the unbounded last stage.
@@ -618,6 +622,16 @@ void launch_flow_c(int det, int fw, int blocks, size_t lds, hipStream_t st, cons
             if (fw == 2) {
                 hipLaunchKernelGGL((k_flow<4, 0, FLOW_BRENT, WALK, false, BLK, CAP, 0xFFFFFFFFu>), dim3(blocks), dim3(256), lds, st, A);
                 return;
+            }
+            if constexpr (!BLK && !CAP) {
+                if (fw == 3) {
+                    hipLaunchKernelGGL((k_flow<4, 0, FLOW_BRENT, WALK, false, false, false, 127u, true>), dim3(blocks), dim3(256), lds, st, A);
+                    return;
+                }
+                if (fw == 4) {
+                    hipLaunchKernelGGL((k_flow<4, 0, FLOW_BRENT, WALK, false, false, false, 0xFFFFFFFFu, true>), dim3(blocks), dim3(256), lds, st, A);
+                    return;
+                }
             }
         }
         hipLaunchKernelGGL((k_flow<W, RULE, FLOW_BRENT, WALK, MULTI, BLK, CAP>), dim3(blocks), dim3(256), lds, st, A);
@@ -767,7 +781,7 @@ int plan_launch(Engine &E, std::vector<Job> &jobs, const war_tuning &t, int hb, 
     if ((!flow || walk) && (tiles > qbufs<W>(E).tmax || tiles * tile_pos > qbufs<W>(E).cap))
         return fail(32, "batch exceeds the queue capacity");
     if (flow && W == 4 && RULE == 0 && det == WAR_DET_BRENT && !multi && !jobs.empty())
-        out->key.fw = fw_of((int)E.slot_n[jobs[0].req]);
+        out->key.fw = fw_of((int)E.slot_n[jobs[0].req], (int)E.slot_mode[jobs[0].req], capped);
     out->jobs = jobs;
     out->flow = flow;
     return 0;
@@ -1770,7 +1784,7 @@ int war_run_range_noalloc(const war_request *req, war_result *res) {
         const bool blk = deal_blocks(r, W, lo, hi, bt);
         GraphKey key{W, r.rule, 1, bt.waves_per_cu, bt.ranks_per_refill / 64, r.step_cap ? 1 : 0, 0, 0, round_pow2(r.hbins),
                      round_pow2(r.lbins), 3, r.rule ? WAR_DET_BRENT : r.detector, (blk && W == 4) ? 1 : 0};
-        if (W == 4 && r.rule == 0 && key.det == WAR_DET_BRENT) key.fw = fw_of(r.n);
+        if (W == 4 && r.rule == 0 && key.det == WAR_DET_BRENT) key.fw = fw_of(r.n, r.mode, r.step_cap != 0);
         HIPCHK(hipEventRecord(E.ev0, E.st));
         record_launch_any(E, key, E.st, pipe, hdr, job, rd);
         HIPCHK(hipGetLastError());

@@ -1033,8 +1033,16 @@ constexpr u32 FLOW_SPAN = ALT_K * FLOW_BLOCKS;
 // scalar registers it spills into vector lanes, 1 % of its time at n = 14 canonical).
 // FWM: Brent under rule 0, Brent's first window as its power minus 1 (0: every window, the tortoise
 // first at age 7; 127: first at age 127; 0xFFFFFFFF: none), a variant for the same reason.
-template <int W, int RULE, int DET, bool WALK, bool MULTI, bool BLK = false, bool CAP = false, u32 FWM = 0>
+// OFF: canonical positions dealt after their first turn (canon_top1), which player 1's top card n - 1
+// always wins: a game is born one turn old (its birth clock one before the span start), so ages at a
+// span start are 1 mod SPAN and the old games' tortoise moves fall on turn SPAN - 3; the played turn
+// count, the ages and every output are unchanged. With a first window of 16 or more only (no move in
+// a game's first span).
+template <int W, int RULE, int DET, bool WALK, bool MULTI, bool BLK = false, bool CAP = false, u32 FWM = 0,
+          bool OFF = false>
 __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
+    static_assert(!OFF || (W == 4 && RULE == 0 && DET == 0 && !MULTI && !BLK && !CAP && FWM >= 15u),
+                  "positions dealt after their first turn: 4-bit canonical single requests, Brent, rule 0, first window >= 16");
     static_assert(!BLK || (W == 4 && !WALK), "deal blocks: 4-bit deals without walks");
     static_assert(DET == FLOW_BRENT || ((DET == FLOW_ALT || DET == FLOW_PAIR) && RULE == 0),
                   "the alternation and pair-run detectors are for rule 0");
@@ -1162,7 +1170,8 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
         }
         if (left && fresh) {
             if constexpr (W == 4) {                          // 4-bit games play in turn_top's form
-                ps = (mode == 2) ? canon_top(qr, (int)n, (int)k) : flip_piles(deal<W>(q, (int)n));
+                ps = (mode == 2) ? (OFF ? canon_top1(qr, (int)n, (int)k) : canon_top(qr, (int)n, (int)k))
+                                 : flip_piles(deal<W>(q, (int)n));
             } else {
                 ps = (mode == 2) ? canon<W>(q, (int)n, (int)k) : deal<W>(q, (int)n);
             }
@@ -1247,7 +1256,7 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
                     s = ns;
                     open_sa<W>(s, A.zero);
                     sb = C - s.sa;
-                    birth = clk;
+                    birth = clk - (OFF ? 1u : 0u);
                     if constexpr (MULTI) { gC = C; greq = preq; }
                     // Brent's first tortoise (the start) needs no copy: no state repeats after one
                     // turn (player 1's card count changes every turn), so the repeat test of a new
@@ -1403,10 +1412,15 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
                 // at this span start (age + 1 = tix + 2) as for the older ones. (A move left out must
                 // not be replaced by an earlier one either: the tortoise would stay there, and its
                 // longer window could see a repeat Brent sees later.)
-                if (((tix + 2u) & (tix + 1u)) == 0u && tix >= (RULE == 0 ? 6u : 0u)) {   // unrolled: folded
+                // (OFF: no move in a game's first span; the old games' at turn SPAN - 3, where age + 1
+                // is a multiple of SPAN)
+                const bool newb = !OFF && ((tix + 2u) & (tix + 1u)) == 0u && tix >= (RULE == 0 ? 6u : 0u) &&
+                                  (RULE != 0 || ((tix + 2u) & FWM) == 0u);
+                const bool oldw = tix == SPAN - 2u - (OFF ? 1u : 0u);
+                if (newb || oldw) {   // unrolled: folded
                     u64 jm = 0;
-                    if (RULE != 0 || ((tix + 2u) & FWM) == 0u) jm = newm & alive;
-                    if (tix == SPAN - 2u) {
+                    if (newb) jm = newm & alive;
+                    if (oldw) {
                         const u32 age1 = clk0 + tix + 2u - birth;    // age + 1
                         jm |= ballot((age1 & ((age1 - 1u) | (RULE == 0 ? FWM : 0u))) == 0u) & alive;
                     }
