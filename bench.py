@@ -48,21 +48,24 @@ PUBLISHED_N13_DEALS_PER_S = 6227020800 / 15.3      # RTX 5080, war_gpu.py 13 (re
 A13 = {"cyc": 4880546113, "maxfin": 57, "hist": {42: 1475465743, 56: 2394985450, 70: 1010094920}}
 C14 = {"cyc": 2872129173, "maxfin": 465, "hist": {96: 43669233, 112: 2828459940}}
 # VALU instructions per War turn in the flow kernel's loop: 16 for the turn (its last subtraction
-# also yields the end test as its borrow) and one compare of player 1's pile with the tortoise's,
-# counted in the gfx950 assembly of k_flow (k_stage's age-synchronous loop also has 17); the
-# algorithmic unit is one turn of the detector run.
-VALU_OPS_PER_TURN = 17
+# also yields the end test as its borrow) and one compare of player 1's pile with the tortoise's on
+# every other turn (every cycle is of even length, so a game meets its tortoise only at an even number
+# of turns after the tortoise was taken): 16.5 on average, counted in the gfx950 assembly of k_flow.
+# k_stage's age-synchronous loop compares on every turn: 17. The algorithmic unit is one turn of the
+# detector run.
+VALU_OPS_PER_TURN = 16.5
+STAGED_VALU_OPS_PER_TURN = 17
 # VALU issue on gfx950, measured with an issue-rate probe at 7 waves per SIMD: the VOP2
 # v_add/v_sub/v_and/v_or/v_xor/v_mov_b32 and the 32- and 16-bit shifts right issue two wave64
 # instructions per SIMD per 4 clocks (dual issue); 64-bit shifts, adds and compares, VOP3 forms,
 # selects, compares and VOP2 forms with a scalar operand issue one per 4 clocks. The peak is
 # therefore 2 x 64 lanes per SIMD per 4 clocks = CUs x 128 lane-ops per clock (78.6 T/s at 256 CUs
-# and 2.4 GHz, the rate the published 157.3 TFLOP/s FP32 vector figure implies). Of the turn's 17
-# instructions 7 can dual-issue (2 ANDs, 2 shifts right, 2 ORs, the length add) and 10 cannot, so
-# the turn itself needs at least 10 x 4 + 7 x 2 = 54 clocks per wave: its instruction mix caps VALU
-# issue at 17 x 4 / 54 = 1.26 instructions per SIMD per 4 clocks, 0.630 of the dual-issue peak.
+# and 2.4 GHz, the rate the published 157.3 TFLOP/s FP32 vector figure implies). Of the turn's 16.5
+# instructions 7 can dual-issue (2 ANDs, 2 shifts right, 2 ORs, the length add) and 9.5 cannot, so
+# the turn itself needs at least 9.5 x 4 + 7 x 2 = 52 clocks per wave: its instruction mix caps VALU
+# issue at 16.5 x 4 / 52 = 1.27 instructions per SIMD per 4 clocks, 0.635 of the dual-issue peak.
 VALU_LANE_OPS_PER_CU_CLOCK = 128
-TURN_MIX_CEILING = (VALU_OPS_PER_TURN * 4.0 / (10 * 4 + 7 * 2)) / 2.0
+TURN_MIX_CEILING = (VALU_OPS_PER_TURN * 4.0 / (9.5 * 4 + 7 * 2)) / 2.0
 # HBM bytes per k_flow launch on the n = 13 workload (2 launches; no records are written: 0.20 MB
 # read, 66.5 MB of tile-grab atomics and counters), from the rocprofv3 byte counters calibrated on a
 # copy of known size.
@@ -267,7 +270,7 @@ def main():
                              "device_ms": bf["flow"]["device_ms"], "turns": bt,
                              "frac": bt * VALU_OPS_PER_TURN / (bf["flow"]["device_ms"] / 1e3) / peak_lane_ops,
                              "staged_device_ms": bf["staged"]["device_ms"],
-                             "staged_frac": bt * VALU_OPS_PER_TURN / (bf["staged"]["device_ms"] / 1e3) / peak_lane_ops,
+                             "staged_frac": bt * STAGED_VALU_OPS_PER_TURN / (bf["staged"]["device_ms"] / 1e3) / peak_lane_ops,
                              "note": "Brent's turns are the reference's Sum of rounds (the same on both paths); "
                                      "frac as for the roofline, on these turns"}
         assert bf["staged"]["turns"] == bt

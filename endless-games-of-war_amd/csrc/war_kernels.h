@@ -1362,8 +1362,17 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
             if constexpr (W == 4) pm = turn_top<RULE, !MULTI, SELASM>(s, sb, MULTI ? gC : C, zt, over);
             else pm = ballot(turn_sb<W, RULE, !MULTI>(s, sb, MULTI ? gC : C, z, over));
             // the common case (no lane ended or matched the tortoise's player-1 pile) costs two
-            // scalar operations and the branch; the scalar unit is busy about 70 % of the time here
-            u64 cand = ballot(s.a == T_.t.a);
+            // scalar operations and the branch; the scalar unit is busy about 70 % of the time here.
+            // Every cycle is of even length (over one cycle player 1's card count comes back, so it
+            // wins as many of the cycle's turns as it loses), so a game meets its tortoise only at
+            // an even number of turns after the tortoise was taken. Tortoises are taken at odd ages
+            // (Brent: 2^k - 1) or even ones (the checkpoint detectors: multiples of ALT_K, and a new
+            // game's start), and births fall on span starts (OFF: one turn before), so the age
+            // parity after turn tix is the same in every lane: the compare runs on every other turn
+            // only, and each detection falls on the same turn as with a compare on every turn.
+            constexpr u32 CMP_PAR = DET == FLOW_BRENT ? (OFF ? 1u : 0u) : 1u;
+            u64 cand = 0;
+            if ((tix & 1u) == CMP_PAR) cand = ballot(s.a == T_.t.a);
             if constexpr (DET == FLOW_BRENT) {
                 // Brent's windows of the tortoise at ages 1 and 3 see only cycles of length <= 2
                 // and <= 4, and no rule-0 War cycle is that short: over one cycle each player plays
