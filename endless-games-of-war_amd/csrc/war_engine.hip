@@ -663,6 +663,7 @@ void record_launch(Engine &E, const GraphKey &key, hipStream_t st, Pipe *pipe, J
     hipLaunchKernelGGL(k_setup, dim3(256), dim3(256), 0, st, pipe, (const JobHdr *)hdr,
                        key.walk ? Q.ccnt : (u32 *)nullptr);
     if (key.kind == 3) {                // the flow kernel: one launch plays the whole batch
+        const size_t flds = lds + flow_pool_bytes<W>();     // the waves' pools ahead of the histograms
         StageArgs<W> A{};
         A.pipe = pipe; A.hdr = hdr; A.jobs = jobsd; A.reqs = reqs;
         A.per = (u32)key.per; A.tile_pos = 64u * (u32)key.per;
@@ -671,17 +672,17 @@ void record_launch(Engine &E, const GraphKey &key, hipStream_t st, Pipe *pipe, J
         if (key.walk) {
             A.cq = cyc_queue<W>(E);
             if constexpr (CW) {
-                if (key.multi) launch_flow<W, RULE, true, true>(key.det, key.fw, key.cap0 != 0, blocks, lds, st, A);
-                else launch_flow<W, RULE, true, false>(key.det, key.fw, key.cap0 != 0, blocks, lds, st, A);
+                if (key.multi) launch_flow<W, RULE, true, true>(key.det, key.fw, key.cap0 != 0, blocks, flds, st, A);
+                else launch_flow<W, RULE, true, false>(key.det, key.fw, key.cap0 != 0, blocks, flds, st, A);
             }
         } else if (key.blk) {
             if constexpr (W == 4) {
-                if (key.multi) launch_flow<4, RULE, false, true, true>(key.det, key.fw, key.cap0 != 0, blocks, lds, st, A);
-                else launch_flow<4, RULE, false, false, true>(key.det, key.fw, key.cap0 != 0, blocks, lds, st, A);
+                if (key.multi) launch_flow<4, RULE, false, true, true>(key.det, key.fw, key.cap0 != 0, blocks, flds, st, A);
+                else launch_flow<4, RULE, false, false, true>(key.det, key.fw, key.cap0 != 0, blocks, flds, st, A);
             }
         } else {
-            if (key.multi) launch_flow<W, RULE, false, true>(key.det, key.fw, key.cap0 != 0, blocks, lds, st, A);
-            else launch_flow<W, RULE, false, false>(key.det, key.fw, key.cap0 != 0, blocks, lds, st, A);
+            if (key.multi) launch_flow<W, RULE, false, true>(key.det, key.fw, key.cap0 != 0, blocks, flds, st, A);
+            else launch_flow<W, RULE, false, false>(key.det, key.fw, key.cap0 != 0, blocks, flds, st, A);
         }
     }
     u32 a0 = 0;

@@ -986,16 +986,6 @@ __device__ __forceinline__ P shfl_word(P v, u32 src) {
     }
 }
 
-template <int W>
-__device__ __forceinline__ St<W> shfl_state(const St<W> &x, u32 src) {
-    St<W> r;
-    r.a = shfl_word(x.a, src);
-    r.b = shfl_word(x.b, src);
-    r.sa = (u32)__shfl((int)x.sa, (int)src);
-    if constexpr (W == 6) r.pb = (u32)__shfl((int)x.pb, (int)src);
-    return r;
-}
-
 // The turn of the span each lane's game ended on, from the bit planes of the end turns: bit b of the
 // turn is this lane's bit of fb[b]. Horner's rule in add-with-carry steps (e = 2e + bit, the bit as
 // the carry in), one VALU instruction per plane instead of a select and an add.
@@ -1023,6 +1013,13 @@ constexpr int FLOW_BRENT = 0, FLOW_ALT = 2, FLOW_PAIR = 3;
 // seen. Every output is exact and equals Brent's; only `turns` (its own) differs.
 constexpr u32 PAIR_P = 128;
 constexpr u32 FLOW_SPAN = ALT_K * FLOW_BLOCKS;
+// The flow kernel's pools: 64 dealt positions per wave in LDS, ahead of the waves' histograms (blocks
+// of 4 waves); an entry's sa is FLOW_POOL_EMPTY when the run that deals it holds no more positions
+// (sa of a real position is a bit offset below 256).
+constexpr u32 FLOW_POOL_ENTRIES = 64, FLOW_POOL_EMPTY = 0xFFFFFFFFu;
+template <int W> __host__ __device__ constexpr u32 flow_pool_bytes() {
+    return 4u * FLOW_POOL_ENTRIES * (u32)sizeof(St<W>);
+}
 
 // MULTI: the batch holds several requests (the serving loop's steps, war_run_batch). A tile belongs
 // to one job, so the pool's positions share one request (its n, mode and C change at a tile grab);
@@ -1053,8 +1050,10 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
     extern __shared__ u32 lds[];
     const u32 wid = threadIdx.x >> 6;
     const u32 lane = lane_id();
+    // LDS: the four waves' pools (FLOW_POOL_ENTRIES dealt positions each), then their histograms
+    St<W> *pool = reinterpret_cast<St<W> *>(lds) + wid * FLOW_POOL_ENTRIES;
     WaveAcc acc;
-    acc.init(lds + wid * (A.hb + A.lb), A.hb, A.lb);
+    acc.init(lds + flow_pool_bytes<W>() / 4u + wid * (A.hb + A.lb), A.hb, A.lb);
     const ReqDev *reqs = A.reqs;
     const u32 req = A.hdr->req0;
     acc.select(req, reqs);
@@ -1069,8 +1068,8 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
     u64 nxt = 0;
     if (lane == 0) nxt = atomicAdd((unsigned long long *)grab, 1ull);
 
-    // ---- the pool: lane l holds the current position of its run, already dealt (the deal of a
-    // whole generation of 64 runs as one wave step)
+    // ---- the pool: lane l deals the current position of its run (a whole generation of 64 runs as
+    // one wave step) into entry l of the wave's pool in LDS, from where free lanes take them
     P q = 0;
     u64 qr = 0;                // 4-bit canonical runs: the order reversed (canon_top builds from it)
     u32 k = 0, left = 0;       // this lane's run: current order / split; positions left, it included
@@ -1176,6 +1175,13 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
                 ps = (mode == 2) ? canon<W>(q, (int)n, (int)k) : deal<W>(q, (int)n);
             }
         }
+        // entry `lane` of the pool: this lane's position, or empty once its run is done (every lane
+        // writes its own entry; the wave's later reads of other lanes' entries follow in program
+        // order, and LDS operations of one wave complete in order)
+        St<W> e = ps;
+        if (!left) e.sa = FLOW_POOL_EMPTY;
+        pool[lane] = e;
+        __builtin_amdgcn_wave_barrier();
         ptr = 0;
         return true;
     };
@@ -1245,15 +1251,15 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
                 const u32 i = rank_in(dead);
                 const u64 tm = ballot(i < 64u - ptr) & dead;       // the first free lanes, one per entry
                 const u32 src = (ptr + i) & 63u;
-                const St<W> ns = shfl_state<W>(ps, src);
-                const u32 lf = (u32)__shfl((int)left, (int)src);
+                // a free lane's state is not read (its game is over), so the lanes served take their
+                // entry straight into it, empty entries too (those lanes stay free)
+                if (in_mask(tm)) s = pool[src];
                 u64 npos = 0;
                 if constexpr (WALK) npos = shfl_word(pp, src);
                 ptr += (u32)__popcll(tm);
                 dead &= ~tm;
-                const u64 gm = ballot(lf != 0u) & tm;               // entries that hold a position
+                const u64 gm = ballot(s.sa != FLOW_POOL_EMPTY) & tm;    // entries that hold a position
                 if (in_mask(gm)) {
-                    s = ns;
                     open_sa<W>(s, A.zero);
                     sb = C - s.sa;
                     birth = clk - (OFF ? 1u : 0u);
