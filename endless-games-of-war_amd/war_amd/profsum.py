@@ -4,14 +4,16 @@
     python -m war_amd.profsum pmc    <results.db>      counters per kernel (summed over dispatches)
     python -m war_amd.profsum busy   <results.db>      device occupancy: share of the span with a kernel
                                                        running, idle gaps, persistent kernels at once
-    python -m war_amd.profsum issue  <results.db> [turns]
+    python -m war_amd.profsum issue  <results.db> [turns [valu_per_turn]]
                                                        VALU issue read from the SQ counters (the
                                                        pass of SQ_INSTS_VALU, SQ_ACTIVE_INST_VALU,
                                                        SQ_ACTIVE_INST_VALU2, SQ_THREAD_CYCLES_VALU,
                                                        SQ_INSTS_SALU, GRBM_GUI_ACTIVE): per SIMD
                                                        quad-cycle, lanes active, scalar issue; with
-                                                       turns, the useful share (turns x 17 / 64 /
-                                                       VALU instructions)
+                                                       turns, the useful share (turns x VALU
+                                                       instructions per turn / 64 / VALU
+                                                       instructions; 16.5 per turn by default, the
+                                                       flow kernel's loop)
 
 A batch of the persistent pipeline starts with k_setup, so the trace view also lists, for the
 first and last batches, the duration of each of its kernels (the flow kernel, or the stage
@@ -75,7 +77,7 @@ def pmc(db):
     return "\n".join(out)
 
 
-def issue(db, turns=None, simds=1024, cus=256, xcds=8, valu_per_turn=17):
+def issue(db, turns=None, simds=1024, cus=256, xcds=8, valu_per_turn=16.5):
     """VALU issue of each kernel of the run, from the SQ counters summed over its dispatches.
     GRBM_GUI_ACTIVE is summed over the XCDs (divided by `xcds` for the kernel's clocks);
     SQ_ACTIVE_INST_VALU counts the VALU instructions issued and SQ_ACTIVE_INST_VALU2 the SIMD
@@ -150,7 +152,8 @@ if __name__ == "__main__":
     if len(sys.argv) < 3 or sys.argv[1] not in ("trace", "pmc", "issue", "busy"):
         sys.exit(__doc__)
     if sys.argv[1] == "issue":
-        print(issue(sys.argv[2], int(sys.argv[3]) if len(sys.argv) > 3 else None))
+        print(issue(sys.argv[2], int(sys.argv[3]) if len(sys.argv) > 3 else None,
+                    valu_per_turn=float(sys.argv[4]) if len(sys.argv) > 4 else 16.5))
     elif sys.argv[1] == "busy":
         print(busy(sys.argv[2]))
     else:
