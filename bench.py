@@ -24,7 +24,9 @@ every step. One step = one whole enumeration. Beside it, on the same GPU in the 
                  per clock, with the fraction of the single-issue rate and of the turn's own
                  instruction-mix ceiling beside it;
   cpu_baseline   the CPU oracle (OpenMP, every CPU the process may use) on bounded samples of the
-                 n = 13 deals, all n = 9 deals and sampled n = 14 canonical chunks.
+                 n = 13 deals, all n = 9 deals and sampled n = 14 canonical chunks; beside it the
+                 reference's own cpu/war.cpp (built from its source into oracle/_ref/ where the build
+                 had the reference tree) over every n = 12 deal, with the oracle on the same deals.
 
   python bench.py [--gpus 1] [--steps 3] [--warmup 1]
                   [--only main|brent13|staged13|canon14|canon14walk|plain13|plaincanon14|serving|serveburst|
@@ -619,6 +621,43 @@ def cpu_legs(eng, n13_chunks=64):
                             "gpu_over_cpu": None}}
     log("cpu baseline (%d threads of %s): n13 %.3e deals/s, n9 %.3e, n14 canon %.3e positions/s"
         % (threads, hc["cpu_model"], c2, out["c1_n9_full"]["value"], c3))
+    ref = reference_program(eng, threads)
+    if ref:
+        out["reference_program"] = ref
+    return out
+
+
+def reference_program(eng, threads, n=12):
+    """The reference's own CPU program (cpu/war.cpp, compiled from its source into oracle/_ref/war
+    by oracle/Makefile; present where the build had the reference tree) over every deal of n = 12
+    (its command line takes a whole n; 4.79e8 deals), timed on the same threads, beside the oracle
+    on the same range; its line is compared with the GPU's counts. None if the program is absent."""
+    import subprocess
+    exe = os.path.join(ROOT, "oracle", "_ref", "war")
+    if not os.access(exe, os.X_OK):
+        return None
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    t0 = time.time()
+    try:
+        p = subprocess.run([exe, str(n), "0"], env=env, stdout=subprocess.PIPE, stderr=subprocess.DEVNULL,
+                           timeout=300, check=True, text=True)
+    except (OSError, subprocess.SubprocessError) as e:
+        log("reference cpu/war.cpp not timed: %s" % e)
+        return None
+    el = time.time() - t0
+    f = dict(kv.split("=") for kv in p.stdout.split() if "=" in kv)
+    g = eng.run_range(n, "full")
+    assert int(f["deals"]) == math.factorial(n) == g["done"]
+    assert (int(f["cycling"]), int(f["longest_finite"])) == (g["cyc"], g["maxfin"]), (p.stdout, g)
+    res, elp = cpu_baseline(n, "full", [(0, math.factorial(n))], threads)
+    assert (res[0]["cyc"], res[0]["maxfin"]) == (g["cyc"], g["maxfin"])
+    out = {"value": math.factorial(n) / el, "unit": "deals/s", "cores": threads, "kind": "reference",
+           "sample": "oracle/_ref/war %d 0: every n = %d deal (%d), the reference's cpu/war.cpp built from its "
+                     "source; its line (cycling %s, longest finite %s) equal to the GPU's" %
+                     (n, n, math.factorial(n), f["cycling"], f["longest_finite"]),
+           "port_value": math.factorial(n) / elp, "port_over_reference": el / elp}
+    log("reference cpu/war.cpp (%d threads): n%d %.3e deals/s; the oracle on the same deals %.3e"
+        % (threads, n, out["value"], out["port_value"]))
     return out
 
 
