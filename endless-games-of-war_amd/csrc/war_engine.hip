@@ -663,7 +663,7 @@ void record_launch(Engine &E, const GraphKey &key, hipStream_t st, Pipe *pipe, J
     hipLaunchKernelGGL(k_setup, dim3(256), dim3(256), 0, st, pipe, (const JobHdr *)hdr,
                        key.walk ? Q.ccnt : (u32 *)nullptr);
     if (key.kind == 3) {                // the flow kernel: one launch plays the whole batch
-        const size_t flds = lds + flow_pool_bytes<W>();     // the waves' pools ahead of the histograms
+        const size_t flds = lds + flow_pool_bytes<W>(key.multi != 0, key.walk != 0);   // pools, then histograms
         StageArgs<W> A{};
         A.pipe = pipe; A.hdr = hdr; A.jobs = jobsd; A.reqs = reqs;
         A.per = (u32)key.per; A.tile_pos = 64u * (u32)key.per;

@@ -1013,12 +1013,15 @@ constexpr int FLOW_BRENT = 0, FLOW_ALT = 2, FLOW_PAIR = 3;
 // seen. Every output is exact and equals Brent's; only `turns` (its own) differs.
 constexpr u32 PAIR_P = 128;
 constexpr u32 FLOW_SPAN = ALT_K * FLOW_BLOCKS;
-// The flow kernel's pools: 64 dealt positions per wave in LDS, ahead of the waves' histograms (blocks
-// of 4 waves); an entry's sa is FLOW_POOL_EMPTY when the run that deals it holds no more positions
-// (sa of a real position is a bit offset below 256).
-constexpr u32 FLOW_POOL_ENTRIES = 64, FLOW_POOL_EMPTY = 0xFFFFFFFFu;
-template <int W> __host__ __device__ constexpr u32 flow_pool_bytes() {
-    return 4u * FLOW_POOL_ENTRIES * (u32)sizeof(St<W>);
+// The flow kernel's pools: dealt positions in LDS, ahead of the waves' histograms (blocks of 4
+// waves); an entry's sa is FLOW_POOL_EMPTY when the run that deals it holds no more positions (sa of a
+// real position is a bit offset below 256). A batch of one request without cycle walks keeps a ring
+// of two generations of 64 (every entry shares the request and needs no position index or home
+// tile), so one refill round serves every free lane; other batches keep one generation.
+constexpr u32 FLOW_POOL_EMPTY = 0xFFFFFFFFu;
+__host__ __device__ constexpr u32 flow_pool_entries(bool multi, bool walk) { return (multi || walk) ? 64u : 128u; }
+template <int W> __host__ __device__ constexpr u32 flow_pool_bytes(bool multi, bool walk) {
+    return 4u * flow_pool_entries(multi, walk) * (u32)sizeof(St<W>);
 }
 
 // MULTI: the batch holds several requests (the serving loop's steps, war_run_batch). A tile belongs
@@ -1050,10 +1053,12 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
     extern __shared__ u32 lds[];
     const u32 wid = threadIdx.x >> 6;
     const u32 lane = lane_id();
-    // LDS: the four waves' pools (FLOW_POOL_ENTRIES dealt positions each), then their histograms
-    St<W> *pool = reinterpret_cast<St<W> *>(lds) + wid * FLOW_POOL_ENTRIES;
+    // LDS: the four waves' pools (PE dealt positions each), then their histograms
+    constexpr bool RING = !MULTI && !WALK;
+    constexpr u32 PE = flow_pool_entries(MULTI, WALK);
+    St<W> *pool = reinterpret_cast<St<W> *>(lds) + wid * PE;
     WaveAcc acc;
-    acc.init(lds + flow_pool_bytes<W>() / 4u + wid * (A.hb + A.lb), A.hb, A.lb);
+    acc.init(lds + flow_pool_bytes<W>(MULTI, WALK) / 4u + wid * (A.hb + A.lb), A.hb, A.lb);
     const ReqDev *reqs = A.reqs;
     const u32 req = A.hdr->req0;
     acc.select(req, reqs);
@@ -1077,6 +1082,7 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
     u64 pp = 0;                // its index (cycle walks rebuild the start from it)
     u32 tile = 0;              // the pool's tile: the home of its positions' cycle records
     u32 ptr = 64;              // next pool entry to hand out (64: all handed out)
+    u32 cons = 0, avail = 0;   // RING: the first entry not handed out, and how many follow it
     bool more = true;          // positions remain (in the pool or in tiles not yet taken)
     // A canonical position with split n - 2 (player 2 holds one card, below player 1's top card
     // n - 1) ends on its first turn: a game of length 1. Single-request batches (SKIP) count such
@@ -1180,9 +1186,14 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
         // order, and LDS operations of one wave complete in order)
         St<W> e = ps;
         if (!left) e.sa = FLOW_POOL_EMPTY;
-        pool[lane] = e;
+        if constexpr (RING) {      // the generation after the entries not yet handed out
+            pool[((cons + avail) & (PE - 1u)) + lane] = e;
+            avail += 64u;
+        } else {
+            pool[lane] = e;
+            ptr = 0;
+        }
         __builtin_amdgcn_wave_barrier();
-        ptr = 0;
         return true;
     };
 
@@ -1247,16 +1258,30 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
         for (;;) {
             u64 dead = ~alive;
             while (dead) {
-                if (ptr == 64u && !advance()) { more = false; break; }
                 const u32 i = rank_in(dead);
-                const u64 tm = ballot(i < 64u - ptr) & dead;       // the first free lanes, one per entry
-                const u32 src = (ptr + i) & 63u;
+                u64 tm;                // the first free lanes, one per entry
+                u32 src;
+                if constexpr (RING) {
+                    // fewer entries left than free lanes (at most 64): the other generation's slots
+                    // are free, so deal one there
+                    if (avail < (u32)__popcll(dead) && more && !advance()) more = false;
+                    if (avail == 0u) break;
+                    tm = ballot(i < avail) & dead;
+                    src = (cons + i) & (PE - 1u);
+                    const u32 took = (u32)__popcll(tm);
+                    cons = (cons + took) & (PE - 1u);
+                    avail -= took;
+                } else {
+                    if (ptr == 64u && !advance()) { more = false; break; }
+                    tm = ballot(i < 64u - ptr) & dead;
+                    src = (ptr + i) & 63u;
+                    ptr += (u32)__popcll(tm);
+                }
                 // a free lane's state is not read (its game is over), so the lanes served take their
                 // entry straight into it, empty entries too (those lanes stay free)
                 if (in_mask(tm)) s = pool[src];
                 u64 npos = 0;
                 if constexpr (WALK) npos = shfl_word(pp, src);
-                ptr += (u32)__popcll(tm);
                 dead &= ~tm;
                 const u64 gm = ballot(s.sa != FLOW_POOL_EMPTY) & tm;    // entries that hold a position
                 if (in_mask(gm)) {
