@@ -230,7 +230,8 @@ int check_request(const war_request *r) {
     if (r->dbins && (r->mode != WAR_MODE_CANON || r->n > 16))
         return fail(9, "dbins needs canon mode and n <= 16 (4-bit piles)");
     if (r->step_cap > WAR_SAFETY_CAP) return fail(13, "step_cap must be <= 2^24 (got %u)", r->step_cap);
-    if (r->flags & ~WAR_FLAG_UNDET_ERROR) return fail(13, "unknown flags 0x%x", r->flags);
+    if (r->flags & ~(WAR_FLAG_UNDET_ERROR | WAR_FLAG_CYCLE_TABLE | WAR_FLAG_NO_CYCLE_TABLE))
+        return fail(13, "unknown flags 0x%x", r->flags);
     if (r->shape_blk && r->mode != WAR_MODE_CANON) return fail(10, "shape needs canon mode");
     if (r->shape_blk < 0 || r->shape_blk > 512) return fail(10, "shape_blk must be in [0, 512]");
     if (r->detector != WAR_DET_BRENT && r->detector != WAR_DET_STREAK && r->detector != WAR_DET_ALT &&
@@ -343,7 +344,31 @@ void pool_release(Engine &E) {
     E.pool.clear();
 }
 
-// The request's device memory (counters | hist | lenhist | dhist | htab), zeroed, and its descriptor
+// The cycle-state table of a canonical rule-0 request with cycle walks, as log2 of its slots (0: none):
+// at least twice the states on the cycles of n, from the reference's published distinct cycles and
+// their lengths (data/checks.txt, data/long_runs.txt: n = 5: 2 of 6; 7: 52 of 8; 9: 144 of 20 and 192
+// of 30; 10: 48 of 60; 11: 42560 of 12 and 13032 of 24; 13: 86400 of 42, 129600 of 56 and 103680 of
+// 70; 14: 720 of 96 and 25920 of 112; none for n = 6, 8, 12, 16), each cycle in both player orders.
+// n = 15 (48 million cycles) keeps the walks. Ranges of at least 2^28 positions by default
+// (WAR_FLAG_CYCLE_TABLE / WAR_FLAG_NO_CYCLE_TABLE force it either way).
+constexpr u64 STAB_MIN_RANGE = 1ull << 28;
+u32 stab_log2(const war_request &r) {
+    const bool walk = r.mode == WAR_MODE_CANON && (r.distinct || r.dbins || r.shape_blk);
+    if (!walk || r.rule != 0 || (r.flags & WAR_FLAG_NO_CYCLE_TABLE)) return 0;
+    if (!(r.flags & WAR_FLAG_CYCLE_TABLE) && r.hi - r.lo < STAB_MIN_RANGE) return 0;
+    switch (r.n) {
+        case 5: return 8;      // 24 states
+        case 7: return 12;     // 832
+        case 9: return 16;     // 17 280
+        case 10: return 14;    // 5 760
+        case 11: return 22;    // 1 648 512
+        case 13: return 27;    // 36 288 000
+        case 14: return 24;    // 5 944 320
+        default: return 0;
+    }
+}
+
+// The request's device memory (counters | hist | lenhist | dhist | htab | nolen | stab), zeroed, and its descriptor
 // in the device table, both on stream st. pooled: memory from the serving loop's pool (a block of the
 // request's size class when one is free), returned there by req_free.
 int req_alloc(Engine &E, ReqHost &h, hipStream_t st = nullptr, bool pooled = false) {
@@ -358,7 +383,9 @@ int req_alloc(Engine &E, ReqHost &h, hipStream_t st = nullptr, bool pooled = fal
         nt = (size_t)1 << hlog2;
     }
     const size_t nr = h.reach ? 4096 : 0;
-    h.bytes = (nc + nh + nl + nd + nt + nr) * sizeof(u64);
+    const u32 slog2 = stab_log2(r);
+    const size_t ns = slog2 ? (size_t)1 << slog2 : 0;
+    h.bytes = (nc + nh + nl + nd + nt + nr + ns) * sizeof(u64);
     void *p = nullptr;
     h.pooled = pooled;
     if (pooled) {
@@ -386,6 +413,8 @@ int req_alloc(Engine &E, ReqHost &h, hipStream_t st = nullptr, bool pooled = fal
     d.reach = h.reach ? 1u : 0u;
     d.reach_cap = h.reach_cap;
     d.nolen = nr ? d.dhist + nd + nt : nullptr;
+    d.stab = ns ? d.dhist + nd + nt + nr : nullptr;
+    d.slog2 = slog2;
     d.cap = r.step_cap;
     E.slot_cap[h.slot] = r.step_cap;
     E.slot_n[h.slot] = (u32)r.n;
@@ -715,7 +744,10 @@ void record_launch(Engine &E, const GraphKey &key, hipStream_t st, Pipe *pipe, J
             WalkArgs<W> WA{};
             WA.pipe = pipe; WA.hdr = hdr; WA.reqs = reqs; WA.cq = cyc_queue<W>(E); WA.tile_pos = 64u * (u32)key.per;
             WA.stack = E.reach_stack;
-            hipLaunchKernelGGL((k_walk<W, RULE>), dim3(blocks), dim3(256), 0, st, WA);
+            for (u32 ph = 0; ph < 2; ph++) {     // the cycle-state tables' inserts, then the walks
+                WA.phase = ph;
+                hipLaunchKernelGGL((k_walk<W, RULE>), dim3(blocks), dim3(256), 0, st, WA);
+            }
         }
     }
 }
@@ -1021,7 +1053,10 @@ int run_plain_batch(Engine &E, ReqHost &h, u64 lo, u64 hi, const war_tuning &t, 
         WalkArgs<W> WA{};
         WA.pipe = E.pipe; WA.hdr = E.hdr; WA.reqs = E.reqs; WA.cq = cyc_queue<W>(E); WA.tile_pos = PLAIN_CYC_TILE;
         WA.stack = E.reach_stack;
-        hipLaunchKernelGGL((k_walk<W, RULE>), dim3(std::max(1, E.cus * 8)), dim3(256), 0, E.st, WA);
+        for (u32 ph = 0; ph < 2; ph++) {         // the cycle-state tables' inserts, then the walks
+            WA.phase = ph;
+            hipLaunchKernelGGL((k_walk<W, RULE>), dim3(std::max(1, E.cus * 8)), dim3(256), 0, E.st, WA);
+        }
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipEventRecord(E.ev1, E.st));
@@ -1404,8 +1439,10 @@ void serve_loop(Engine *Ep) {
             }
             size_t fr = 0, tot = 0;
             (void)hipMemGetInfo(&fr, &tot);
+            const u32 sl = stab_log2(h->rq);
             const size_t need = (size_t)(C_N + h->rq.hbins + h->rq.lbins + h->rq.dbins) * 8 +
-                                (h->rq.distinct ? ((size_t)8 << 27) : 0) + ((size_t)1 << 30);
+                                (h->rq.distinct ? ((size_t)8 << 27) : 0) + (sl ? (size_t)8 << sl : 0) +
+                                ((size_t)1 << 30);
             if (fr < need && !active.empty()) break;   // wait for memory to free up
             {
                 std::lock_guard<std::mutex> lk(E.smu);

@@ -27,7 +27,7 @@ namespace war {
 
 enum {
     C_CYC = 0, C_DONE, C_MAXFIN, C_TURNS, C_ONCYCLE, C_MINKEY, C_NONSTD, C_UNCHK, C_UNDET,
-    C_HASH, C_HOVF, C_QOVF, C_RYES, C_RNO, C_RUND, C_RDEPTH, C_N = 16
+    C_HASH, C_HOVF, C_QOVF, C_RYES, C_RNO, C_RUND, C_RDEPTH, C_SOVF, C_N = 17
 };
 
 constexpr u32 AGE_CAP = 1u << 24;      // safety stop of the last stage (never reached by a valid game)
@@ -41,6 +41,8 @@ struct ReqDev {
     u64 *cnt, *hist, *lenhist, *dhist, *htab;
     u64 reach_cap;      // reach: positions a cycle's backward search may visit
     u64 *nolen;         // reach: cycle lengths of the cycles no deal enters (4096 bins)
+    u64 *stab;          // cycle walks: the cycle-state table (2^slog2 slots), or null
+    u32 slog2, pad2;
 };
 
 struct Job { u64 lo, hi, tile_base; u32 req, pad; };
@@ -55,9 +57,10 @@ struct Pipe {
     u64 grab[MAXSTAGE];     // grab counter of stage s
     u64 stotal[MAXSTAGE];   // records written by stage s (sizes the next stage's grabs)
     u64 nreg[MAXSTAGE];     // output regions allocated by refill stage s
-    u64 wgrab, ctotal;      // walk grab counter; cycle records written
+    u64 wgrab, ctotal;      // walk grab counter (second pass); cycle records written
     u64 ovf;                // output regions refused for lack of room (an error; never with the sizing)
-    u64 pad[5];
+    u64 wgrab0;             // walk grab counter, first pass (the cycle-state table's inserts)
+    u64 pad[4];
 };
 
 // Survivor records live in per-tile regions of tile_pos slots: tile t's records are
@@ -1727,6 +1730,7 @@ template <int W> struct WalkArgs {
     u32 tile_pos;
     u32 zero;      // always 0 (see Zh)
     u64 *stack;    // reach: REACH_DEPTH 16-byte entries per thread of the grid
+    u32 phase;     // 0: insert the cycles not yet in the requests' cycle-state tables; 1: the walks
 };
 
 // ------------------------------------------------------------------------------------------
@@ -1805,6 +1809,59 @@ __device__ __forceinline__ u32 hash_insert(const ReqDev &R, u64 K) {
     return 0u;
 }
 
+// The cycle-state table of a canonical request with cycle walks (4-bit piles, n <= 14): every state of
+// every cycle the request's positions met, as its key4 (exact for n <= 15; bits 0..3 are zero for
+// n <= 14) with the cycle's shape verdict in bit 0 (STAB_NONSTD); 0 marks an empty slot. The walks'
+// first pass walks each cycle not yet in the table once per record that finds it missing and inserts
+// every state; the second pass, after the whole first pass, reads a record's shape verdict from its
+// meeting state's entry and decides whether its start lies on its cycle by looking the start up (a
+// state in the table lies on a cycle, and the start's cycle, if it is on one, is the meeting state's,
+// which the first pass inserted whole). An insert that finds no free slot within STAB_PROBES counts
+// C_SOVF, after which the second pass walks every record as without a table.
+constexpr u64 STAB_NONSTD = 1ull, STAB_FLAGS = 15ull;
+constexpr u32 STAB_PROBES = 256;
+__device__ __forceinline__ u64 stab_slot(u64 K, u32 lg) { return (K * 0x9E3779B97F4A7C15ull) >> (64 - lg); }
+// insert entry E (key | flags); false if no free slot was found
+__device__ __forceinline__ bool stab_insert(const ReqDev &R, u64 E) {
+    const u64 mask = (1ull << R.slog2) - 1ull, K = E & ~STAB_FLAGS;
+    u64 h = stab_slot(K, R.slog2);
+    for (u32 probe = 0; probe < STAB_PROBES; probe++) {
+        const u64 prev = atomicCAS((unsigned long long *)&R.stab[h], 0ull, (unsigned long long)E);
+        if (prev == 0ull || (prev & ~STAB_FLAGS) == K) return true;
+        h = (h + 1ull) & mask;
+    }
+    return false;
+}
+// the entry of key K, or 0. Plain loads: the second pass runs after the first pass's kernel has
+// completed, whose inserts it then sees; in the first pass a load that misses an insert made
+// meanwhile elsewhere only makes the record walk and insert its cycle again.
+__device__ __forceinline__ u64 stab_find(const ReqDev &R, u64 K) {
+    const u64 mask = (1ull << R.slog2) - 1ull;
+    u64 h = stab_slot(K, R.slog2);
+    for (u32 probe = 0; probe < STAB_PROBES; probe++) {
+        const u64 e = R.stab[h];
+        if (e == 0ull) return 0ull;
+        if ((e & ~STAB_FLAGS) == K) return e;
+        h = (h + 1ull) & mask;
+    }
+    return 0ull;
+}
+__device__ __forceinline__ u64 key_of4(const St<4> &v) { return key4(v.a, v.b, bits1<4>(v)); }
+
+// Add the number of lanes where pred holds to counter ctr of each such lane's request, one atomic per
+// request among them (a wave's records may belong to several requests); every lane calls this. Per-
+// lane atomics on one counter serialise: the n = 14 canonical shape test counts 4.4e7 positions.
+__device__ __forceinline__ void wave_count(bool pred, u32 req, const ReqDev &R, u32 ctr) {
+    u64 m = ballot(pred);
+    while (m) {
+        const u32 l = (u32)__ffsll((long long)m) - 1u;
+        const u32 r = lanev(req, l);
+        const u64 mr = ballot(pred && req == r);
+        m &= ~mr;
+        if (lane_id() == l) atomicAdd((unsigned long long *)&R.cnt[ctr], (unsigned long long)__popcll(mr));
+    }
+}
+
 template <int W, int RULE>
 __global__ __launch_bounds__(256) void k_walk(WalkArgs<W> A) {
     const u32 lane = lane_id();
@@ -1814,8 +1871,9 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs<W> A) {
     if (recs == 0) return;
     const u32 G = group_size(recs, T_, A.tile_pos);
     const u32 TP = A.tile_pos;
+    const bool ins = A.phase == 0u;     // the first pass: the cycle-state tables' inserts
     for (;;) {
-      const u64 g0 = wave_fetch_add(&A.pipe->wgrab, 1) * G;
+      const u64 g0 = wave_fetch_add(ins ? &A.pipe->wgrab0 : &A.pipe->wgrab, 1) * G;
       if (g0 >= T_) break;
       const u32 ng = (u32)((T_ - g0) < G ? (T_ - g0) : G);
       const u32 cc = lane < ng ? A.cq.cnt[g0 + lane] : 0u;
@@ -1845,10 +1903,68 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs<W> A) {
         u32 blk = 0, mode = 0;
         bool dist = false;
         if (act) { R = A.reqs[req]; blk = R.shape_blk; mode = R.mode; dist = (R.dbins || R.distinct) && mode == 2; }
+        // the record's request keeps a cycle-state table (and it has not overflowed: read after the
+        // first pass, in the second)
+        bool tab = false;
+        if constexpr (W == 4) tab = act && R.stab != nullptr && lam <= 512u && (ins || R.cnt[C_SOVF] == 0ull);
+
+        if (ins) {
+            // first pass: a record whose meeting state is not in the table walks its cycle for the
+            // shape verdict (if asked), then walks it again inserting every state with the verdict
+            if constexpr (W == 4) {
+                const bool miss = tab && stab_find(R, key_of4(m)) == 0ull;
+                if (ballot(miss)) {
+                    const u32 L = miss ? lam : 0u;
+                    u32 maxL = L;
+                    for (int o = 32; o > 0; o >>= 1) { const u32 x = __shfl_xor(maxL, o); maxL = x > maxL ? x : maxL; }
+                    u64 fl = 0;
+                    if (ballot(miss && blk != 0)) {
+                        u32 first = 0, cur = 0, run = 0, first_run = 0;
+                        bool changed = false, bad = false;
+                        St<W> v = m;
+                        open_sa<W>(v, A.zero);
+                        for (u32 t = 0; t < maxL; t++) {
+                            const u32 w = turn<W, RULE>(v, C, z) ? 1u : 0u;
+                            if (t < L) {
+                                if (t == 0) { first = w; cur = w; run = 1; }
+                                else if (w == cur) run++;
+                                else {
+                                    if (!changed) { first_run = run; changed = true; }
+                                    else if (run != blk) bad = true;
+                                    cur = w; run = 1;
+                                }
+                            }
+                        }
+                        u32 shp;
+                        if (!changed || bad) shp = 1;
+                        else if (cur == first) shp = (run + first_run == blk) ? 0u : 1u;
+                        else shp = (first_run == blk && run == blk) ? 0u : 1u;
+                        if (miss && blk != 0 && shp) fl = STAB_NONSTD;
+                    }
+                    St<W> v = m;
+                    open_sa<W>(v, A.zero);
+                    bool full = false;
+                    for (u32 t = 0; t < maxL; t++) {
+                        if (t < L) {
+                            St<W> u = v;
+                            u.sa &= ~3u;
+                            full |= !stab_insert(R, key_of4(u) | fl);
+                            turn<W, RULE>(v, C, z);
+                        }
+                    }
+                    if (full) atomicAdd((unsigned long long *)&R.cnt[C_SOVF], 1ull);
+                }
+            }
+            continue;
+        }
+        u64 ment = 0;                  // the meeting state's entry: its cycle is in the table whole
+        if constexpr (W == 4) if (tab) ment = stab_find(R, key_of4(m));
+        const bool known = ment != 0ull;
+        wave_count(known && blk != 0 && (ment & STAB_NONSTD) != 0ull, req, R, C_NONSTD);
 
         // (1) loop shape from the meeting state (gpu/war_gpu.py:51-73), online over the winners
-        if (ballot(act && blk != 0)) {
-            const bool doit = act && blk != 0 && lam <= 512u;
+        if (ballot(act && blk != 0 && !known)) {
+            const bool doit = act && blk != 0 && !known && lam <= 512u;
             u32 first = 0, cur = 0, run = 0, first_run = 0;
             bool changed = false, bad = false;
             u32 L = doit ? lam : 0u;
@@ -1868,16 +1984,15 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs<W> A) {
                     }
                 }
             }
+            u32 shp = 0;
             if (doit) {
-                u32 shp;
                 if (!changed) shp = 1;                         // one player wins every turn
                 else if (bad) shp = 1;
                 else if (cur == first) shp = (run + first_run == blk) ? 0u : 1u;
                 else shp = (first_run == blk && run == blk) ? 0u : 1u;
-                if (shp) atomicAdd((unsigned long long *)&R.cnt[C_NONSTD], 1ull);
-            } else if (act && blk != 0) {
-                atomicAdd((unsigned long long *)&R.cnt[C_UNCHK], 1ull);
             }
+            wave_count(doit && shp != 0u, req, R, C_NONSTD);
+            wave_count(act && blk != 0 && !known && !doit, req, R, C_UNCHK);
         }
 
         // (2) distinct: does the start lie on its cycle? (lam turns from the start return to it)
@@ -1891,13 +2006,17 @@ __global__ __launch_bounds__(256) void k_walk(WalkArgs<W> A) {
                     const int kk = (int)(pos - r * mm);
                     s0 = canon<W>(unrank<W>(r, (int)mm), (int)n, kk);
                 }
-                const u32 L = dist ? lam : 0u;
+                // with the cycle in the table the start lies on its cycle iff the table holds it;
+                // otherwise lam turns from the start come back to it
+                const bool look = dist && known;
+                const bool onk = look && stab_find(R, key_of4(s0)) != 0ull;
+                const u32 L = (dist && !known) ? lam : 0u;
                 u32 maxL = L;
                 for (int off = 32; off > 0; off >>= 1) { const u32 o = __shfl_xor(maxL, off); maxL = o > maxL ? o : maxL; }
                 open_sa<W>(s0, A.zero);
                 St<W> v = s0;
                 for (u32 t = 0; t < maxL; t++) if (t < L) turn<W, RULE>(v, C, z);
-                const bool onc = dist && same<W>(v, s0);
+                const bool onc = look ? onk : (dist && same<W>(v, s0));
                 if (ballot(onc)) {
                     // (3) weight, minimum key and hash insert for positions on their cycle
                     u64 c1n = 0, c2n = 0, K = ~0ull;

@@ -16,6 +16,8 @@ KERNEL_PLAIN, KERNEL_PERSISTENT = 0, 1
 KERNELS = {"plain": KERNEL_PLAIN, "persistent": KERNEL_PERSISTENT}
 DET_BRENT, DET_STREAK, DET_ALT, DET_PAIR = 0, 1, 2, 3
 FLAG_UNDET_ERROR = 1
+FLAG_CYCLE_TABLE = 2
+FLAG_NO_CYCLE_TABLE = 4
 STREAK_CAP, SAFETY_CAP = 1 << 22, 1 << 24
 DETECTORS = {"brent": DET_BRENT, "streak": DET_STREAK, "alt": DET_ALT, "pair": DET_PAIR}
 

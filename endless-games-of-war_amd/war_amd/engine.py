@@ -47,16 +47,19 @@ def _tuning_dict(t):
 
 
 def make_request(n, mode="full", lo=0, hi=None, rule=0, kernel="persistent", hbins=0, lbins=0, dbins=0,
-                 distinct=False, shape_blk=0, tuning=None, detector="brent", step_cap=0, undet_error=False):
+                 distinct=False, shape_blk=0, tuning=None, detector="brent", step_cap=0, undet_error=False,
+                 cycle_table=None):
     """step_cap: turns after which an unresolved game counts as undetermined (0 = the default);
-    undet_error: fail the request instead if any game is undetermined."""
+    undet_error: fail the request instead if any game is undetermined; cycle_table: True / False
+    forces the cycle walks' state table on or off (None: the engine's default)."""
     mode = MODES.get(mode, mode)
     if hi is None:
         hi = total_positions(n, mode)
     return WarRequest(int(n), int(mode), int(rule), int(KERNELS.get(kernel, kernel)), int(lo), int(hi),
                       int(hbins), int(lbins), int(dbins), int(bool(distinct)), int(shape_blk),
                       int(DETECTORS.get(detector, detector)), _tuning(tuning), int(step_cap),
-                      _lib.FLAG_UNDET_ERROR if undet_error else 0)
+                      (_lib.FLAG_UNDET_ERROR if undet_error else 0) |
+                      (0 if cycle_table is None else _lib.FLAG_CYCLE_TABLE if cycle_table else _lib.FLAG_NO_CYCLE_TABLE))
 
 
 class _ResultBuf:

@@ -80,6 +80,12 @@ typedef struct war_tuning {
 /* war_request.flags */
 #define WAR_FLAG_UNDET_ERROR 1u   /* fail the request (status 21) if any game is undetermined, instead
                                      of only counting it in war_result.undetermined               */
+#define WAR_FLAG_CYCLE_TABLE 2u   /* CANON, rule 0, n <= 14, with cycle walks (distinct / dbins / shape):
+                                     keep the cycle-state table whatever the range (by default only
+                                     ranges of at least 2^28 positions keep it). The table holds every
+                                     state of the cycles met, so the walks look a record up instead of
+                                     walking its cycle; the outputs are the same                   */
+#define WAR_FLAG_NO_CYCLE_TABLE 4u  /* walk every cycle record (no cycle-state table)              */
 
 /* war_request.step_cap = 0: the default cap. The streak test stops a game after 2^22 turns, as the
  * reference's CAP (gpu/war24.py:57,103); Brent and the checkpoint detectors are exact and end every
