@@ -344,6 +344,8 @@ void pool_release(Engine &E) {
     E.pool.clear();
 }
 
+u32 first_window(int n);
+
 // The cycle-state table of a canonical rule-0 request with cycle walks, as log2 of its slots (0: none):
 // at least twice the states on the cycles of n, from the reference's published distinct cycles and
 // their lengths (data/checks.txt, data/long_runs.txt: n = 5: 2 of 6; 7: 52 of 8; 9: 144 of 20 and 192
@@ -403,7 +405,10 @@ int req_alloc(Engine &E, ReqHost &h, hipStream_t st = nullptr, bool pooled = fal
     d.n = (u32)r.n; d.mode = (u32)r.mode; d.rule = (u32)r.rule;
     d.hbins = (u32)r.hbins; d.lbins = (u32)r.lbins; d.dbins = (u32)r.dbins;
     d.distinct = (u32)r.distinct; d.shape_blk = (u32)r.shape_blk;
-    d.walk = (r.mode == WAR_MODE_CANON && (r.distinct || r.dbins || r.shape_blk)) ? 1u : 0u;
+    // cycle walks, except where no position cycles (rule 0, n = 12 and 16: first_window 0): the walk
+    // variants and their smaller batches would only add time, every walk output being zero there
+    const bool nocyc = r.rule == 0 && r.n >= 5 && r.n <= 16 && first_window(r.n) == 0u;
+    d.walk = (r.mode == WAR_MODE_CANON && (r.distinct || r.dbins || r.shape_blk) && !nocyc) ? 1u : 0u;
     d.hlog2 = hlog2;
     d.cnt = h.dmem;
     d.hist = d.cnt + nc;
