@@ -2132,6 +2132,7 @@ int war_autotune_batch(const war_request *reqs, int32_t count, double budget_ms,
     // leave half a wave without a position), waves per CU (1..8 as the occupancy sweep, and the
     // larger counts the flow kernel prefers), the first stage boundary (staged schedules only: the
     // flow kernel has no stages) and, for n = 17, the pile layout (5-bit or card 16 held apart)
+    constexpr double AUTOTUNE_MARGIN = 0.02;
     const int kinds[] = {3, 1, 2};
     const int tiles[] = {64, 128, 256, 512, 1024, 2048, 4096, 8192};
     const int wpcs[] = {1, 2, 3, 4, 5, 6, 7, 8, 12, 16, 20, 24, 28, 32};
@@ -2165,7 +2166,10 @@ int war_autotune_batch(const war_request *reqs, int32_t count, double budget_ms,
                 if (E.verbose)
                     fprintf(stderr, "autotune n=%d mode=%d: kind=%d tile=%d wpc=%d cap=%d layout=%d: %.3f ms\n", reqs[0].n,
                             reqs[0].mode, t.stage_kind, t.ranks_per_refill, t.waves_per_cu, t.brent_cap, t.pile_layout, ms);
-                if (ms < best_ms) { best_ms = ms; bestt = t; }
+                // a schedule replaces the best so far only if at least AUTOTUNE_MARGIN faster on the
+                // sample: one timing per candidate, so noise alone does not move the choice away from
+                // the default the descent starts from
+                if (ms < best_ms * (1.0 - AUTOTUNE_MARGIN)) { best_ms = ms; bestt = t; }
             }
             cur = bestt;
         }
