@@ -322,7 +322,10 @@ def main():
         npos = sum(hi - lo for lo, hi in ch)
         canon.update({"sample": "8 chunks of 13*2^22 positions (whole q-ranks), seed 14",
                       "plain_sample_value": npos / (tp / 1e3), "persistent_sample_value": npos / (tq / 1e3),
-                      "persistent_over_plain": tp / tq, "tuning": eng.tuning_for(14, "canon")})
+                      "persistent_over_plain": tp / tq, "tuning": eng.tuning_for(14, "canon"),
+                      "tiles": "whole scan: lane runs of six whole orders (78 positions, tiles of 4992), the "
+                               "engine's choice for single canonical ranges of at least 2^31 positions; the "
+                               "sampled chunks: two orders (26 positions, tiles of 1664)"})
         out["canon14"] = canon
         log("canon14: %.3e pos/s persistent (whole), ratio %.2f on samples" % (tot14 / cw, tp / tq))
 
