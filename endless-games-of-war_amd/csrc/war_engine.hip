@@ -277,10 +277,13 @@ war_tuning default_tuning(int n, int mode, int det) {
 // serving loop's steps and batches keep two, so each wave still gets many tiles. Measured against
 // two: n = 14 whole 1453 -> 1423 ms, n = 16 (2^35 positions) 599 -> 585 ms, n = 13 239 -> 237 ms,
 // n = 15 54.6 -> 54.3 s; five and seven orders were slower than six at n = 14; on ranges of 5.5e7
-// positions (n = 14), six orders were 19 % slower than two.
+// positions (n = 14), six orders were 19 % slower than two, and on the 2^28-position batches of
+// cycle walks (n = 14 with distinct + shape) 5 % slower.
 constexpr u64 BIG_CANON_RANGE = 1ull << 31;
+bool walks_of(const war_request &r);
 void big_canon_tile(const Engine &E, const war_request &r, war_tuning &t) {
     if (r.mode != WAR_MODE_CANON || r.n < 3 || r.hi - r.lo < BIG_CANON_RANGE) return;
+    if (walks_of(r)) return;         // walk batches hold 2^28 positions: two orders keep their tiles many
     if (r.tuning.ranks_per_refill > 0 || E.tun.ranks_per_refill > 0) return;
     if (E.tuned.count(std::make_tuple(r.n, r.mode, r.rule, r.detector))) return;
     t.ranks_per_refill = (int)std::min<u64>(64ull * 6u * (u64)(r.n - 1), MAX_TILE);
@@ -361,6 +364,14 @@ void pool_release(Engine &E) {
 
 u32 first_window(int n);
 
+// Cycle walks (distinct / dbins / shape on canonical positions), except where no position cycles
+// (rule 0, n = 12 and 16: first_window 0): the walk variants and their smaller batches would only add
+// time, every walk output being zero there.
+bool walks_of(const war_request &r) {
+    const bool nocyc = r.rule == 0 && r.n >= 5 && r.n <= 16 && first_window(r.n) == 0u;
+    return r.mode == WAR_MODE_CANON && (r.distinct || r.dbins || r.shape_blk) && !nocyc;
+}
+
 // The cycle-state table of a canonical rule-0 request with cycle walks, as log2 of its slots (0: none):
 // at least twice the states on the cycles of n, from the reference's published distinct cycles and
 // their lengths (data/checks.txt, data/long_runs.txt: n = 5: 2 of 6; 7: 52 of 8; 9: 144 of 20 and 192
@@ -420,10 +431,7 @@ int req_alloc(Engine &E, ReqHost &h, hipStream_t st = nullptr, bool pooled = fal
     d.n = (u32)r.n; d.mode = (u32)r.mode; d.rule = (u32)r.rule;
     d.hbins = (u32)r.hbins; d.lbins = (u32)r.lbins; d.dbins = (u32)r.dbins;
     d.distinct = (u32)r.distinct; d.shape_blk = (u32)r.shape_blk;
-    // cycle walks, except where no position cycles (rule 0, n = 12 and 16: first_window 0): the walk
-    // variants and their smaller batches would only add time, every walk output being zero there
-    const bool nocyc = r.rule == 0 && r.n >= 5 && r.n <= 16 && first_window(r.n) == 0u;
-    d.walk = (r.mode == WAR_MODE_CANON && (r.distinct || r.dbins || r.shape_blk) && !nocyc) ? 1u : 0u;
+    d.walk = walks_of(r) ? 1u : 0u;
     d.hlog2 = hlog2;
     d.cnt = h.dmem;
     d.hist = d.cnt + nc;
