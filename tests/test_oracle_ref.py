@@ -1,5 +1,5 @@
 """The oracle against the reference's own CPU programs, compiled from their sources into
-oracle/_ref/ (make -C oracle ref). Skipped where they were not built (the GPU box)."""
+oracle/_ref/ (make -C oracle ref). Skipped where they were not built (no reference tree at build time)."""
 import os
 import re
 import subprocess
