@@ -153,6 +153,17 @@ __device__ __forceinline__ bool turn_held(St<6> &s, u32 C, Zh<6> &z) { return tu
 
 template <int W> __device__ __forceinline__ bool ended(const St<W> &s, u32 C) { return s.sa > C; }
 
+// Whether the state's player 1 is the position's player 1, for a position as dealt (before its first
+// turn); detectors that name a turn's winner by player need it (the alternation detector counts the
+// position's player 1 as the winner of the turn before the first). Always, except in layout 6, whose
+// player 1 is the holder of card n-1: that is the position's player 1 for canonical positions (mode 2;
+// player 1 leads with card n-1) and, for a deal, iff it holds the deal's (n+1)/2 cards, that is
+// (n-1)/2 besides card n-1 in its word (sa = 4 * (n-1)/2; the other player's would be one fewer).
+template <int W> __device__ __forceinline__ bool p1_is_dealt_p1(const St<W> &s, u32 mode) {
+    if constexpr (W == 6) return mode == 2u || s.sa == 4u * ((HELD_N - 1u) >> 1);
+    else { (void)s; (void)mode; return true; }
+}
+
 // d = C - x and the wave's mask of lanes where it borrows (x > C), in one instruction (bits of
 // inactive lanes are 0). UC: C is wave-uniform and sits in a scalar register.
 template <bool UC>

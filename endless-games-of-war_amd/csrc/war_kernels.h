@@ -616,7 +616,8 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs<W> A) {
                 if constexpr ((FLAGS & F_ALT) != 0) {
                     AltLane<W> al;
                     al.t = s; al.R = 0; al.L = ALT_L0;
-                    u64 prevm = ~0ull;
+                    // before the first turn, the position's player 1 counts as the last winner
+                    u64 prevm = ballot(p1_is_dealt_p1<W>(s, mode));
                     if (alive) alive = play_alt<W, RULE, FRESH, FLAGS>(A, acc, s, al, prevm, C, n, alive, true, L);
                     if (alive) {
                         put_survivors<W, FLAGS>(A, span, out_n, alive, s, n, L, &al, prevm);
@@ -1310,7 +1311,10 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
                 }
                 alive |= gm & ~e0;
                 newm |= gm & ~e0;
-                prevm |= gm;   // before the first turn, player 1 counts as the last winner
+                // before the first turn, the position's player 1 counts as the last winner (layout
+                // 6 names players by who holds card n-1)
+                if constexpr (W == 6) prevm = (prevm & ~gm) | (ballot(p1_is_dealt_p1<W>(s, mode)) & gm);
+                else prevm |= gm;
                 preveqm &= ~gm;   // ... and as having changed hands (pair-run detector)
             }
             if (alive || !more) return;

@@ -294,6 +294,18 @@ def test_alt_sampled_and_layouts(eng, orc):
         check_alt(eng, orc, 17, "full", 10**14, 10**14 + 300_000, tuning={"pile_layout": lay})
 
 
+@pytest.mark.parametrize("stage_kind", ["flow", "refill", "lockstep"])
+def test_alt_held_card_to_player2(eng, orc, stage_kind):
+    """n = 17 deals whose card 16 goes to player 2 (deck position 13 at the range's start): layout 6
+    names the players by who holds card 16, and the detector must still count the deal's player 1
+    as the winner of the turn before the first. With the holder counted instead, `turns` here is
+    22 862 471 over the whole range instead of 22 749 831 (every other output the same)."""
+    lo = 302_876_792_870_854
+    assert list(orc.unrank(lo, 17)).index(16) == 13
+    for lay in (6, 5):
+        check_alt(eng, orc, 17, "full", lo, lo + 196_845, tuning={"pile_layout": lay, "stage_kind": stage_kind})
+
+
 @pytest.mark.parametrize("n", [9, 11])
 def test_alt_walks(eng, orc, n):
     print("case alt walks", n, flush=True)
