@@ -138,3 +138,43 @@ def test_sweep_noalloc(eng, orc):
         print("noalloc", i, n, mode, lo, hi, kw, flush=True)
         got = eng.run_range_noalloc(n, mode, lo, hi, **kw)
         check(i, got, orc.run(n, mode, lo, hi, **okw), n, mode, lo, hi, kw, okw)
+
+
+def test_sweep_large_cross_kernel(eng):
+    """Ranges of 10^7.5 to 10^10 positions (10^9.3 at n = 17, 10^8.7 for the 5-bit n = 18..20), longer than the oracle plays
+    in a test: the persistent kernel at a random schedule, tile, waves per CU and layout against the
+    plain kernel (one game per lane, no pools or tortoise schedule) with the same detector (Brent's
+    counts for the pair-run detector, whose turns are its own); cycle walks on canonical n <= 14 (the
+    state table on by default from 2^28 positions). Every output equal."""
+    rng = np.random.Generator(np.random.PCG64(20261208))
+    for i in range(40):
+        n = int(rng.integers(13, 21))
+        modes = ["full", "canon"] + (["half"] if n % 2 == 0 else [])
+        mode = modes[int(rng.integers(len(modes)))]
+        rule = int(rng.random() < 0.3)
+        top = 10.0 if n <= 16 else (9.3 if n == 17 else 8.7)
+        length = min(total(n, mode), int(10 ** rng.uniform(7.5, top)))
+        lo = int(rng.integers(0, total(n, mode) - length + 1))
+        det = "brent"
+        if rule == 0 and n % 2 == 1 and rng.random() < 0.6:
+            det = "alt"
+        elif rule == 0 and n % 4 == 2 and rng.random() < 0.5:
+            det = "pair"
+        kw = {"rule": rule, "hbins": 256, "lbins": 512}
+        if mode == "canon" and n <= 14 and rng.random() < 0.4:
+            kw.update(distinct=True, dbins=64, shape_blk=n & -n)
+        tun = {"ranks_per_refill": 64 * int(rng.integers(1, 129)), "waves_per_cu": int(rng.choice([2, 4, 8, 16, 28])),
+               "brent_cap": int(rng.choice([15, 31, 63]))}
+        if det != "pair":
+            tun["stage_kind"] = str(rng.choice(["flow", "flow", "refill", "lockstep"]))
+        if n == 17 and not kw.get("distinct"):
+            tun["pile_layout"] = int(rng.choice([5, 6]))
+        print("large", i, n, mode, lo, lo + length, det, kw, tun, flush=True)
+        got = eng.run_range(n, mode, lo, lo + length, kernel="persistent", detector=det, tuning=tun, **kw)
+        ref = eng.run_range(n, mode, lo, lo + length, kernel="plain", detector="brent" if det == "pair" else det, **kw)
+        keys = ("cyc", "done", "maxfin", "undetermined", "hist", "lenhist") + (() if det == "pair" else ("turns",))
+        if mode == "canon":
+            keys += WALK_KEYS + ("distinct_hashset",)
+        for k in keys:
+            assert got[k] == ref[k], (i, n, mode, lo, lo + length, det, kw, tun, k, got[k], ref[k])
+        print("  %.1f ms persistent, %.1f ms plain" % (got["device_ms"], ref["device_ms"]), flush=True)
