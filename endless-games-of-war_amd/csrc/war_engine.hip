@@ -1446,6 +1446,7 @@ void finish_step(StepCtx &c) {
 
 void serve_loop(Engine *Ep) {
     Engine &E = *Ep;
+    (void)hipSetDevice(E.device);     // this thread's current device: the engine's
     std::vector<std::shared_ptr<ReqHost>> active;
     int cur = 0;          // the context of the next step
     int prev = -1;        // the context of the step in flight (-1: none)
@@ -1600,7 +1601,11 @@ void serve_loop(Engine *Ep) {
 
 // Every GPU entry point other than the serving worker waits for the serving loop's step in flight
 // (if any) before it uses the device, which it then has to itself while it holds the engine's mutex.
+// Before GPU work on the calling thread (the engine mutex held): make the engine's device current
+// (HIP's current device is per host thread, and a call may come from any thread), then wait for the
+// serving loop's steps in flight.
 void drain(Engine &E) {
+    (void)hipSetDevice(E.device);
     for (StepCtx &c : E.sc)
         if (c.nl) (void)hipStreamSynchronize(c.st);
 }
@@ -1678,6 +1683,7 @@ int war_shutdown(void) {
     Engine *E = g_eng;
     {
         std::lock_guard<std::mutex> g(E->mu);
+        (void)hipSetDevice(E->device);
         (void)hipStreamSynchronize(E->st);
         for (auto &kv : E->graphs) (void)hipGraphExecDestroy(kv.second);
         pool_release(*E);
@@ -1724,6 +1730,7 @@ int war_device_info(war_devinfo *out) {
     memset(out, 0, sizeof *out);
     out->cus = g_eng->cus; out->max_clock_mhz = g_eng->clock_mhz; out->wave_size = 64;
     size_t fr = 0, tot = 0;
+    HIPCHK(hipSetDevice(g_eng->device));      // the free memory of the engine's device
     HIPCHK(hipMemGetInfo(&fr, &tot));
     out->free_bytes = fr; out->total_bytes = tot;
     snprintf(out->arch, sizeof out->arch, "%s", g_eng->arch.c_str());
