@@ -59,7 +59,7 @@ def draw(rng, target="single"):
             kw["dbins"] = okw["dbins"] = 64
         if n <= 15 and (rng.random() < 0.6 or not (kw.get("distinct") or kw.get("dbins"))):
             kw["shape_blk"] = okw["shape_blk"] = n & -n
-        if rule == 0 and n <= 14 and target == "single":
+        if rule == 0 and n <= 14:
             kw["cycle_table"] = [None, True, False][int(rng.integers(3))]
     flow = True
     tun = {}
