@@ -1485,8 +1485,11 @@ __global__ __launch_bounds__(256) void k_flow(StageArgs<W> A) {
                     const u64 js = (ex | pat) & alive;
                     streakm = 0;
                     if (js) {
-                        if (in_mask(js)) {
-                            T_.t = s; T_.R = now;
+                        if (in_mask(js)) { T_.t = s; T_.R = now; }
+                        // the timed jumps double their wait; they are rare (waits of ALT_L0 turns
+                        // and longer, against pattern jumps at most checkpoints), so behind a
+                        // branch the common checkpoint skips the wait's select and shift
+                        if (__builtin_expect(ex != 0ull, 0)) {
                             if (in_mask(ex)) T_.L <<= 1;
                         }
                     }
